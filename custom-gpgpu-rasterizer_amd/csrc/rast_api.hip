@@ -1,0 +1,855 @@
+// rast_api.hip -- host side of librast.so: the C ABI of include/rast.h.
+//
+// Owns the device objects (renderer, meshes, textures), computes the frame
+// constants with the reference's host camera math, and enqueues one launch
+// per pipeline stage on the renderer's stream.  Every HIP status is checked
+// and mapped to a RAST_E_* code; nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rast.h"
+#include "rast_launch.h"
+#include "rast_model.h"
+#include "rast_state.h"
+
+using namespace rast;
+
+// ----------------------------------------------------------------- errors
+
+static thread_local std::string g_last_error;
+
+int rast_set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+extern "C" int rast_last_error(char *buf, size_t n)
+{
+    const size_t len = g_last_error.size();
+    if (buf && n) {
+        const size_t c = len < n - 1 ? len : n - 1;
+        memcpy(buf, g_last_error.data(), c);
+        buf[c] = 0;
+    }
+    return (int)len;
+}
+
+// ------------------------------------------------------------ host math
+
+// Matrices use the reference storage data[col][row] (CCR/Matrix.h:9-44).
+static inline float &M4(float *m, int r, int c) { return m[c * 4 + r]; }
+static inline float M4c(const float *m, int r, int c) { return m[c * 4 + r]; }
+
+// Matrix4 * Matrix4 (CCR/Matrix4.h:120-148), sums left to right.
+static void mat4_mul(const float *lm, const float *rm, float *out)
+{
+    float t[16];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c)
+            M4(t, r, c) = M4c(lm, r, 0) * M4c(rm, 0, c) + M4c(lm, r, 1) * M4c(rm, 1, c) +
+                          M4c(lm, r, 2) * M4c(rm, 2, c) + M4c(lm, r, 3) * M4c(rm, 3, c);
+    memcpy(out, t, sizeof t);
+}
+
+struct V3f { float x, y, z; };
+static V3f cross3(V3f a, V3f b) { return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
+static float dot3(V3f a, V3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static V3f normalized(V3f v)
+{  // CCR/Vector.h:41-63
+    const float m = std::sqrt(dot3(v, v));
+    if (m == 0.f) return { 0.f, 0.f, 0.f };
+    const float inv = 1.f / m;
+    return { v.x * inv, v.y * inv, v.z * inv };
+}
+
+// Inverse (CCR/Matrix4.h:327-363), FGED1 formulation in the same order.
+static void mat4_inverse(const float *m, float *out)
+{
+    const V3f a{ m[0], m[1], m[2] }, b{ m[4], m[5], m[6] }, c{ m[8], m[9], m[10] }, d{ m[12], m[13], m[14] };
+    const float x = M4c(m, 3, 0), y = M4c(m, 3, 1), z = M4c(m, 3, 2), w = M4c(m, 3, 3);
+    V3f s = cross3(a, b), t = cross3(c, d);
+    V3f u{ a.x * y - b.x * x, a.y * y - b.y * x, a.z * y - b.z * x };
+    V3f v{ c.x * w - d.x * z, c.y * w - d.y * z, c.z * w - d.z * z };
+    const float det = dot3(s, v) + dot3(t, u);
+    const float invDet = 1.0f / det;
+    s = { s.x * invDet, s.y * invDet, s.z * invDet };
+    t = { t.x * invDet, t.y * invDet, t.z * invDet };
+    u = { u.x * invDet, u.y * invDet, u.z * invDet };
+    v = { v.x * invDet, v.y * invDet, v.z * invDet };
+    const V3f bv = cross3(b, v), va = cross3(v, a), du = cross3(d, u), uc = cross3(u, c);
+    const V3f r0{ bv.x + t.x * y, bv.y + t.y * y, bv.z + t.z * y };
+    const V3f r1{ va.x - t.x * x, va.y - t.y * x, va.z - t.z * x };
+    const V3f r2{ du.x + s.x * w, du.y + s.y * w, du.z + s.z * w };
+    const V3f r3{ uc.x - s.x * z, uc.y - s.y * z, uc.z - s.z * z };
+    const float rows[4][4] = { { r0.x, r0.y, r0.z, -dot3(b, t) },
+                               { r1.x, r1.y, r1.z, dot3(a, t) },
+                               { r2.x, r2.y, r2.z, -dot3(d, s) },
+                               { r3.x, r3.y, r3.z, dot3(c, s) } };
+    for (int r = 0; r < 4; ++r)
+        for (int cc = 0; cc < 4; ++cc) M4(out, r, cc) = rows[r][cc];
+}
+
+// Camera (CCR/Camera.cpp:6-31,97-117): look-at = {right, up, forward, pos};
+// view = Inverse(look-at); projection keeps Identity()'s [3][3] = 1.
+static void camera_frame(const rast_camera &cam, uint32_t W, uint32_t H, rast_frame_consts &fr)
+{
+    const V3f worldUp{ 0.f, 1.f, 0.f };
+    const V3f fwd = normalized({ cam.forward[0], cam.forward[1], cam.forward[2] });
+    const V3f right = normalized(cross3(worldUp, fwd));
+    const V3f up = normalized(cross3(fwd, right));
+    float lookAt[16] = { right.x, right.y, right.z, 0.f, up.x, up.y, up.z, 0.f,
+                         fwd.x, fwd.y, fwd.z, 0.f, cam.position[0], cam.position[1], cam.position[2], 1.f };
+    const float pi_f = 3.14159265358979323846f;
+    const float radians = cam.fov_deg * (pi_f / 180.f);
+    const float fov = std::tan(radians / 2.f);
+    const float aspect = (float)W / (float)H;
+    const float n = cam.near_z, f = cam.far_z;
+    const float A = f / (n - f);
+    const float B = (f * n) / (n - f);
+    float *P = fr.proj;
+    for (int i = 0; i < 16; ++i) P[i] = (i % 5 == 0) ? 1.f : 0.f;
+    P[0] = 1.f / (aspect * fov);
+    P[5] = 1.f / fov;
+    P[10] = A;
+    P[11] = -1.f;  // data[2][3]
+    P[14] = B;     // data[3][2]
+    mat4_inverse(lookAt, fr.view);
+    mat4_mul(fr.proj, fr.view, fr.viewproj);
+    const float cp[3] = { cam.position[0], cam.position[1], cam.position[2] };
+    const float cf[3] = { fwd.x, fwd.y, fwd.z }, cr[3] = { right.x, right.y, right.z }, cu[3] = { up.x, up.y, up.z };
+    memcpy(fr.cam_pos, cp, sizeof cp);
+    memcpy(fr.cam_fwd, cf, sizeof cf);
+    memcpy(fr.cam_right, cr, sizeof cr);
+    memcpy(fr.cam_up, cu, sizeof cu);
+}
+
+static hipError_t grow(rast_renderer *r, DBuf &b, size_t bytes, bool *synced)
+{
+    if (b.p && b.n >= bytes) return hipSuccess;
+    if (!*synced) {
+        hipError_t e = hipStreamSynchronize(r->stream);
+        if (e != hipSuccess) return e;
+        *synced = true;
+    }
+    dfree(b);
+    size_t nb = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&b.p, nb);
+    if (e != hipSuccess) { b.p = nullptr; return e; }
+    b.n = nb;
+    return hipSuccess;
+}
+
+static uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+static uint32_t mesh_triangles(int topology, uint32_t ni)
+{  // CCR/CUDAMesh.cpp:27-40
+    return topology == RAST_TOPOLOGY_LIST ? ni / 3 : (ni >= 3 ? ni - 2 : 0);
+}
+
+extern "C" int rast_device_query(int device, rast_device_info *out)
+{
+    if (!out) return rast_set_error(RAST_E_INVALID_ARG, "rast_device_query: null output");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return rast_set_error(RAST_E_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= n) return rast_set_error(RAST_E_INVALID_ARG, "device %d out of range", device);
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, device));
+    memset(out, 0, sizeof *out);
+    snprintf(out->name, sizeof out->name, "%s", p.name);
+    snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+    out->compute_units = p.multiProcessorCount;
+    out->clock_khz = p.clockRate;
+    out->global_mem_bytes = p.totalGlobalMem;
+    out->lds_bytes_per_block = (int)p.sharedMemPerBlock;
+    out->wavefront_size = p.warpSize;
+    return RAST_OK;
+}
+
+extern "C" int rast_camera_frame(const rast_camera *cam, uint32_t width, uint32_t height, rast_frame_consts *out)
+{
+    if (!cam || !out || !width || !height) return rast_set_error(RAST_E_INVALID_ARG, "rast_camera_frame: bad arguments");
+    camera_frame(*cam, width, height, *out);
+    return RAST_OK;
+}
+
+static int alloc_framebuffers(rast_renderer *r)
+{
+    bool synced = false;
+    const size_t P = (size_t)r->W * r->H;
+    HIP_TRY(grow(r, r->vis, P * 4, &synced));
+    HIP_TRY(grow(r, r->depth, P * 4, &synced));
+    HIP_TRY(grow(r, r->rgba, P * 4, &synced));
+    r->frame_valid = false;
+    return RAST_OK;
+}
+
+extern "C" int rast_renderer_create(int device, uint32_t width, uint32_t height, rast_renderer **out)
+{
+    if (!out) return rast_set_error(RAST_E_INVALID_ARG, "rast_renderer_create: null output");
+    *out = nullptr;
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return rast_set_error(RAST_E_INVALID_ARG, "framebuffer %ux%u out of range (1..16384)", width, height);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return rast_set_error(RAST_E_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= n) return rast_set_error(RAST_E_INVALID_ARG, "device %d out of range", device);
+    HIP_TRY(hipSetDevice(device));
+    rast_renderer *r = new (std::nothrow) rast_renderer();
+    if (!r) return rast_set_error(RAST_E_OUT_OF_MEMORY, "host allocation failed");
+    r->device = device;
+    r->W = width;
+    r->H = height;
+    int rc = RAST_OK;
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) rc = RAST_E_HIP;
+    if (rc == RAST_OK && hipEventCreateWithFlags(&r->staging_done, hipEventDisableTiming) != hipSuccess) rc = RAST_E_HIP;
+    for (int i = 0; rc == RAST_OK && i < 12; ++i)
+        if (hipEventCreate(&r->ev[i]) != hipSuccess) rc = RAST_E_HIP;
+    if (rc == RAST_OK && hipHostMalloc((void **)&r->h_counters, sizeof(FrameCounters)) != hipSuccess) rc = RAST_E_HIP;
+    if (rc == RAST_OK && hipHostMalloc((void **)&r->h_defaults, 16) != hipSuccess) rc = RAST_E_HIP;
+    if (rc == RAST_OK) rc = alloc_framebuffers(r);
+    if (rc != RAST_OK) {
+        rast_renderer_destroy(r);
+        return rast_set_error(rc, "rast_renderer_create: HIP resource allocation failed");
+    }
+    memset(r->h_counters, 0, sizeof(FrameCounters));
+    *out = r;
+    return RAST_OK;
+}
+
+static void free_mesh(rast_mesh *m)
+{
+    if (m->vert16) (void)hipFree(m->vert16);
+    if (m->idx) (void)hipFree(m->idx);
+    delete m;
+}
+
+extern "C" int rast_renderer_destroy(rast_renderer *r)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_renderer_destroy: null renderer");
+    (void)hipSetDevice(r->device);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    rast_model_detach_renderer(r->model, r);
+    for (rast_mesh *m : r->meshes) free_mesh(m);
+    for (rast_texture *t : r->textures) {
+        if (t->texels) (void)hipFree(t->texels);
+        delete t;
+    }
+    DBuf *bufs[] = { &r->d_vs_block_mesh, &r->d_tri_block_mesh, &r->d_meshes, &r->pos4, &r->attr, &r->trirec,
+                     &r->alive, &r->survivors, &r->tile_count[0], &r->tile_count[1], &r->tile_off, &r->tile_cap,
+                     &r->tile_pool, &r->tile_cursor, &r->density, &r->tile_min_depth[0], &r->tile_min_depth[1],
+                     &r->features, &r->queue, &r->pool, &r->big, &r->ovf, &r->counters, &r->defaults, &r->vis, &r->depth,
+                     &r->rgba };
+    for (DBuf *b : bufs) dfree(*b);
+    if (r->h_meshes) (void)hipHostFree(r->h_meshes);
+    if (r->h_counters) (void)hipHostFree(r->h_counters);
+    if (r->h_defaults) (void)hipHostFree(r->h_defaults);
+    for (hipEvent_t &e : r->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (r->staging_done) (void)hipEventDestroy(r->staging_done);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return RAST_OK;
+}
+
+extern "C" int rast_setup_bins(rast_renderer *r, uint32_t tile_size, uint32_t big_threshold)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_setup_bins: null renderer");
+    if (tile_size != 8 && tile_size != 16 && tile_size != 32)
+        return rast_set_error(RAST_E_INVALID_ARG, "tile size %u not in {8, 16, 32}", tile_size);
+    r->tile = tile_size;
+    r->big_threshold = big_threshold ? big_threshold : 256;
+    r->have_prev = false;
+    return RAST_OK;
+}
+
+extern "C" int rast_resize(rast_renderer *r, uint32_t width, uint32_t height)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_resize: null renderer");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return rast_set_error(RAST_E_INVALID_ARG, "framebuffer %ux%u out of range (1..16384)", width, height);
+    HIP_TRY(hipSetDevice(r->device));
+    r->W = width;
+    r->H = height;
+    r->have_prev = false;
+    return alloc_framebuffers(r);
+}
+
+// ------------------------------------------------------------------ meshes
+
+extern "C" int rast_mesh_create(rast_renderer *r, const rast_ivertex *verts, uint32_t num_verts,
+                                const uint32_t *indices, uint32_t num_indices, int topology, const float world[16],
+                                rast_mesh **out)
+{
+    if (!out) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_create: null output");
+    *out = nullptr;
+    if (!r || !verts || !indices || num_verts == 0 || num_indices == 0)
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_create: empty or null mesh data");
+    if (topology != RAST_TOPOLOGY_LIST && topology != RAST_TOPOLOGY_STRIP)
+        return rast_set_error(RAST_E_INVALID_ARG, "unknown topology %d", topology);
+    if (r->meshes.size() >= (size_t)kMaxMeshes) return rast_set_error(RAST_E_INVALID_ARG, "too many meshes");
+    const uint32_t ntri = mesh_triangles(topology, num_indices);
+    uint64_t tris = ntri, verts_total = num_verts;
+    for (rast_mesh *m : r->meshes) { tris += m->ntri; verts_total += m->nv; }
+    if (tris >= 0xFFFFFFF0ull || verts_total >= 0xFFFFFFF0ull)
+        return rast_set_error(RAST_E_INVALID_ARG, "scene exceeds 2^32 triangles or vertices");
+    for (uint32_t i = 0; i < num_indices; ++i)
+        if (indices[i] >= num_verts)
+            return rast_set_error(RAST_E_INVALID_ARG, "index %u at position %u out of range (num_verts %u)", indices[i], i,
+                        num_verts);
+    HIP_TRY(hipSetDevice(r->device));
+    std::vector<float> v16((size_t)num_verts * 16, 0.f);
+    for (uint32_t i = 0; i < num_verts; ++i) memcpy(&v16[(size_t)i * 16], &verts[i], sizeof(rast_ivertex));
+    rast_mesh *m = new (std::nothrow) rast_mesh();
+    if (!m) return rast_set_error(RAST_E_OUT_OF_MEMORY, "host allocation failed");
+    m->r = r;
+    m->nv = num_verts;
+    m->ni = num_indices;
+    m->ntri = ntri;
+    m->topology = topology;
+    for (int i = 0; i < 16; ++i) m->world[i] = world ? world[i] : ((i % 5 == 0) ? 1.f : 0.f);
+    hipError_t e = hipMalloc((void **)&m->vert16, v16.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&m->idx, (size_t)num_indices * 4);
+    if (e == hipSuccess) e = hipMemcpy(m->vert16, v16.data(), v16.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->idx, indices, (size_t)num_indices * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_mesh(m);
+        return rast_set_error(e == hipErrorOutOfMemory ? RAST_E_OUT_OF_MEMORY : RAST_E_HIP, "mesh upload: %s",
+                    hipGetErrorString(e));
+    }
+    r->meshes.push_back(m);
+    r->scene_dirty = true;
+    r->have_prev = false;
+    *out = m;
+    return RAST_OK;
+}
+
+extern "C" int rast_mesh_destroy(rast_mesh *m)
+{
+    if (!m) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_destroy: null mesh");
+    rast_renderer *r = m->r;
+    (void)hipSetDevice(r->device);
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    r->meshes.erase(std::remove(r->meshes.begin(), r->meshes.end(), m), r->meshes.end());
+    free_mesh(m);
+    r->scene_dirty = true;
+    r->have_prev = false;
+    r->frame_valid = false;
+    return RAST_OK;
+}
+
+extern "C" int rast_mesh_set_world(rast_mesh *m, const float world[16])
+{
+    if (!m || !world) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_set_world: null argument");
+    memcpy(m->world, world, sizeof m->world);
+    return RAST_OK;
+}
+
+extern "C" int rast_mesh_set_textures(rast_mesh *m, rast_texture *d, rast_texture *n, rast_texture *s,
+                                      rast_texture *g)
+{
+    if (!m) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_set_textures: null mesh");
+    rast_texture *t[4] = { d, n, s, g };
+    for (int i = 0; i < 4; ++i)
+        if (t[i] && t[i]->r != m->r) return rast_set_error(RAST_E_INVALID_ARG, "texture belongs to another renderer");
+    for (int i = 0; i < 4; ++i) m->tex[i] = t[i];
+    return RAST_OK;
+}
+
+extern "C" int rast_mesh_num_triangles(const rast_mesh *m, uint32_t *out)
+{
+    if (!m || !out) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_num_triangles: null argument");
+    *out = m->ntri;
+    return RAST_OK;
+}
+
+extern "C" int rast_mesh_triangle_base(const rast_mesh *m, uint64_t *out)
+{
+    if (!m || !out) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_triangle_base: null argument");
+    uint64_t base = 0;
+    for (rast_mesh *q : m->r->meshes) {
+        if (q == m) break;
+        base += q->ntri;
+    }
+    *out = base;
+    return RAST_OK;
+}
+
+// ---------------------------------------------------------------- textures
+
+extern "C" int rast_texture_create(rast_renderer *r, const uint32_t *rgba8, uint32_t w, uint32_t h,
+                                   rast_texture **out)
+{
+    if (!out) return rast_set_error(RAST_E_INVALID_ARG, "rast_texture_create: null output");
+    *out = nullptr;
+    if (!r || !rgba8 || !w || !h || w > 32768 || h > 32768)
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_texture_create: bad texture %ux%u", w, h);
+    HIP_TRY(hipSetDevice(r->device));
+    rast_texture *t = new (std::nothrow) rast_texture();
+    if (!t) return rast_set_error(RAST_E_OUT_OF_MEMORY, "host allocation failed");
+    t->r = r;
+    t->w = w;
+    t->h = h;
+    const size_t bytes = (size_t)w * h * 4;
+    hipError_t e = hipMalloc((void **)&t->texels, bytes);
+    if (e == hipSuccess) e = hipMemcpy(t->texels, rgba8, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (t->texels) (void)hipFree(t->texels);
+        delete t;
+        return rast_set_error(e == hipErrorOutOfMemory ? RAST_E_OUT_OF_MEMORY : RAST_E_HIP, "texture upload: %s",
+                    hipGetErrorString(e));
+    }
+    r->textures.push_back(t);
+    *out = t;
+    return RAST_OK;
+}
+
+extern "C" int rast_texture_destroy(rast_texture *t)
+{
+    if (!t) return rast_set_error(RAST_E_INVALID_ARG, "rast_texture_destroy: null texture");
+    rast_renderer *r = t->r;
+    (void)hipSetDevice(r->device);
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    for (rast_mesh *m : r->meshes)
+        for (auto &slot : m->tex)
+            if (slot == t) slot = nullptr;
+    r->textures.erase(std::remove(r->textures.begin(), r->textures.end(), t), r->textures.end());
+    if (t->texels) (void)hipFree(t->texels);
+    delete t;
+    return RAST_OK;
+}
+
+// ------------------------------------------------------------------ frame
+
+static int rebuild_layout(rast_renderer *r)
+{
+    uint64_t vb = 0, tb = 0;
+    r->vs_block_mesh.clear();
+    r->tri_block_mesh.clear();
+    for (size_t i = 0; i < r->meshes.size(); ++i) {
+        rast_mesh *m = r->meshes[i];
+        m->tri_base = tb;
+        const uint32_t nvb = ceil_div(m->nv, 256), ntb = ceil_div(m->ntri, 256);
+        r->vs_block_mesh.insert(r->vs_block_mesh.end(), nvb, (uint32_t)i);
+        r->tri_block_mesh.insert(r->tri_block_mesh.end(), ntb, (uint32_t)i);
+        vb += m->nv;
+        tb += m->ntri;
+    }
+    r->total_verts = (uint32_t)vb;
+    r->total_tris = (uint32_t)tb;
+    r->vs_blocks = (uint32_t)r->vs_block_mesh.size();
+    r->tri_blocks = (uint32_t)r->tri_block_mesh.size();
+    bool synced = false;
+    HIP_TRY(grow(r, r->d_vs_block_mesh, (r->vs_block_mesh.size() + 1) * 4, &synced));
+    HIP_TRY(grow(r, r->d_tri_block_mesh, (r->tri_block_mesh.size() + 1) * 4, &synced));
+    if (!synced) HIP_TRY(hipStreamSynchronize(r->stream));
+    if (!r->vs_block_mesh.empty())
+        HIP_TRY(hipMemcpy(r->d_vs_block_mesh.p, r->vs_block_mesh.data(), r->vs_block_mesh.size() * 4,
+                          hipMemcpyHostToDevice));
+    if (!r->tri_block_mesh.empty())
+        HIP_TRY(hipMemcpy(r->d_tri_block_mesh.p, r->tri_block_mesh.data(), r->tri_block_mesh.size() * 4,
+                          hipMemcpyHostToDevice));
+    r->scene_dirty = false;
+    return RAST_OK;
+}
+
+static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
+{
+    bool synced = false;
+    const size_t nv = (size_t)r->total_verts + 1, nt = (size_t)r->total_tris + 1;
+    HIP_TRY(grow(r, r->pos4, nv * 16, &synced));
+    HIP_TRY(grow(r, r->attr, nv * 64, &synced));
+    HIP_TRY(grow(r, r->trirec, nt * kTriRecWords * 4, &synced));
+    HIP_TRY(grow(r, r->alive, nt, &synced));
+    HIP_TRY(grow(r, r->survivors, nt * 4, &synced));
+    const size_t tb = ((size_t)ntiles + 64) * 4;
+    for (int i = 0; i < 2; ++i) {
+        HIP_TRY(grow(r, r->tile_count[i], tb, &synced));
+        HIP_TRY(grow(r, r->tile_min_depth[i], tb, &synced));
+    }
+    HIP_TRY(grow(r, r->tile_off, tb, &synced));
+    HIP_TRY(grow(r, r->tile_cap, tb, &synced));
+    HIP_TRY(grow(r, r->tile_pool, tb, &synced));
+    HIP_TRY(grow(r, r->tile_cursor, tb, &synced));
+    HIP_TRY(grow(r, r->density, tb, &synced));
+    HIP_TRY(grow(r, r->features, ((size_t)ntiles + 64) * kNumFeatures * 4, &synced));
+    HIP_TRY(grow(r, r->counters, sizeof(FrameCounters) + 64, &synced));
+    HIP_TRY(grow(r, r->defaults, 64, &synced));
+    if (r->queue_cap == 0) r->queue_cap = std::max<uint32_t>(1u << 20, std::min<uint64_t>(4ull * nt, 0x7FFFFFFFull));
+    if (r->pool_cap == 0) r->pool_cap = 1u << 20;
+    if (r->big_cap == 0) r->big_cap = 1u << 16;
+    if (r->ovf_cap == 0) r->ovf_cap = 1u << 20;
+    HIP_TRY(grow(r, r->ovf, (size_t)r->ovf_cap * 8, &synced));
+    HIP_TRY(grow(r, r->queue, (size_t)r->queue_cap * 4, &synced));
+    HIP_TRY(grow(r, r->pool, (size_t)r->pool_cap * 4, &synced));
+    HIP_TRY(grow(r, r->big, (size_t)r->big_cap * 4, &synced));
+    if (synced) r->have_prev = false;
+    return RAST_OK;
+}
+
+static int stage_meshes(rast_renderer *r, const rast_frame_consts &fc)
+{
+    const size_t n = r->meshes.size();
+    if (n > r->h_meshes_cap) {
+        HIP_TRY(hipStreamSynchronize(r->stream));
+        if (r->h_meshes) (void)hipHostFree(r->h_meshes);
+        r->h_meshes = nullptr;
+        const size_t cap = std::max<size_t>(n, 16);
+        HIP_TRY(hipHostMalloc((void **)&r->h_meshes, cap * sizeof(DevMesh)));
+        r->h_meshes_cap = cap;
+        bool synced = true;
+        HIP_TRY(grow(r, r->d_meshes, cap * sizeof(DevMesh), &synced));
+    } else {
+        // the previous frame's upload must have left the staging buffer
+        HIP_TRY(hipEventSynchronize(r->staging_done));
+    }
+    uint32_t vb = 0, vsb = 0, tsb = 0;
+    const uint32_t *dflt = (const uint32_t *)r->defaults.p;
+    for (size_t i = 0; i < n; ++i) {
+        const rast_mesh *m = r->meshes[i];
+        DevMesh &d = r->h_meshes[i];
+        d.ivert = m->vert16;
+        d.idx = m->idx;
+        d.num_verts = m->nv;
+        d.num_tris = m->ntri;
+        d.vert_base = vb;
+        d.tri_base = (uint32_t)m->tri_base;
+        d.vs_block0 = vsb;
+        d.tri_block0 = tsb;
+        d.topology = m->topology;
+        d.has_diffuse = m->tex[0] != nullptr;
+        mat4_mul(fc.viewproj, m->world, d.wvp);  // CCR/CUDARenderer.cu:241
+        memcpy(d.world, m->world, sizeof d.world);
+        for (int s = 0; s < 4; ++s) {
+            if (m->tex[s]) d.tex[s] = { m->tex[s]->texels, m->tex[s]->w, m->tex[s]->h };
+            else d.tex[s] = { dflt + s, 1u, 1u };
+        }
+        vb += m->nv;
+        vsb += ceil_div(m->nv, 256);
+        tsb += ceil_div(m->ntri, 256);
+    }
+    if (n) HIP_TRY(hipMemcpyAsync(r->d_meshes.p, r->h_meshes, n * sizeof(DevMesh), hipMemcpyHostToDevice, r->stream));
+    HIP_TRY(hipEventRecord(r->staging_done, r->stream));
+    return RAST_OK;
+}
+
+static FrameBuffers frame_buffers(rast_renderer *r)
+{
+    FrameBuffers b{};
+    b.meshes = (const DevMesh *)r->d_meshes.p;
+    b.vs_block_mesh = (const uint32_t *)r->d_vs_block_mesh.p;
+    b.tri_block_mesh = (const uint32_t *)r->d_tri_block_mesh.p;
+    b.vs_blocks = r->vs_blocks;
+    b.tri_blocks = r->tri_blocks;
+    b.pos4 = (float4 *)r->pos4.p;
+    b.attr = (float4 *)r->attr.p;
+    b.trirec = (uint32_t *)r->trirec.p;
+    b.alive = (uint8_t *)r->alive.p;
+    b.tile_count = (uint32_t *)r->tile_count[r->cur].p;
+    b.tile_off = (uint32_t *)r->tile_off.p;
+    b.tile_cap = (uint32_t *)r->tile_cap.p;
+    b.tile_pool = (uint32_t *)r->tile_pool.p;
+    b.tile_cursor = (uint32_t *)r->tile_cursor.p;
+    b.queue = (uint32_t *)r->queue.p;
+    b.queue_cap = r->queue_cap;
+    b.pool = (uint32_t *)r->pool.p;
+    b.pool_cap = r->pool_cap;
+    b.ovf = (uint2 *)r->ovf.p;
+    b.ovf_cap = r->ovf_cap;
+    b.survivors = (uint32_t *)r->survivors.p;
+    b.big = (uint32_t *)r->big.p;
+    b.big_cap = r->big_cap;
+    b.counters = (FrameCounters *)r->counters.p;
+    b.density = nullptr;
+    b.tile_min_depth = (float *)r->tile_min_depth[r->cur].p;
+    b.vis = (uint32_t *)r->vis.p;
+    b.depth = (float *)r->depth.p;
+    b.rgba = (uint32_t *)r->rgba.p;
+    return b;
+}
+
+enum { EV_START, EV_VERTEX, EV_MODEL, EV_SETUP, EV_SCAN, EV_FILL, EV_FALLBACK, EV_RASTER, EV_SHADE, EV_COUNT };
+
+static int mark(rast_renderer *r, int which)
+{
+    if (!r->timing) return RAST_OK;
+    HIP_TRY(hipEventRecord(r->ev[which], r->stream));
+    r->ev_used[which] = 1;
+    return RAST_OK;
+}
+
+static int validate_opts(const rast_frame_opts *o)
+{
+    if (o->cull_mode < 0 || o->cull_mode > 2) return rast_set_error(RAST_E_INVALID_ARG, "cull_mode %d", o->cull_mode);
+    if (o->sample_state < 0 || o->sample_state > 1) return rast_set_error(RAST_E_INVALID_ARG, "sample_state %d", o->sample_state);
+    if (o->visualisation < 0 || o->visualisation > 2)
+        return rast_set_error(RAST_E_INVALID_ARG, "visualisation %d", o->visualisation);
+    if (o->bin_policy < 0 || o->bin_policy > 2) return rast_set_error(RAST_E_INVALID_ARG, "bin_policy %d", o->bin_policy);
+    return RAST_OK;
+}
+
+static int enqueue_frame(rast_renderer *r)
+{
+    const rast_camera &cam = r->last_cam;
+    const rast_frame_opts &o = r->last_opts;
+    HIP_TRY(hipSetDevice(r->device));
+    if (r->scene_dirty) RAST_TRY(rebuild_layout(r));
+    const uint32_t tx = ceil_div(r->W, r->tile), ty = ceil_div(r->H, r->tile), ntiles = tx * ty;
+    RAST_TRY(ensure_frame_storage(r, ntiles));
+    rast_frame_consts fc;
+    camera_frame(cam, r->W, r->H, fc);
+    // 1x1 stand-ins for missing maps (deviation D8), slot 0 unused
+    HIP_TRY(hipEventSynchronize(r->staging_done));
+    r->h_defaults[0] = 0xFFFFFFFFu;
+    r->h_defaults[1] = o.default_normal;
+    r->h_defaults[2] = o.default_specular;
+    r->h_defaults[3] = o.default_gloss;
+    HIP_TRY(hipMemcpyAsync(r->defaults.p, r->h_defaults, 16, hipMemcpyHostToDevice, r->stream));
+    RAST_TRY(stage_meshes(r, fc));
+
+    FrameParams fp{};
+    memcpy(fp.cam_pos, fc.cam_pos, sizeof fp.cam_pos);
+    memcpy(fp.cam_fwd, fc.cam_fwd, sizeof fp.cam_fwd);
+    fp.width = r->W;
+    fp.height = r->H;
+    fp.cull_mode = o.cull_mode;
+    fp.sample_state = o.sample_state;
+    fp.visualisation = o.visualisation;
+    fp.clear_rgba = o.clear_rgba;
+    fp.tile = r->tile;
+    fp.tiles_x = tx;
+    fp.tiles_y = ty;
+    fp.big_threshold = r->big_threshold;
+    fp.num_tris = r->total_tris;
+    fp.num_verts = r->total_verts;
+    fp.num_meshes = (uint32_t)r->meshes.size();
+
+    r->timing = (o.flags & RAST_FLAG_TIMING) != 0;
+    memset(r->ev_used, 0, sizeof r->ev_used);
+    const bool had_prev = r->have_prev;
+    bool single = o.bin_policy != RAST_BIN_EXACT && had_prev;
+    if (o.bin_policy == RAST_BIN_LEARNED && !r->model) single = false;
+    FrameBuffers b = frame_buffers(r);
+    const uint32_t *prev_count = (const uint32_t *)r->tile_count[r->cur ^ 1].p;
+    const float *prev_depth = (const float *)r->tile_min_depth[r->cur ^ 1].p;
+    if (r->model) b.density = (uint32_t *)r->density.p;  // feature of the capacity model
+
+    RAST_TRY(mark(r, EV_START));
+    HIP_TRY(launch_frame_init(fp, b, single, r->stream));
+    HIP_TRY(launch_vertex(fp, b, r->stream));
+    RAST_TRY(mark(r, EV_VERTEX));
+    if (single) {
+        if (o.bin_policy == RAST_BIN_LEARNED) {
+            RAST_TRY(rast_model_capacities(r->model, fp, prev_count, prev_depth, b.density, (float *)r->features.p,
+                                           r->margin, r->slack, b.tile_cap, r->stream));
+        } else {
+            HIP_TRY(launch_prevframe_capacity(ntiles, prev_count, r->margin, r->slack, b.tile_cap, r->stream));
+        }
+        RAST_TRY(mark(r, EV_MODEL));
+        HIP_TRY(launch_scan(ntiles, b.tile_cap, b.tile_off, nullptr, r->queue_cap, b.counters, true, r->stream));
+        RAST_TRY(mark(r, EV_SCAN));
+        HIP_TRY(launch_setup(fp, b, true, r->stream));
+        RAST_TRY(mark(r, EV_SETUP));
+        HIP_TRY(launch_fallback(fp, b, r->stream));
+        RAST_TRY(mark(r, EV_FALLBACK));
+    } else {
+        b.tile_pool = nullptr;
+        b.tile_cap = nullptr;
+        HIP_TRY(launch_setup(fp, b, false, r->stream));
+        RAST_TRY(mark(r, EV_SETUP));
+        HIP_TRY(launch_scan(ntiles, b.tile_count, b.tile_off, b.tile_cursor, r->queue_cap, b.counters, false,
+                            r->stream));
+        RAST_TRY(mark(r, EV_SCAN));
+        HIP_TRY(launch_fill(fp, b, r->stream));
+        RAST_TRY(mark(r, EV_FILL));
+    }
+    HIP_TRY(launch_raster(fp, b, r->small_limit, r->stream));
+    RAST_TRY(mark(r, EV_RASTER));
+    if (!(o.flags & RAST_FLAG_NO_SHADE)) {
+        HIP_TRY(launch_shade(fp, b, r->stream));
+        RAST_TRY(mark(r, EV_SHADE));
+    }
+    HIP_TRY(hipMemcpyAsync(r->h_counters, b.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, r->stream));
+    r->stats.num_triangles = r->total_tris;
+    r->stats.num_tiles = ntiles;
+    r->frame_pending = true;
+    r->frame_valid = false;
+    r->have_prev = true;
+    r->frame_had_prev = had_prev;
+    r->cur ^= 1;
+    r->last_single = single;
+    return RAST_OK;
+}
+
+// Re-run the last frame after growing whichever storage overflowed.
+static int check_frame(rast_renderer *r)
+{
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        HIP_TRY(hipStreamSynchronize(r->stream));
+        const FrameCounters c = *r->h_counters;
+        if (!c.storage_overflow) {
+            r->frame_pending = false;
+            r->frame_valid = true;
+            return RAST_OK;
+        }
+        const uint64_t need_q = std::max<uint64_t>(c.total_pairs, c.capacity_total);
+        if (need_q >= r->queue_cap) r->queue_cap = (uint32_t)std::min<uint64_t>(need_q + need_q / 4 + 1024, 0xFFFFFFF0ull);
+        if (c.pool_used >= r->pool_cap) r->pool_cap = (uint32_t)std::min<uint64_t>((uint64_t)c.pool_used * 5 / 4 + 1024, 0xFFFFFFF0ull);
+        if (c.num_big >= r->big_cap) r->big_cap = (uint32_t)std::min<uint64_t>((uint64_t)c.num_big * 5 / 4 + 1024, 0xFFFFFFF0ull);
+        if (c.ovf_entries >= r->ovf_cap) r->ovf_cap = (uint32_t)std::min<uint64_t>((uint64_t)c.ovf_entries * 5 / 4 + 1024, 0x1FFFFFF0ull);
+        r->cur ^= 1;  // redo the same frame into the same buffers
+        r->have_prev = r->prev_valid_before;
+        ++r->rerenders;
+        RAST_TRY(enqueue_frame(r));
+    }
+    return rast_set_error(RAST_E_STATE, "frame storage kept overflowing");
+}
+
+extern "C" int rast_render(rast_renderer *r, const rast_camera *cam, const rast_frame_opts *opts)
+{
+    if (!r || !cam || !opts) return rast_set_error(RAST_E_INVALID_ARG, "rast_render: null argument");
+    RAST_TRY(validate_opts(opts));
+    if (!(cam->near_z > 0.f) || !(cam->far_z > cam->near_z) || !(cam->fov_deg > 0.f && cam->fov_deg < 180.f))
+        return rast_set_error(RAST_E_INVALID_ARG, "camera near/far/fov out of range");
+    if (r->frame_pending) RAST_TRY(check_frame(r));
+    r->last_cam = *cam;
+    r->last_opts = *opts;
+    r->prev_valid_before = r->have_prev;
+    if (r->meshes.empty()) return rast_set_error(RAST_E_STATE, "rast_render: no meshes loaded");
+    return enqueue_frame(r);
+}
+
+extern "C" int rast_sync(rast_renderer *r)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_sync: null renderer");
+    HIP_TRY(hipSetDevice(r->device));
+    if (r->frame_pending) return check_frame(r);
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return RAST_OK;
+}
+
+extern "C" int rast_readback(rast_renderer *r, uint32_t *vis, float *depth, uint32_t *rgba)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_readback: null renderer");
+    RAST_TRY(rast_sync(r));
+    if (!r->frame_valid) return rast_set_error(RAST_E_STATE, "rast_readback: no rendered frame");
+    const size_t bytes = (size_t)r->W * r->H * 4;
+    if (vis) HIP_TRY(hipMemcpy(vis, r->vis.p, bytes, hipMemcpyDeviceToHost));
+    if (depth) HIP_TRY(hipMemcpy(depth, r->depth.p, bytes, hipMemcpyDeviceToHost));
+    if (rgba) HIP_TRY(hipMemcpy(rgba, r->rgba.p, bytes, hipMemcpyDeviceToHost));
+    return RAST_OK;
+}
+
+static float elapsed(rast_renderer *r, int a, int b)
+{
+    if (!r->ev_used[a] || !r->ev_used[b]) return 0.f;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r->ev[a], r->ev[b]) != hipSuccess) return 0.f;
+    return ms;
+}
+
+extern "C" int rast_get_stats(rast_renderer *r, rast_stats *out)
+{
+    if (!r || !out) return rast_set_error(RAST_E_INVALID_ARG, "rast_get_stats: null argument");
+    RAST_TRY(rast_sync(r));
+    const FrameCounters c = *r->h_counters;
+    rast_stats s = r->stats;
+    s.num_visible = c.num_alive;
+    s.num_bin_pairs = c.total_pairs;
+    s.num_big = c.num_big;
+    s.overflow_tiles = c.overflow_tiles;
+    s.capacity_total = r->last_single ? c.capacity_total : 0;
+    s.rerenders = r->rerenders;
+    s.ms_vertex = elapsed(r, EV_START, EV_VERTEX);
+    if (r->last_single) {
+        s.ms_model = elapsed(r, EV_VERTEX, EV_MODEL);
+        s.ms_scan = elapsed(r, EV_MODEL, EV_SCAN);
+        s.ms_setup = elapsed(r, EV_SCAN, EV_SETUP);
+        s.ms_fallback = elapsed(r, EV_SETUP, EV_FALLBACK);
+        s.ms_raster = elapsed(r, EV_FALLBACK, EV_RASTER);
+    } else {
+        s.ms_setup = elapsed(r, EV_VERTEX, EV_SETUP);
+        s.ms_scan = elapsed(r, EV_SETUP, EV_SCAN);
+        s.ms_fill = elapsed(r, EV_SCAN, EV_FILL);
+        s.ms_raster = elapsed(r, EV_FILL, EV_RASTER);
+    }
+    s.ms_count = s.ms_setup;
+    s.ms_shade = elapsed(r, EV_RASTER, EV_SHADE);
+    s.ms_total = r->ev_used[EV_SHADE] ? elapsed(r, EV_START, EV_SHADE) : elapsed(r, EV_START, EV_RASTER);
+    *out = s;
+    return RAST_OK;
+}
+
+extern "C" int rast_frame_buffers(rast_renderer *r, void **vis, void **depth, void **rgba)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_frame_buffers: null renderer");
+    if (vis) *vis = r->vis.p;
+    if (depth) *depth = r->depth.p;
+    if (rgba) *rgba = r->rgba.p;
+    return RAST_OK;
+}
+
+extern "C" int rast_debug_read(rast_renderer *r, int which, void *dst, size_t bytes, size_t *size_out)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_debug_read: null renderer");
+    RAST_TRY(rast_sync(r));
+    const uint32_t ntiles = ceil_div(r->W, r->tile) * ceil_div(r->H, r->tile);
+    const int last = r->cur ^ 1;  // buffers of the frame just rendered
+    size_t size = 0;
+    const void *src = nullptr;
+    std::vector<uint32_t> tmp;
+    switch (which) {
+    case RAST_DEBUG_VERTEX_POS: size = (size_t)r->total_verts * 16; src = r->pos4.p; break;
+    case RAST_DEBUG_VERTEX_ATTR: size = (size_t)r->total_verts * 64; src = r->attr.p; break;
+    case RAST_DEBUG_TRI_RECORDS: size = (size_t)r->total_tris * kTriRecWords * 4; src = r->trirec.p; break;
+    case RAST_DEBUG_TRI_ALIVE: size = r->total_tris; src = r->alive.p; break;
+    case RAST_DEBUG_TILE_COUNTS: size = (size_t)ntiles * 4; src = r->tile_count[last].p; break;
+    case RAST_DEBUG_BIG_LIST: {
+        const uint32_t nb = std::min(r->h_counters->num_big, r->big_cap);
+        tmp.resize(nb + 1);
+        tmp[0] = nb;
+        if (nb) HIP_TRY(hipMemcpy(tmp.data() + 1, r->big.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
+        size = tmp.size() * 4;
+        if (size_out) *size_out = size;
+        if (dst) memcpy(dst, tmp.data(), std::min(size, bytes));
+        return RAST_OK;
+    }
+    default: return rast_set_error(RAST_E_INVALID_ARG, "unknown debug buffer %d", which);
+    }
+    if (size_out) *size_out = size;
+    if (dst && size && src) HIP_TRY(hipMemcpy(dst, src, std::min(size, bytes), hipMemcpyDeviceToHost));
+    return RAST_OK;
+}
+
+extern "C" int rast_renderer_set_binmodel(rast_renderer *r, rast_binmodel *m, float margin, uint32_t slack)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_renderer_set_binmodel: null renderer");
+    if (!(margin >= 1.f) || margin > 64.f) return rast_set_error(RAST_E_INVALID_ARG, "margin %g not in [1, 64]", margin);
+    r->model = m;
+    r->margin = margin;
+    r->slack = slack;
+    return RAST_OK;
+}
+
+extern "C" int rast_render_batch(const rast_job *jobs, uint32_t n)
+{
+    if (!jobs && n) return rast_set_error(RAST_E_INVALID_ARG, "rast_render_batch: null jobs");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!jobs[i].target) return rast_set_error(RAST_E_INVALID_ARG, "job %u has no target renderer", i);
+        RAST_TRY(rast_render(jobs[i].target, &jobs[i].camera, &jobs[i].opts));
+    }
+    for (uint32_t i = 0; i < n; ++i) RAST_TRY(rast_sync(jobs[i].target));
+    return RAST_OK;
+}

@@ -1,0 +1,610 @@
+// rast_mlp.hip -- learned bin-capacity model for gfx950.
+//
+// A per-tile MLP in -> H -> H -> 1 (ReLU, fp32 weights and accumulation)
+// predicts log(1 + queue length) of each screen tile from features available
+// before binning: the previous frame's exact queue lengths (the tile and its
+// 3x3 neighbourhood), the previous frame's min depth, the current frame's
+// projected-vertex density and the tile position.  Capacities
+// ceil(expm1(pred) * margin) + slack let binning run in one pass; entries
+// beyond a tile's capacity are re-binned exactly (rast_kernels.hip), so the
+// prediction never changes the frame.  The reference has no counterpart: it
+// sizes every queue to a compile-time constant (CCR/DEFINES.h:21-23).
+//
+// Training runs on the GPU with the kernels below: forward + backward per
+// sample (lane = sample), weight gradients as fixed-order reductions over the
+// batch (deterministic), and Adam.  Loss: pinball (quantile tau) on
+// target - pred, averaged over the batch; its gradient is checked against
+// the CPU oracle.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "rast_launch.h"
+#include "rast_model.h"
+#include "rast_state.h"
+
+using namespace rast;
+
+namespace {
+
+constexpr int F = kNumFeatures;
+
+struct ParamView {
+    const float *W1, *b1, *W2, *b2, *W3, *b3;
+};
+
+__host__ __device__ inline size_t num_params(int H) { return (size_t)H * F + H + (size_t)H * H + H + H + 1; }
+
+__device__ __forceinline__ ParamView view(const float *p, int H)
+{
+    ParamView v;
+    v.W1 = p;
+    v.b1 = v.W1 + H * F;
+    v.W2 = v.b1 + H;
+    v.b2 = v.W2 + H * H;
+    v.W3 = v.b2 + H;
+    v.b3 = v.W3 + H;
+    return v;
+}
+
+// ---------------------------------------------------------------- features
+
+__global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t *__restrict__ prev,
+                                                  const float *__restrict__ prev_depth,
+                                                  const uint32_t *__restrict__ density, float *__restrict__ out)
+{
+    const uint32_t n = fp.tiles_x * fp.tiles_y;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n) return;
+    const int tx = (int)(t % fp.tiles_x), ty = (int)(t / fp.tiles_x);
+    float sum = 0.f, mx = 0.f, dsum = 0.f;
+    int cnt = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int x = tx + dx, y = ty + dy;
+            if (x < 0 || y < 0 || x >= (int)fp.tiles_x || y >= (int)fp.tiles_y) continue;
+            const uint32_t q = (uint32_t)y * fp.tiles_x + (uint32_t)x;
+            const float c = (float)prev[q];
+            sum += c;
+            mx = c > mx ? c : mx;
+            dsum += density ? (float)density[q] : 0.f;
+            ++cnt;
+        }
+    float *o = out + (size_t)t * F;
+    o[0] = log1pf((float)prev[t]);
+    o[1] = log1pf(sum / (float)cnt);
+    o[2] = log1pf(mx);
+    o[3] = log1pf(density ? (float)density[t] : 0.f);
+    o[4] = log1pf(dsum / (float)cnt);
+    o[5] = prev_depth[t];
+    o[6] = ((float)ty + 0.5f) / (float)fp.tiles_y;
+    o[7] = ((float)tx + 0.5f) / (float)fp.tiles_x;
+}
+
+__global__ __launch_bounds__(256) void k_targets(uint32_t n, const uint32_t *__restrict__ count,
+                                                 float *__restrict__ y)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < n) y[t] = log1pf((float)count[t]);
+}
+
+__global__ __launch_bounds__(256) void k_prevframe_capacity(uint32_t n, const uint32_t *__restrict__ prev,
+                                                            float margin, uint32_t slack, uint32_t *__restrict__ cap)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < n) {
+        const float c = ceilf((float)prev[t] * margin);
+        cap[t] = (c < 1.0e8f ? (uint32_t)c : 100000000u) + slack;
+    }
+}
+
+// ----------------------------------------------------------------- forward
+
+// One sample per lane; weights staged in LDS (broadcast reads).
+template <int H>
+__device__ __forceinline__ float mlp_forward_lane(const float *__restrict__ w, const float *x, float *h1)
+{
+    const float *W1 = w, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        float a = b1[k];
+#pragma unroll
+        for (int f = 0; f < F; ++f) a += W1[k * F + f] * x[f];
+        h1[k] = a > 0.f ? a : 0.f;
+    }
+    float out = b3[0];
+    for (int j = 0; j < H; ++j) {
+        float a = b2[j];
+#pragma unroll
+        for (int k = 0; k < H; ++k) a += W2[j * H + k] * h1[k];
+        out += W3[j] * (a > 0.f ? a : 0.f);
+    }
+    return out;
+}
+
+template <int H>
+__global__ __launch_bounds__(256) void k_mlp_infer(uint32_t n, const float *__restrict__ x, const float *__restrict__ p,
+                                                   float margin, uint32_t slack, uint32_t *__restrict__ cap,
+                                                   float *__restrict__ pred)
+{
+    constexpr int NP = H * F + H + H * H + H + H + 1;
+    __shared__ float w[NP];
+    for (int i = threadIdx.x; i < NP; i += 256) w[i] = p[i];
+    __syncthreads();
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n) return;
+    float xv[F], h1[H];
+#pragma unroll
+    for (int f = 0; f < F; ++f) xv[f] = x[(size_t)t * F + f];
+    const float o = mlp_forward_lane<H>(w, xv, h1);
+    if (pred) pred[t] = o;
+    if (cap) {
+        const float c = ceilf(expm1f(o < 20.f ? o : 20.f) * margin);
+        cap[t] = (c > 0.f ? (uint32_t)c : 0u) + slack;
+    }
+}
+
+// ----------------------------------------------------------------- training
+
+// Per sample: activations and deltas for the weight-gradient reduction.
+template <int H>
+__global__ __launch_bounds__(256) void k_mlp_fwd_bwd(uint32_t B, uint32_t N, const float *__restrict__ X,
+                                                     const float *__restrict__ Y, const uint32_t *__restrict__ idx,
+                                                     const float *__restrict__ p, float tau, float *__restrict__ Xb,
+                                                     float *__restrict__ H1, float *__restrict__ H2,
+                                                     float *__restrict__ D1, float *__restrict__ D2,
+                                                     float *__restrict__ dout, float *__restrict__ L)
+{
+    constexpr int NP = H * F + H + H * H + H + H + 1;
+    __shared__ float w[NP];
+    for (int i = threadIdx.x; i < NP; i += 256) w[i] = p[i];
+    __syncthreads();
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= B) return;
+    const float *W1 = w, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
+    uint32_t src = idx ? idx[s] : s;
+    src = src < N ? src : N - 1;
+    float xv[F], h1[H], h2[H], d2[H];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        xv[f] = X[(size_t)src * F + f];
+        Xb[(size_t)s * F + f] = xv[f];
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        float a = b1[k];
+#pragma unroll
+        for (int f = 0; f < F; ++f) a += W1[k * F + f] * xv[f];
+        h1[k] = a > 0.f ? a : 0.f;
+    }
+    float out = b3[0];
+    for (int j = 0; j < H; ++j) {
+        float a = b2[j];
+#pragma unroll
+        for (int k = 0; k < H; ++k) a += W2[j * H + k] * h1[k];
+        h2[j] = a > 0.f ? a : 0.f;
+        out += W3[j] * h2[j];
+    }
+    const float r = Y[src] - out;
+    L[s] = r > 0.f ? tau * r : (tau - 1.f) * r;
+    const float g = (r > 0.f ? -tau : 1.f - tau) / (float)B;
+    dout[s] = g;
+#pragma unroll
+    for (int j = 0; j < H; ++j) d2[j] = h2[j] > 0.f ? g * W3[j] : 0.f;
+    for (int k = 0; k < H; ++k) {
+        float a = 0.f;
+#pragma unroll
+        for (int j = 0; j < H; ++j) a += W2[j * H + k] * d2[j];
+        D1[(size_t)s * H + k] = h1[k] > 0.f ? a : 0.f;
+        H1[(size_t)s * H + k] = h1[k];
+    }
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        D2[(size_t)s * H + j] = d2[j];
+        H2[(size_t)s * H + j] = h2[j];
+    }
+}
+
+// Weight gradients, one block per hidden unit j, fixed summation order.
+template <int H>
+__global__ __launch_bounds__(256) void k_mlp_wgrad(uint32_t B, const float *__restrict__ Xb,
+                                                   const float *__restrict__ H1, const float *__restrict__ H2,
+                                                   const float *__restrict__ D1, const float *__restrict__ D2,
+                                                   const float *__restrict__ dout, const float *__restrict__ L,
+                                                   float *__restrict__ grad, float *__restrict__ loss)
+{
+    const int j = blockIdx.x;
+    const int k = threadIdx.x & 63, part = threadIdx.x >> 6;
+    float aW2 = 0.f, aW1 = 0.f, ab1 = 0.f, ab2 = 0.f, aW3 = 0.f, ab3 = 0.f, aL = 0.f;
+    for (uint32_t s = part; s < B; s += 4) {
+        const float d2 = D2[(size_t)s * H + j];
+        const float d1 = D1[(size_t)s * H + j];
+        if (k < H) aW2 += d2 * H1[(size_t)s * H + k];
+        if (k < F) aW1 += d1 * Xb[(size_t)s * F + k];
+        if (k == 0) { ab1 += d1; ab2 += d2; }
+        if (k == 1) aW3 += dout[s] * H2[(size_t)s * H + j];
+        if (j == 0 && k == 2) ab3 += dout[s];
+        if (j == 0 && k == 3) aL += L[s];
+    }
+    __shared__ float red[7][256];
+    red[0][threadIdx.x] = aW2; red[1][threadIdx.x] = aW1; red[2][threadIdx.x] = ab1;
+    red[3][threadIdx.x] = ab2; red[4][threadIdx.x] = aW3; red[5][threadIdx.x] = ab3;
+    red[6][threadIdx.x] = aL;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float v[7];
+        for (int q = 0; q < 7; ++q)
+            v[q] = ((red[q][threadIdx.x] + red[q][threadIdx.x + 64]) + red[q][threadIdx.x + 128]) + red[q][threadIdx.x + 192];
+        float *gW1 = grad, *gb1 = gW1 + H * F, *gW2 = gb1 + H, *gb2 = gW2 + H * H, *gW3 = gb2 + H, *gb3 = gW3 + H;
+        if (k < H) gW2[j * H + k] = v[0];
+        if (k < F) gW1[j * F + k] = v[1];
+        if (k == 0) { gb1[j] = v[2]; gb2[j] = v[3]; }
+        if (k == 1) gW3[j] = v[4];
+        if (j == 0 && k == 2) gb3[0] = v[5];
+        if (j == 0 && k == 3 && loss) loss[0] = v[6] / (float)B;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adam(uint32_t n, float *__restrict__ p, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v, float lr, float b1c,
+                                              float b2c)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / b1c) / (sqrtf(vi / b2c) + 1e-8f);
+}
+
+__global__ __launch_bounds__(256) void k_batch_index(uint32_t B, uint32_t N, uint32_t seed, uint32_t *__restrict__ idx)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= B) return;
+    uint32_t x = seed ^ (s * 0x9E3779B9u);
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    idx[s] = (uint32_t)(((uint64_t)x * N) >> 32);
+}
+
+uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ object
+
+struct rast_binmodel {
+    int device = 0;
+    int H = 64;
+    hipStream_t stream = nullptr;
+    DBuf params, grad, adam_m, adam_v;
+    DBuf X, Y;            // recorded training set
+    uint32_t n = 0, cap = 0;
+    DBuf idx, Xb, H1, H2, D1, D2, dout, L, loss, io_x, io_y, io_p;
+    uint32_t batch_cap = 0;
+    uint64_t adam_t = 0;
+    std::vector<rast_renderer *> users;
+};
+
+namespace rast_detail {
+
+int ensure(DBuf &b, size_t bytes)
+{
+    if (b.p && b.n >= bytes) return RAST_OK;
+    dfree(b);
+    HIP_TRY(hipMalloc(&b.p, bytes < 256 ? 256 : bytes));
+    b.n = bytes < 256 ? 256 : bytes;
+    return RAST_OK;
+}
+
+int launch_infer(int H, uint32_t n, const float *x, const float *p, float margin, uint32_t slack, uint32_t *cap,
+                 float *pred, hipStream_t s)
+{
+    const dim3 g(cdiv(n, 256)), b(256);
+    switch (H) {
+    case 16: hipLaunchKernelGGL(k_mlp_infer<16>, g, b, 0, s, n, x, p, margin, slack, cap, pred); break;
+    case 32: hipLaunchKernelGGL(k_mlp_infer<32>, g, b, 0, s, n, x, p, margin, slack, cap, pred); break;
+    case 64: hipLaunchKernelGGL(k_mlp_infer<64>, g, b, 0, s, n, x, p, margin, slack, cap, pred); break;
+    default: return rast_set_error(RAST_E_INVALID_ARG, "hidden size %d not in {16, 32, 64}", H);
+    }
+    HIP_TRY(hipGetLastError());
+    return RAST_OK;
+}
+
+int ensure_batch(rast_binmodel *m, uint32_t B)
+{
+    if (B <= m->batch_cap) return RAST_OK;
+    const size_t H = (size_t)m->H;
+    RAST_TRY(ensure(m->idx, (size_t)B * 4));
+    RAST_TRY(ensure(m->Xb, (size_t)B * F * 4));
+    RAST_TRY(ensure(m->H1, (size_t)B * H * 4));
+    RAST_TRY(ensure(m->H2, (size_t)B * H * 4));
+    RAST_TRY(ensure(m->D1, (size_t)B * H * 4));
+    RAST_TRY(ensure(m->D2, (size_t)B * H * 4));
+    RAST_TRY(ensure(m->dout, (size_t)B * 4));
+    RAST_TRY(ensure(m->L, (size_t)B * 4));
+    RAST_TRY(ensure(m->loss, 64));
+    m->batch_cap = B;
+    return RAST_OK;
+}
+
+// One forward/backward over B samples (X, Y, optional index list) -> grad, loss.
+int loss_grad(rast_binmodel *m, uint32_t B, uint32_t N, const float *X, const float *Y, const uint32_t *idx, float tau)
+{
+    RAST_TRY(ensure_batch(m, B));
+    const dim3 g(cdiv(B, 256)), b(256);
+    const float *p = (const float *)m->params.p;
+    float *Xb = (float *)m->Xb.p, *H1 = (float *)m->H1.p, *H2 = (float *)m->H2.p, *D1 = (float *)m->D1.p,
+          *D2 = (float *)m->D2.p, *dout = (float *)m->dout.p, *L = (float *)m->L.p;
+    switch (m->H) {
+    case 16:
+        hipLaunchKernelGGL(k_mlp_fwd_bwd<16>, g, b, 0, m->stream, B, N, X, Y, idx, p, tau, Xb, H1, H2, D1, D2, dout, L);
+        hipLaunchKernelGGL(k_mlp_wgrad<16>, dim3(16), b, 0, m->stream, B, Xb, H1, H2, D1, D2, dout, L,
+                           (float *)m->grad.p, (float *)m->loss.p);
+        break;
+    case 32:
+        hipLaunchKernelGGL(k_mlp_fwd_bwd<32>, g, b, 0, m->stream, B, N, X, Y, idx, p, tau, Xb, H1, H2, D1, D2, dout, L);
+        hipLaunchKernelGGL(k_mlp_wgrad<32>, dim3(32), b, 0, m->stream, B, Xb, H1, H2, D1, D2, dout, L,
+                           (float *)m->grad.p, (float *)m->loss.p);
+        break;
+    default:
+        hipLaunchKernelGGL(k_mlp_fwd_bwd<64>, g, b, 0, m->stream, B, N, X, Y, idx, p, tau, Xb, H1, H2, D1, D2, dout, L);
+        hipLaunchKernelGGL(k_mlp_wgrad<64>, dim3(64), b, 0, m->stream, B, Xb, H1, H2, D1, D2, dout, L,
+                           (float *)m->grad.p, (float *)m->loss.p);
+        break;
+    }
+    HIP_TRY(hipGetLastError());
+    return RAST_OK;
+}
+
+}  // namespace rast_detail
+
+using namespace rast_detail;
+
+hipError_t launch_prevframe_capacity(uint32_t n, const uint32_t *prev, float margin, uint32_t slack, uint32_t *cap,
+                                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prevframe_capacity, dim3(cdiv(n, 256)), dim3(256), 0, s, n, prev, margin, slack, cap);
+    return hipGetLastError();
+}
+
+namespace rast {
+hipError_t launch_tile_features(const FrameParams &fp, const uint32_t *prev_count, const float *prev_depth,
+                                const uint32_t *density, float *features, hipStream_t s)
+{
+    const uint32_t n = fp.tiles_x * fp.tiles_y;
+    hipLaunchKernelGGL(k_features, dim3(cdiv(n, 256)), dim3(256), 0, s, fp, prev_count, prev_depth, density, features);
+    return hipGetLastError();
+}
+}  // namespace rast
+
+int rast_model_capacities(rast_binmodel *m, const FrameParams &fp, const uint32_t *prev_count, const float *prev_depth,
+                          const uint32_t *density, float *features, float margin, uint32_t slack, uint32_t *cap,
+                          hipStream_t s)
+{
+    if (!m) return rast_set_error(RAST_E_STATE, "learned binning without a model");
+    HIP_TRY(launch_tile_features(fp, prev_count, prev_depth, density, features, s));
+    return launch_infer(m->H, fp.tiles_x * fp.tiles_y, features, (const float *)m->params.p, margin, slack, cap, nullptr,
+                        s);
+}
+
+void rast_model_detach_renderer(rast_binmodel *m, rast_renderer *r)
+{
+    if (!m) return;
+    for (auto &u : m->users)
+        if (u == r) u = nullptr;
+}
+
+// --------------------------------------------------------------------- ABI
+
+extern "C" int rast_binmodel_num_features(void) { return F; }
+
+static void init_params(std::vector<float> &p, int H, uint32_t seed)
+{
+    uint32_t state = seed * 2654435761u + 12345u;
+    auto next = [&]() {
+        state ^= state << 13; state ^= state >> 17; state ^= state << 5;
+        return (float)(state >> 8) * (1.0f / 16777216.0f);
+    };
+    size_t o = 0;
+    const float a1 = std::sqrt(6.0f / F), a2 = std::sqrt(6.0f / H), a3 = std::sqrt(3.0f / H);
+    for (int i = 0; i < H * F; ++i) p[o++] = (2.f * next() - 1.f) * a1;
+    for (int i = 0; i < H; ++i) p[o++] = 0.01f;
+    for (int i = 0; i < H * H; ++i) p[o++] = (2.f * next() - 1.f) * a2;
+    for (int i = 0; i < H; ++i) p[o++] = 0.01f;
+    for (int i = 0; i < H; ++i) p[o++] = (2.f * next() - 1.f) * a3;
+    p[o++] = 1.0f;
+}
+
+extern "C" int rast_binmodel_create(rast_renderer *r, uint32_t hidden, uint32_t seed, rast_binmodel **out)
+{
+    if (!out) return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_create: null output");
+    *out = nullptr;
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_create: null renderer");
+    if (hidden != 16 && hidden != 32 && hidden != 64)
+        return rast_set_error(RAST_E_INVALID_ARG, "hidden size %u not in {16, 32, 64}", hidden);
+    HIP_TRY(hipSetDevice(r->device));
+    rast_binmodel *m = new (std::nothrow) rast_binmodel();
+    if (!m) return rast_set_error(RAST_E_OUT_OF_MEMORY, "host allocation failed");
+    m->device = r->device;
+    m->H = (int)hidden;
+    m->stream = r->stream;
+    const size_t np = num_params(m->H);
+    std::vector<float> p(np);
+    init_params(p, m->H, seed);
+    int rc = ensure(m->params, np * 4);
+    if (rc == RAST_OK) rc = ensure(m->grad, np * 4);
+    if (rc == RAST_OK) rc = ensure(m->adam_m, np * 4);
+    if (rc == RAST_OK) rc = ensure(m->adam_v, np * 4);
+    if (rc == RAST_OK && hipMemcpy(m->params.p, p.data(), np * 4, hipMemcpyHostToDevice) != hipSuccess) rc = RAST_E_HIP;
+    if (rc == RAST_OK && hipMemset(m->adam_m.p, 0, np * 4) != hipSuccess) rc = RAST_E_HIP;
+    if (rc == RAST_OK && hipMemset(m->adam_v.p, 0, np * 4) != hipSuccess) rc = RAST_E_HIP;
+    if (rc != RAST_OK) {
+        rast_binmodel_destroy(m);
+        return rast_set_error(rc, "rast_binmodel_create: allocation failed");
+    }
+    m->users.push_back(r);
+    *out = m;
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_destroy(rast_binmodel *m)
+{
+    if (!m) return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_destroy: null model");
+    (void)hipSetDevice(m->device);
+    for (rast_renderer *r : m->users)
+        if (r) {
+            (void)hipStreamSynchronize(r->stream);
+            if (r->model == m) r->model = nullptr;
+        }
+    DBuf *bufs[] = { &m->params, &m->grad, &m->adam_m, &m->adam_v, &m->X, &m->Y, &m->idx, &m->Xb, &m->H1,
+                     &m->H2, &m->D1, &m->D2, &m->dout, &m->L, &m->loss, &m->io_x, &m->io_y, &m->io_p };
+    for (DBuf *b : bufs) dfree(*b);
+    delete m;
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_num_params(rast_binmodel *m, size_t *out)
+{
+    if (!m || !out) return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_num_params: null argument");
+    *out = num_params(m->H);
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_get_params(rast_binmodel *m, float *dst, size_t count)
+{
+    if (!m || !dst || count != num_params(m->H))
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_get_params: bad arguments");
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpy(dst, m->params.p, count * 4, hipMemcpyDeviceToHost));
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_set_params(rast_binmodel *m, const float *src, size_t count)
+{
+    if (!m || !src || count != num_params(m->H))
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_set_params: bad arguments");
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpy(m->params.p, src, count * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(m->adam_m.p, 0, count * 4));
+    HIP_TRY(hipMemset(m->adam_v.p, 0, count * 4));
+    m->adam_t = 0;
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_clear_data(rast_binmodel *m)
+{
+    if (!m) return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_clear_data: null model");
+    m->n = 0;
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_record(rast_binmodel *m, rast_renderer *r)
+{
+    if (!m || !r) return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_record: null argument");
+    RAST_TRY(rast_sync(r));
+    if (!r->frame_valid || !r->frame_had_prev || r->model != m)
+        return rast_set_error(RAST_E_STATE,
+                              "rast_binmodel_record: needs two consecutive frames rendered with this model attached");
+    HIP_TRY(hipSetDevice(m->device));
+    const uint32_t tx = (r->W + r->tile - 1) / r->tile, ty = (r->H + r->tile - 1) / r->tile, nt = tx * ty;
+    if ((uint64_t)m->n + nt > 0x7FFFFFFFull) return rast_set_error(RAST_E_STATE, "training set full");
+    if (m->n + nt > m->cap) {
+        const uint32_t ncap = std::max<uint32_t>(m->cap * 2, m->n + nt);
+        DBuf nx, ny;
+        if (hipMalloc(&nx.p, (size_t)ncap * F * 4) != hipSuccess) return rast_set_error(RAST_E_OUT_OF_MEMORY, "training set");
+        nx.n = (size_t)ncap * F * 4;
+        if (hipMalloc(&ny.p, (size_t)ncap * 4) != hipSuccess) {
+            dfree(nx);
+            return rast_set_error(RAST_E_OUT_OF_MEMORY, "training set");
+        }
+        ny.n = (size_t)ncap * 4;
+        if (m->n) {
+            HIP_TRY(hipMemcpy(nx.p, m->X.p, (size_t)m->n * F * 4, hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(ny.p, m->Y.p, (size_t)m->n * 4, hipMemcpyDeviceToDevice));
+        }
+        dfree(m->X);
+        dfree(m->Y);
+        m->X = nx;
+        m->Y = ny;
+        m->cap = ncap;
+    }
+    // features of the frame just rendered: previous frame's counts/depth and
+    // this frame's density; target: this frame's exact counts
+    FrameParams fp{};
+    fp.tiles_x = tx;
+    fp.tiles_y = ty;
+    const int last = r->cur ^ 1, before = r->cur;
+    float *xdst = (float *)m->X.p + (size_t)m->n * F;
+    float *ydst = (float *)m->Y.p + (size_t)m->n;
+    HIP_TRY(launch_tile_features(fp, (const uint32_t *)r->tile_count[before].p, (const float *)r->tile_min_depth[before].p,
+                                 (const uint32_t *)r->density.p, xdst, r->stream));
+    hipLaunchKernelGGL(k_targets, dim3(cdiv(nt, 256)), dim3(256), 0, r->stream, nt,
+                       (const uint32_t *)r->tile_count[last].p, ydst);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    m->n += nt;
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_train(rast_binmodel *m, uint32_t epochs, uint32_t batch, float lr, float tau,
+                                   float *final_loss)
+{
+    if (!m || batch == 0 || batch > (1u << 22) || !(lr > 0.f) || !(tau > 0.f && tau < 1.f))
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_train: bad arguments");
+    if (m->n == 0) return rast_set_error(RAST_E_STATE, "rast_binmodel_train: no recorded data");
+    HIP_TRY(hipSetDevice(m->device));
+    RAST_TRY(ensure_batch(m, batch));
+    const uint32_t steps = std::max<uint32_t>(1, (uint32_t)(((uint64_t)m->n * epochs + batch - 1) / batch));
+    const uint32_t np = (uint32_t)num_params(m->H);
+    for (uint32_t st = 0; st < steps; ++st) {
+        hipLaunchKernelGGL(k_batch_index, dim3(cdiv(batch, 256)), dim3(256), 0, m->stream, batch, m->n,
+                           (uint32_t)(m->adam_t * 0x9E3779B1ull + 7u), (uint32_t *)m->idx.p);
+        RAST_TRY(loss_grad(m, batch, m->n, (const float *)m->X.p, (const float *)m->Y.p, (const uint32_t *)m->idx.p, tau));
+        ++m->adam_t;
+        const float b1c = 1.f - std::pow(0.9f, (float)m->adam_t), b2c = 1.f - std::pow(0.999f, (float)m->adam_t);
+        hipLaunchKernelGGL(k_adam, dim3(cdiv(np, 256)), dim3(256), 0, m->stream, np, (float *)m->params.p,
+                           (const float *)m->grad.p, (float *)m->adam_m.p, (float *)m->adam_v.p, lr, b1c, b2c);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    if (final_loss) HIP_TRY(hipMemcpy(final_loss, m->loss.p, 4, hipMemcpyDeviceToHost));
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_forward(rast_binmodel *m, const float *x, uint32_t n, float *pred)
+{
+    if (!m || !x || !pred || n == 0) return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_forward: bad arguments");
+    HIP_TRY(hipSetDevice(m->device));
+    RAST_TRY(ensure(m->io_x, (size_t)n * F * 4));
+    RAST_TRY(ensure(m->io_p, (size_t)n * 4));
+    HIP_TRY(hipMemcpy(m->io_x.p, x, (size_t)n * F * 4, hipMemcpyHostToDevice));
+    RAST_TRY(launch_infer(m->H, n, (const float *)m->io_x.p, (const float *)m->params.p, 1.f, 0, nullptr,
+                          (float *)m->io_p.p, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpy(pred, m->io_p.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return RAST_OK;
+}
+
+extern "C" int rast_binmodel_loss_grad(rast_binmodel *m, const float *x, const float *target, uint32_t n, float tau,
+                                       float *loss, float *grad)
+{
+    if (!m || !x || !target || !grad || n == 0 || n > (1u << 22))
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_loss_grad: bad arguments");
+    HIP_TRY(hipSetDevice(m->device));
+    RAST_TRY(ensure(m->io_x, (size_t)n * F * 4));
+    RAST_TRY(ensure(m->io_y, (size_t)n * 4));
+    HIP_TRY(hipMemcpy(m->io_x.p, x, (size_t)n * F * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->io_y.p, target, (size_t)n * 4, hipMemcpyHostToDevice));
+    RAST_TRY(loss_grad(m, n, n, (const float *)m->io_x.p, (const float *)m->io_y.p, nullptr, tau));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpy(grad, m->grad.p, num_params(m->H) * 4, hipMemcpyDeviceToHost));
+    if (loss) HIP_TRY(hipMemcpy(loss, m->loss.p, 4, hipMemcpyDeviceToHost));
+    return RAST_OK;
+}

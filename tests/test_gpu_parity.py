@@ -1,0 +1,141 @@
+"""GPU parity: librast.so (HIP, gfx950) against the CPU oracle on identical
+seeded inputs.  Visibility and depth must be bit-exact; RGBA8 within 1/255
+per channel (the only non-bitwise step is powf in the Phong term)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import rast as R
+import scenes as S
+from scene_util import mesh, random_triangle_scene, screen_camera
+
+pytestmark = pytest.mark.gpu
+
+RGBA_TOL = 1  # per channel, in units of 1/255
+
+
+def gpu_render(scene, cam, opts, W, H, tile=16, policy=R.BIN_EXACT, frames=1, big_threshold=256):
+    with R.Renderer(W, H, tile=tile, big_threshold=big_threshold) as r:
+        r.add_scene(scene)
+        for _ in range(frames):
+            r.render(cam, opts, policy)
+        vis, depth, rgba = r.readback()
+        st = r.stats()
+    return vis, depth, rgba, st
+
+
+def assert_bitwise(name, got, want):
+    g = got.view(np.uint32)
+    w = want.view(np.uint32)
+    bad = np.argwhere(g != w)
+    if bad.size:
+        y, x = bad[0]
+        raise AssertionError(f"{name}: {len(bad)} pixels differ; first at (x={x}, y={y}): "
+                             f"got {g[y, x]:#010x} want {w[y, x]:#010x}")
+
+
+def assert_rgba_close(got, want, tol=RGBA_TOL):
+    g = got.view(np.uint8).astype(np.int16)
+    w = want.view(np.uint8).astype(np.int16)
+    d = np.abs(g - w)
+    if d.max() > tol:
+        y, xc = np.argwhere(d > tol)[0]
+        raise AssertionError(f"rgba: {(d > tol).sum()} channels off by > {tol}; max {d.max()}; "
+                             f"first at pixel ({xc // 4}, {y})")
+
+
+def check_frame(scene, cam, opts, W, H, tile=16, policy=R.BIN_EXACT, threads=8):
+    ovis, odepth, orgba, _ = O.render(scene, cam, opts, W, H, threads=threads)
+    vis, depth, rgba, st = gpu_render(scene, cam, opts, W, H, tile=tile, policy=policy)
+    assert_bitwise("vis", vis, ovis)
+    assert_bitwise("depth", depth, odepth)
+    assert_rgba_close(rgba, orgba)
+    return st
+
+
+def test_device_is_gfx950():
+    info = R.device_info(0)
+    print(info)
+    assert info["arch"].startswith("gfx950")
+
+
+def test_config1_uvsphere_parity():
+    w = S.config1()
+    st = check_frame(w.scene, w.camera, w.opts, w.width, w.height)
+    assert st["num_triangles"] == 4032
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("cull", [S.CULL_BACK, S.CULL_NONE, S.CULL_FRONT])
+def test_random_triangles_parity(seed, cull):
+    scene = random_triangle_scene(seed, 300)
+    opts = S.Opts(cull_mode=cull)
+    check_frame(scene, screen_camera(), opts, 160, 96)
+
+
+def test_vertex_outputs_bitwise():
+    w = S.config1()
+    with R.Renderer(w.width, w.height) as r:
+        r.add_scene(w.scene)
+        r.render(w.camera, w.opts)
+        r.sync()
+        pos = r.debug_read(R.DEBUG_VERTEX_POS, np.float32).reshape(-1, 4)
+        attr = r.debug_read(R.DEBUG_VERTEX_ATTR, np.float32).reshape(-1, 16)
+    opos, oattr = O.vertex_transform(w.scene, w.camera, w.width, w.height)
+    assert_bitwise("pos4", pos, opos)
+    # attr layout on the GPU: uv n tan vd c 1/w in one 16-float record (same order as the oracle)
+    assert_bitwise("attr", attr[:, :15], oattr[:, :15])
+
+
+def test_setup_records_bitwise():
+    w = S.config1()
+    with R.Renderer(w.width, w.height) as r:
+        r.add_scene(w.scene)
+        r.render(w.camera, w.opts)
+        r.sync()
+        alive = r.debug_read(R.DEBUG_TRI_ALIVE, np.uint8)
+        rec = r.debug_read(R.DEBUG_TRI_RECORDS, np.uint32).reshape(-1, 16)
+    oflags, orec = O.setup(w.scene, w.camera, w.opts.cull_mode, w.width, w.height)
+    np.testing.assert_array_equal(alive, oflags)
+    live = oflags.astype(bool)
+    assert_bitwise("trirec", rec[live][:, :12], orec[live])
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_tile_counts_exact(tile):
+    w = S.config1()
+    with R.Renderer(w.width, w.height, tile=tile, big_threshold=64) as r:
+        r.add_scene(w.scene)
+        r.render(w.camera, w.opts)
+        r.sync()
+        counts = r.debug_read(R.DEBUG_TILE_COUNTS, np.uint32)
+        big = r.debug_read(R.DEBUG_BIG_LIST, np.uint32)
+    ocounts, obig = O.bin_counts(w.scene, w.camera, w.opts.cull_mode, w.width, w.height, tile, tile, 64)
+    np.testing.assert_array_equal(counts, ocounts.ravel())
+    assert sorted(big[1:1 + big[0]].tolist()) == obig.tolist()
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_tile_size_invariance(tile):
+    w = S.config1()
+    check_frame(w.scene, w.camera, w.opts, w.width, w.height, tile=tile)
+
+
+def test_repeated_frames_bit_identical():
+    w = S.config1()
+    a = gpu_render(w.scene, w.camera, w.opts, w.width, w.height)
+    b = gpu_render(w.scene, w.camera, w.opts, w.width, w.height, frames=3)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def test_big_triangles_screen_list():
+    # two screen-covering triangles plus small ones; big_threshold 1 forces the screen-wide list
+    scene = random_triangle_scene(7, 50)
+    quad = mesh([(-50, -50, -20), (50, -50, -20), (50, 50, -20), (-50, 50, -20)], [(0, 1, 2), (0, 2, 3)])
+    scene = S.Scene(scene.meshes + [quad])
+    for bt in (1, 4, 256):
+        ovis, odepth, _, _ = O.render(scene, screen_camera(), S.Opts(), 128, 96, threads=4, want_rgba=False)
+        vis, depth, _, st = gpu_render(scene, screen_camera(), S.Opts(), 128, 96, big_threshold=bt)
+        assert_bitwise(f"vis bt={bt}", vis, ovis)
+        assert_bitwise(f"depth bt={bt}", depth, odepth)
