@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X tiled rasterizer on the 4K, 2M-triangle scene.
+
+Workload: a 1024x1024-quad fBm terrain (2,097,152 triangles) seen at a 10
+degree grazing angle plus 8 screen-covering backdrop triangles, 3840x2160,
+four 2048^2 RGBA8 maps, fp32 depth, tiles of 16x16.  A step is one frame over
+the next camera of a seeded 100-frame path (disjoint from the 200-frame path
+the bin-capacity model is trained on).  Scene data is uploaded once, so the
+inputs are resident in HBM when the timed region starts.
+
+Prints one JSON line: whole-frame Mtris/s (value) for learned single-pass
+binning, the same for exact two-pass binning, shaded Gpix/s, the shading
+pass against the HBM roofline (algorithmic bytes from the committed fixture
+tests/golden/terrain_shading_bytes.json), and the CPU oracle timed on the
+host cores (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "custom-gpgpu-rasterizer_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+SHADING_FIXTURE = os.path.join(ROOT, "tests", "golden", "terrain_shading_bytes.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--train-frames", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--margin", type=float, default=1.25)
+    ap.add_argument("--slack", type=int, default=4)
+    ap.add_argument("--cpu-frames", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--policies", default="exact,learned,prevframe")
+    return ap.parse_args()
+
+
+def time_policy(R, r, cams, opts, policy, warmup, steps):
+    """Warm up, then time `steps` frames; returns (seconds, per-stage ms sums, overflow stats)."""
+    for i in range(warmup):
+        r.render(cams[i % len(cams)], opts, policy, R.FLAG_TIMING)
+    r.sync()
+    keys = ("ms_vertex", "ms_model", "ms_setup", "ms_scan", "ms_fill", "ms_fallback", "ms_raster", "ms_shade",
+            "ms_total")
+    acc = {k: 0.0 for k in keys}
+    ovf_tiles = 0
+    cap_total = 0
+    pairs = 0
+    rerenders0 = r.stats()["rerenders"]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r.render(cams[(warmup + i) % len(cams)], opts, policy, R.FLAG_TIMING)
+        st = r.stats()  # waits for the frame: per-stage HIP events are read here
+        for k in keys:
+            acc[k] += st[k]
+        ovf_tiles += st["overflow_tiles"]
+        cap_total += st["capacity_total"]
+        pairs += st["num_bin_pairs"]
+    r.sync()
+    secs = time.perf_counter() - t0
+    extra = {"overflow_tiles_per_frame": ovf_tiles / steps, "capacity_entries_per_frame": cap_total / steps,
+             "bin_pairs_per_frame": pairs / steps, "rerenders": r.stats()["rerenders"] - rerenders0}
+    return secs, {k: v / steps for k, v in acc.items()}, extra
+
+
+def cpu_baseline(w, cams, frames):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    O.render(w.scene, cams[0], w.opts, w.width, w.height, threads=threads)  # warm caches
+    t0 = time.perf_counter()
+    for i in range(frames):
+        O.render(w.scene, cams[i % len(cams)], w.opts, w.width, w.height, threads=threads)
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    O.render(w.scene, cams[0], w.opts, w.width, w.height, threads=1)
+    dt1 = time.perf_counter() - t1
+    ntri = w.scene.num_triangles
+    return {"value": round(ntri * frames / dt / 1e6, 3), "unit": "Mtris/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} frames of the terrain test path rendered by the C oracle (restated reference CPU "
+                      f"path) with {threads} threads, vis+depth+shading",
+            "single_thread_mtris_s": round(ntri / dt1 / 1e6, 3), "cpu_model": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build renders on one GPU only; run with --gpus 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import rast as R
+    import scenes as S
+
+    w = S.config3()
+    train_path, test_path = S.train_test_paths(args.train_frames, 100)
+    r = R.Renderer(w.width, w.height, tile=args.tile)
+    r.add_scene(w.scene)
+    ntri = w.scene.num_triangles
+    P = w.width * w.height
+
+    policies = [p for p in args.policies.split(",") if p]
+    results = {}
+    model = None
+    train_s = 0.0
+    if "learned" in policies:
+        t0 = time.perf_counter()
+        model = R.BinModel(r, hidden=64, seed=7)
+        model.attach(r, args.margin, args.slack)
+        for i, cam in enumerate(train_path):
+            r.render(cam, w.opts, R.BIN_EXACT, R.FLAG_NO_SHADE)
+            if i > 0:
+                model.record(r)
+        model.train(epochs=args.epochs, batch=8192, lr=2e-3, tau=0.95)
+        train_s = time.perf_counter() - t0
+    code = {"exact": R.BIN_EXACT, "learned": R.BIN_LEARNED, "prevframe": R.BIN_PREVFRAME}
+    for p in policies:
+        # the projected-vertex density feeding the model is computed only while a model is attached
+        if model is not None:
+            if p == "learned":
+                model.attach(r, args.margin, args.slack)
+            else:
+                model.detach(r)
+        secs, stages, extra = time_policy(R, r, test_path, w.opts, code[p], args.warmup, args.steps)
+        results[p] = {"ms_per_frame": secs * 1e3 / args.steps, "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+                      **extra}
+
+    head = "learned" if "learned" in results else policies[0]
+    ms = results[head]["ms_per_frame"]
+    value = ntri / (ms * 1e-3) / 1e6
+
+    roofline = None
+    if os.path.exists(SHADING_FIXTURE):
+        with open(SHADING_FIXTURE) as f:
+            fx = json.load(f)
+        frames = fx["frames"]
+        used = [(args.warmup + i) % len(test_path) for i in range(args.steps)]
+        b_shade = sum(frames[str(i)]["bytes"] for i in used) / len(used)
+        t_shade = results[head]["stages_ms"]["ms_shade"] * 1e-3
+        if t_shade > 0:
+            ach = b_shade / t_shade / 1e9
+            roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_shade",
+                        "algorithmic_bytes_per_launch": int(b_shade)}
+
+    cpu = None if args.no_cpu else cpu_baseline(w, test_path, args.cpu_frames)
+    line = {
+        "metric": "Mtris/s + shaded Gpix/s at 4K, 2M-tri scene; learned vs exact binning",
+        "value": round(value, 2),
+        "unit": "Mtris/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural terrain, textures and camera path)",
+        "config": {"workload": w.name, "triangles": ntri, "width": w.width, "height": w.height, "tile": args.tile,
+                   "binning": head, "camera_path_frames": 100, "train_frames": args.train_frames},
+        "gpix_per_s": round(P / (ms * 1e-3) / 1e9, 3),
+        "policies": {k: {"mtris_per_s": round(ntri / (v["ms_per_frame"] * 1e-3) / 1e6, 2), **v}
+                     for k, v in results.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "train_seconds": round(train_s, 2),
+    }
+    if "exact" in results and "learned" in results:
+        line["learned_over_exact"] = round(results["exact"]["ms_per_frame"] / results["learned"]["ms_per_frame"], 3)
+    if roofline:
+        line["shading_gpix_per_s"] = round(P / (results[head]["stages_ms"]["ms_shade"] * 1e-3) / 1e9, 2)
+    if model is not None:
+        model.close()
+    r.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -235,8 +235,9 @@ int rsc_terrain(uint32_t n, float size, float amplitude, uint32_t octaves, uint3
             V3 nn = norm(V3{ -dhdx, 1.0, -dhdz });
             V3 t = norm(V3{ 1.0, dhdx, 0.0 });
             V3 p{ -0.5 * size + dx * i, h[(size_t)j * m + i], -0.5 * size + dx * j };
-            put_vertex(verts + (size_t)kVF * ((size_t)j * m + i), p, uv_tiling * i / n, uv_tiling * j / n, nn, t,
-                       V3{ 1, 1, 1 });
+            // texel centres stay inside (0, uv_tiling): the reference sampler does not wrap
+            put_vertex(verts + (size_t)kVF * ((size_t)j * m + i), p, uv_tiling * (i + 0.5) / (n + 1),
+                       uv_tiling * (j + 0.5) / (n + 1), nn, t, V3{ 1, 1, 1 });
         }
     uint32_t *ix = indices;
     for (uint32_t j = 0; j < n; ++j)
@@ -316,15 +317,17 @@ int rsc_camera_path(uint32_t seed, uint32_t frames, float terrain_size, float he
         ph[k] = rsc_uniform(seed, 10 + k, 0.0f, (float)(2 * kPi));
         fr[k] = rsc_uniform(seed, 20 + k, 0.5f, 1.5f);
     }
+    // The camera stays outside the terrain's +z edge with |yaw| <= 0.12 rad, so
+    // every terrain vertex is in front of the eye plane (the reference does not
+    // clip at the near plane, CCR/CUDARenderer.cu:702-796).
     const double half = 0.5 * terrain_size;
     for (uint32_t f = 0; f < frames; ++f) {
         double t = 2 * kPi * f / frames;
-        double x = 0.25 * half * rsc_sin(fr[0] * t + ph[0]);
-        double z = half * (0.75 + 0.15 * rsc_sin(fr[1] * t + ph[1]));
-        double y = height * (1.0 + 0.15 * rsc_sin(fr[2] * t + ph[2]));
-        double yaw = 0.35 * rsc_sin(fr[3] * t + ph[3]);
-        double pitch = -(pitch_deg + 3.0 * rsc_sin(fr[4] * t + ph[4])) * kPi / 180.0;
-        // view direction: towards -z, yawed about +y, pitched about the camera x axis
+        double x = 0.3 * half * rsc_sin(fr[0] * t + ph[0]);
+        double z = half + 0.25 * half * (1.2 + 0.5 * rsc_sin(fr[1] * t + ph[1]));
+        double y = height * (1.0 + 0.1 * rsc_sin(fr[2] * t + ph[2]));
+        double yaw = 0.12 * rsc_sin(fr[3] * t + ph[3]);
+        double pitch = -(pitch_deg + 2.0 * rsc_sin(fr[4] * t + ph[4])) * kPi / 180.0;
         V3 d{ rsc_sin(yaw) * rsc_cos(pitch), rsc_sin(pitch), -rsc_cos(yaw) * rsc_cos(pitch) };
         float *o = out + 6 * (size_t)f;
         o[0] = (float)x; o[1] = (float)y; o[2] = (float)z;

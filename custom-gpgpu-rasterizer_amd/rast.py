@@ -266,6 +266,9 @@ class BinModel:
     def attach(self, renderer: Renderer, margin: float = 1.25, slack: int = 4) -> None:
         check(lib().rast_renderer_set_binmodel(renderer.h, self.h, margin, slack), "rast_renderer_set_binmodel")
 
+    def detach(self, renderer: Renderer) -> None:
+        check(lib().rast_renderer_set_binmodel(renderer.h, None, 1.0, 0), "rast_renderer_set_binmodel")
+
     def record(self, renderer: Renderer) -> None:
         check(lib().rast_binmodel_record(self.h, renderer.h), "rast_binmodel_record")
 

@@ -135,7 +135,7 @@ def icosphere(subdiv=5, radius=1.0) -> Mesh:
     return _mesh_from(lib().rsc_icosphere, subdiv, radius)
 
 
-def terrain(n=1024, size=80.0, amplitude=6.0, octaves=6, seed=SEED_BASE + 3, uv_tiling=8.0) -> Mesh:
+def terrain(n=1024, size=80.0, amplitude=6.0, octaves=6, seed=SEED_BASE + 3, uv_tiling=1.0) -> Mesh:
     return _mesh_from(lib().rsc_terrain, n, size, amplitude, octaves, seed, uv_tiling)
 
 
@@ -169,13 +169,13 @@ def uniform(seed: int, index: int, lo: float, hi: float) -> float:
     return float(lib().rsc_uniform(seed, index, lo, hi))
 
 
-def camera_path(seed: int, frames: int, terrain_size=80.0, height=7.0, pitch_deg=10.0) -> List[Camera]:
+def camera_path(seed: int, frames: int, terrain_size=80.0, height=9.0, pitch_deg=10.0) -> List[Camera]:
     out = np.zeros((frames, 6), np.float32)
     lib().rsc_camera_path(seed, frames, terrain_size, height, pitch_deg, out.ctypes.data)
     return [Camera(position=tuple(map(float, r[:3])), forward=tuple(map(float, r[3:]))) for r in out]
 
 
-def backdrop(cam: Camera, aspect: float, d0=60.0, step=2.0) -> Mesh:
+def backdrop(cam: Camera, aspect: float, d0=120.0, step=2.0) -> Mesh:
     v = np.zeros((24, 14), np.float32)
     ix = np.zeros(24, np.uint32)
     p = np.asarray(cam.position, np.float32)
@@ -219,12 +219,12 @@ def config2(grid=16, subdiv=5, tex_size=2048) -> Workload:
             e = tuple(uniform(seed, 3 * k + q, 0.0, 6.2831853) for q in range(3))
             s = uniform(seed, 1000 + k, 0.5, 1.5)
             meshes.append(Mesh(verts=base.verts, indices=base.indices, world=world_matrix(t, e, s), textures=maps))
-    cam = Camera(position=(0.0, 0.0, 51.0))
+    cam = Camera(position=(0.0, 0.0, 45.0))
     return Workload("icospheres16x16_s5_1920x1080", Scene(meshes), cam, 1920, 1080, Opts())
 
 
 def terrain_camera() -> Camera:
-    return camera_path(SEED_BASE + 3, 1, 80.0, 7.0, 10.0)[0]
+    return camera_path(SEED_BASE + 3, 1)[0]
 
 
 def config3(n=1024, tex_size=2048, width=3840, height=2160, camera: Optional[Camera] = None) -> Workload:
@@ -232,7 +232,7 @@ def config3(n=1024, tex_size=2048, width=3840, height=2160, camera: Optional[Cam
     plus 8 screen-covering backdrop triangles, 3840x2160."""
     seed = SEED_BASE + 3
     cam = camera or terrain_camera()
-    ter = terrain(n, 80.0, 6.0, 6, seed, 8.0)
+    ter = terrain(n, 80.0, 6.0, 6, seed, 1.0)
     maps = noise_maps(tex_size, seed)
     ter.textures = maps
     bd = backdrop(terrain_camera(), width / height)
@@ -242,5 +242,4 @@ def config3(n=1024, tex_size=2048, width=3840, height=2160, camera: Optional[Cam
 
 def train_test_paths(train=200, test=100):
     """Disjoint seeded camera paths over the terrain (different seeds)."""
-    return (camera_path(SEED_BASE + 0x300, train, 80.0, 7.0, 10.0),
-            camera_path(SEED_BASE + 0x301, test, 80.0, 7.0, 10.0))
+    return (camera_path(SEED_BASE + 0x300, train), camera_path(SEED_BASE + 0x301, test))
