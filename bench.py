@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--micro", type=int, default=None, help="bbox pixel limit of the setup-time raster")
+    ap.add_argument("--small", type=int, default=None, help="tile coverage from which a triangle is rasterized per pixel")
     ap.add_argument("--train-frames", type=int, default=200)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--margin", type=float, default=1.25)
@@ -117,6 +119,8 @@ def main():
     w = S.config3()
     train_path, test_path = S.train_test_paths(args.train_frames, 100)
     r = R.Renderer(w.width, w.height, tile=args.tile)
+    if args.micro is not None or args.small is not None:
+        r.set_tuning(args.micro if args.micro is not None else 16, args.small if args.small is not None else 16)
     r.add_scene(w.scene)
     ntri = w.scene.num_triangles
     P = w.width * w.height

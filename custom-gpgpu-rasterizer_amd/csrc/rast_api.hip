@@ -193,6 +193,8 @@ static int alloc_framebuffers(rast_renderer *r)
     HIP_TRY(grow(r, r->vis, P * 4, &synced));
     HIP_TRY(grow(r, r->depth, P * 4, &synced));
     HIP_TRY(grow(r, r->rgba, P * 4, &synced));
+    HIP_TRY(grow(r, r->keys, P * 8, &synced));
+    r->keys_dirty = true;
     r->frame_valid = false;
     return RAST_OK;
 }
@@ -249,7 +251,7 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
     }
     DBuf *bufs[] = { &r->d_vs_block_mesh, &r->d_tri_block_mesh, &r->d_meshes, &r->pos4, &r->attr, &r->trirec,
                      &r->alive, &r->survivors, &r->tile_count[0], &r->tile_count[1], &r->tile_off, &r->tile_cap,
-                     &r->tile_pool, &r->tile_cursor, &r->density, &r->tile_min_depth[0], &r->tile_min_depth[1],
+                     &r->tile_pool, &r->tile_cursor, &r->density, &r->tile_min_depth[0], &r->tile_min_depth[1], &r->keys,
                      &r->features, &r->queue, &r->pool, &r->big, &r->ovf, &r->counters, &r->defaults, &r->vis, &r->depth,
                      &r->rgba };
     for (DBuf *b : bufs) dfree(*b);
@@ -272,6 +274,16 @@ extern "C" int rast_setup_bins(rast_renderer *r, uint32_t tile_size, uint32_t bi
     r->tile = tile_size;
     r->big_threshold = big_threshold ? big_threshold : 256;
     r->have_prev = false;
+    return RAST_OK;
+}
+
+extern "C" int rast_set_tuning(rast_renderer *r, uint32_t micro_limit, uint32_t small_limit)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_set_tuning: null renderer");
+    if (micro_limit > 1024 || small_limit > 1024)
+        return rast_set_error(RAST_E_INVALID_ARG, "micro_limit %u / small_limit %u above 1024", micro_limit, small_limit);
+    r->micro_limit = micro_limit;
+    r->small_limit = small_limit;
     return RAST_OK;
 }
 
@@ -574,6 +586,7 @@ static FrameBuffers frame_buffers(rast_renderer *r)
     b.counters = (FrameCounters *)r->counters.p;
     b.density = nullptr;
     b.tile_min_depth = (float *)r->tile_min_depth[r->cur].p;
+    b.keys = (unsigned long long *)r->keys.p;
     b.vis = (uint32_t *)r->vis.p;
     b.depth = (float *)r->depth.p;
     b.rgba = (uint32_t *)r->rgba.p;
@@ -632,6 +645,7 @@ static int enqueue_frame(rast_renderer *r)
     fp.tiles_x = tx;
     fp.tiles_y = ty;
     fp.big_threshold = r->big_threshold;
+    fp.micro_limit = r->micro_limit;
     fp.num_tris = r->total_tris;
     fp.num_verts = r->total_verts;
     fp.num_meshes = (uint32_t)r->meshes.size();
@@ -647,6 +661,9 @@ static int enqueue_frame(rast_renderer *r)
     if (r->model) b.density = (uint32_t *)r->density.p;  // feature of the capacity model
 
     RAST_TRY(mark(r, EV_START));
+    const bool was_dirty = r->keys_dirty;
+    r->keys_dirty = true;  // until the frame's resolve pass has been enqueued
+    if (was_dirty) HIP_TRY(launch_clear_keys(b.keys, (size_t)r->W * r->H, r->stream));
     HIP_TRY(launch_frame_init(fp, b, single, r->stream));
     HIP_TRY(launch_vertex(fp, b, r->stream));
     RAST_TRY(mark(r, EV_VERTEX));
@@ -676,6 +693,7 @@ static int enqueue_frame(rast_renderer *r)
         RAST_TRY(mark(r, EV_FILL));
     }
     HIP_TRY(launch_raster(fp, b, r->small_limit, r->stream));
+    r->keys_dirty = false;
     RAST_TRY(mark(r, EV_RASTER));
     if (!(o.flags & RAST_FLAG_NO_SHADE)) {
         HIP_TRY(launch_shade(fp, b, r->stream));

@@ -4,7 +4,8 @@
 // HBM layout of one frame (all device buffers owned by the renderer):
 //   ivert    per mesh, 14 floats / vertex (the 56-byte IVertex, as uploaded)
 //   pos4     float4 / global vertex: NDC x, y, z and clip w        (16 B)
-//   attr     4 x float4 / global vertex: uv, n', tan', vd, c, 1/w (64 B)
+//   attr     4 x float4 / global vertex: u v 1/w n'.x | n'.yz tan'.xy |
+//            tan'.z vd | c.rgb 0 (64 B; PBR shading reads the first 48 B)
 //   trirec   16 words / global triangle ID (written for survivors):
 //            [0..5] screen x0,y0,x1,y1,x2,y2  [6..8] z0,z1,z2  [9] |area|
 //            [10] xmin | ymin << 16  [11] xmax | ymax << 16 (pixel range
@@ -12,6 +13,8 @@
 //   alive    u8 / triangle
 //   tiles    count / offset / cursor / capacity u32 per tile, queue entries
 //            u32 triangle IDs, plus one screen-wide "big" list
+//   keys     u64 per pixel: (z bits << 32) | ~(id + 1), min = visible
+//            fragment; reset to (1.0, none) by the resolve pass
 //   vis, depth  u32 / f32 per pixel, rgba u32 per pixel
 #pragma once
 
@@ -58,6 +61,7 @@ struct FrameParams {
     uint32_t tile;            // tile edge in pixels
     uint32_t tiles_x, tiles_y;
     uint32_t big_threshold;   // max tile span kept in per-tile queues
+    uint32_t micro_limit;     // bbox pixel count up to which setup rasterizes directly
     uint32_t num_tris;        // all meshes
     uint32_t num_verts;
     uint32_t num_meshes;

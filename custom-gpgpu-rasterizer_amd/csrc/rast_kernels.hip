@@ -6,14 +6,24 @@
 // CPU restatement of the reference path.  Reference files are cited relative
 // to kdedeurw/Custom-GPGPU-Rasterizer (CCR = CustomCUDARasterizer/).
 //
+// Depth resolution is order-free: every fragment becomes a 64-bit key
+// (z bits << 32 | ~(id + 1)) and the smallest key wins, which is exactly the
+// sequential "reject if z > depth, later triangle wins ties" rule of
+// CCR/ERenderer.cpp:544-554 applied in ascending triangle order.
+//
 // Pipeline (one launch per stage, all meshes at once):
 //   k_vertex   per vertex: clip/NDC position + shading attributes
-//   k_setup    per triangle: assembly, cull, screen setup record, then either
-//              tile counts (exact two-pass) or direct queue fill (single pass)
+//   k_setup    per triangle: assembly, cull, screen setup record; triangles
+//              whose bounding box has at most micro_limit pixels are
+//              rasterized on the spot with 64-bit atomic min into the key
+//              buffer; larger ones are counted per tile (exact two-pass) or
+//              written straight into per-tile queues (single pass); triangles
+//              spanning more than big_threshold tiles go to one screen list
 //   k_scan     per-tile exclusive scan of counts / capacities
-//   k_fill     exact second pass: ordered-by-nothing queue fill
-//   k_fallback single pass: refill tiles whose capacity overflowed
-//   k_raster   one workgroup per tile, depth resolved in LDS
+//   k_fill     exact second pass over the binned triangles
+//   k_fallback single pass: exact placement of entries beyond capacity
+//   k_resolve  one workgroup per tile: keys -> LDS (and cleared for the next
+//              frame), queued triangles rasterized in LDS, vis/depth written
 //   k_shade    per pixel, deferred from the visibility buffer
 #include <hip/hip_runtime.h>
 
@@ -36,39 +46,94 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Order-free append of one element per lane with pred set: one atomic per
-// wave (ballot + prefix count).  Returns the lane's slot (garbage if !pred).
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
+// Order-free append of one element per thread with pred set, one atomic per
+// 256-thread block.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool pred, uint32_t *lds)
 {
     const uint64_t m = __ballot(pred);
-    if (m == 0) return 0;
-    const uint32_t first = (uint32_t)__builtin_ctzll(m);
-    uint32_t base = 0;
-    if (lane_id() == first) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __builtin_amdgcn_readlane(base, first);
-    return base + popc_below(m);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) lds[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t a = lds[0], b = lds[1], c = lds[2], d = lds[3];
+        const uint32_t tot = a + b + c + d;
+        const uint32_t base = tot ? atomicAdd(counter, tot) : 0u;
+        lds[0] = base;
+        lds[1] = base + a;
+        lds[2] = base + a + b;
+        lds[3] = base + a + b + c;
+    }
+    __syncthreads();
+    const uint32_t slot = lds[wave] + popc_below(m);
+    __syncthreads();
+    return slot;
 }
 
-// Per-lane atomic increment of counters[t] aggregated over the lanes of the
-// wave that hit the same t ("peeling": one atomic per distinct t).  Must be
-// called by the active lanes only; returns the lane's previous-value slot.
-__device__ __forceinline__ uint32_t agg_increment(uint32_t *counters, uint32_t t)
+// Groups the active lanes of the wave by tile: returns the mask of lanes with
+// the same t (the lane's group).  One ballot per distinct tile, no atomics.
+__device__ __forceinline__ uint64_t match_tile(uint32_t t)
 {
-    uint32_t slot = 0;
-    bool done = false;
-    while (!done) {
+    uint64_t group = 0;
+    bool pending = true;
+    while (pending) {
         const uint32_t lt = __builtin_amdgcn_readfirstlane(t);
         const uint64_t m = __ballot(t == lt);
         if (t == lt) {
-            const uint32_t rank = popc_below(m);
-            uint32_t b = 0;
-            if (rank == 0) b = atomicAdd(counters + lt, (uint32_t)__popcll(m));
-            b = __builtin_amdgcn_readfirstlane(b);
-            slot = b + rank;
-            done = true;
+            group = m;
+            pending = false;
         }
     }
-    return slot;
+    return group;
+}
+
+// block_append for three predicates at once (two barriers).
+__device__ __forceinline__ void block_append3(uint32_t *c0, bool p0, uint32_t *c1, bool p1, uint32_t *c2, bool p2,
+                                              uint32_t *lds, uint32_t &s0, uint32_t &s1, uint32_t &s2)
+{
+    const uint64_t m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        lds[wave] = (uint32_t)__popcll(m0);
+        lds[4 + wave] = (uint32_t)__popcll(m1);
+        lds[8 + wave] = (uint32_t)__popcll(m2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint32_t *cnt = threadIdx.x == 0 ? c0 : (threadIdx.x == 1 ? c1 : c2);
+        uint32_t *l = lds + 4 * threadIdx.x;
+        const uint32_t a = l[0], b = l[1], c = l[2], d = l[3];
+        const uint32_t tot = a + b + c + d;
+        const uint32_t base = tot ? atomicAdd(cnt, tot) : 0u;
+        l[0] = base;
+        l[1] = base + a;
+        l[2] = base + a + b;
+        l[3] = base + a + b + c;
+    }
+    __syncthreads();
+    s0 = lds[wave] + popc_below(m0);
+    s1 = lds[4 + wave] + popc_below(m1);
+    s2 = lds[8 + wave] + popc_below(m2);
+}
+
+// counters[t] += 1 for every active lane: one atomic per distinct tile, all
+// issued by the group leaders in one instruction; nothing waits for a result.
+__device__ __forceinline__ void count_tile(uint32_t *counters, uint32_t t)
+{
+    const uint64_t g = match_tile(t);
+    if (popc_below(g) == 0) atomicAdd(counters + t, (uint32_t)__popcll(g));
+}
+
+// Returns a unique slot in tile t's queue (previous cursor value): the group
+// leaders reserve their group's slots with one atomic each, in one round trip.
+__device__ __forceinline__ uint32_t claim_slot(uint32_t *cursor, uint32_t t)
+{
+    const uint64_t g = match_tile(t);
+    const uint32_t rank = popc_below(g);
+    const int leader = __builtin_ctzll(g);
+    uint32_t base = 0;
+    if (rank == 0) base = atomicAdd(cursor + t, (uint32_t)__popcll(g));
+    base = __shfl(base, leader);
+    return base + rank;
 }
 
 // ------------------------------------------------- reference arithmetic
@@ -106,16 +171,41 @@ __device__ __forceinline__ uint64_t make_key(float z, uint32_t id)
     return ((uint64_t)zb << 32) | (uint32_t)~(id + 1u);
 }
 
-// Raster-ready triangle in LDS: screen vertices, the three edge vectors of
-// CCR/CUDAROPs.cu:131-133, depths, |area| and the tile-clipped pixel range.
+// Raster-ready triangle: screen vertices, the three edge vectors of
+// CCR/CUDAROPs.cu:131-133, depths, |area| and a pixel range.
 struct TriLds {
     float sx0, sy0, sx1, sy1, sx2, sy2;
     float ax, ay, bx, by, cx, cy;
     float z0, z1, z2, area;
     uint32_t xr, yr;  // x0 | x1 << 16, y0 | y1 << 16 (absolute pixels)
     uint32_t id;
-    uint32_t pad;
+    float inv_w;      // 1 / (x1 - x0), for item -> pixel decoding
 };
+
+// Item q of a triangle's pixel range [x0,x1) x [y0,y1), row-major.
+__device__ __forceinline__ void item_pixel(const TriLds &t, uint32_t q, int &px, int &py)
+{
+    const int x0 = (int)(t.xr & 0xFFFFu), x1 = (int)(t.xr >> 16), y0 = (int)(t.yr & 0xFFFFu);
+    const int w = x1 - x0;
+    int r = (int)((float)q * t.inv_w);  // may be off by one; corrected below
+    int c = (int)q - r * w;
+    if (c < 0) { --r; c += w; }
+    if (c >= w) { ++r; c -= w; }
+    px = x0 + c;
+    py = y0 + r;
+}
+
+// Largest j in [0, n) with off[j] <= k (off non-decreasing, off[0] = 0).
+__device__ __forceinline__ uint32_t find_item_owner(const uint32_t *off, uint32_t n, uint32_t k)
+{
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // Fragment test of RasterizePixelAtomic (CCR/CUDAROPs.cu:329-345) with the D2
 // window: edge functions (CCR/CUDAROPs.cu:120-145), weights /= |area|,
@@ -133,6 +223,13 @@ __device__ __forceinline__ bool fragment(const TriLds &t, float px, float py, fl
     return true;
 }
 
+__device__ __forceinline__ void set_edges(TriLds &t)
+{
+    t.ax = t.sx1 - t.sx0; t.ay = t.sy1 - t.sy0;
+    t.bx = t.sx2 - t.sx1; t.by = t.sy2 - t.sy1;
+    t.cx = t.sx0 - t.sx2; t.cy = t.sy0 - t.sy2;
+}
+
 __device__ __forceinline__ void load_tri(const uint32_t *__restrict__ rec, uint32_t id, TriLds &t)
 {
     const uint4 *r = reinterpret_cast<const uint4 *>(rec + (size_t)id * kTriRecWords);
@@ -144,25 +241,30 @@ __device__ __forceinline__ void load_tri(const uint32_t *__restrict__ rec, uint3
     t.z2 = __uint_as_float(q2.x); t.area = __uint_as_float(q2.y);
     t.xr = q2.z;  // bbox lo (xmin | ymin << 16)
     t.yr = q2.w;  // bbox hi (xmax | ymax << 16)
-    t.ax = t.sx1 - t.sx0; t.ay = t.sy1 - t.sy0;
-    t.bx = t.sx2 - t.sx1; t.by = t.sy2 - t.sy1;
-    t.cx = t.sx0 - t.sx2; t.cy = t.sy0 - t.sy2;
+    set_edges(t);
     t.id = id;
-    t.pad = 0;
+    t.inv_w = 0.f;
 }
 
 // ---------------------------------------------------------------- kernels
 
 __global__ void k_frame_init(uint32_t *tile_count, uint32_t *tile_aux, uint32_t n, FrameCounters *ctr,
-                             uint32_t *density)
+                             uint32_t *density, float *tile_min_depth)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         tile_count[i] = 0;
+        if (tile_min_depth) tile_min_depth[i] = 1.0f;
         if (tile_aux) tile_aux[i] = 0xFFFFFFFFu;
         if (density) density[i] = 0;
     }
     if (i < sizeof(FrameCounters) / 4) reinterpret_cast<uint32_t *>(ctr)[i] = 0;
+}
+
+__global__ void k_clear_keys(unsigned long long *keys, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        keys[i] = kClearKey;
 }
 
 // VS_Kernel / GetNDCVertex (CCR/CUDARenderer.cu:673-700, CCR/CUDAROPs.cu:99-118).
@@ -203,24 +305,26 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
     const float w2 = m4(Wd, 2, 0) * px + m4(Wd, 2, 1) * py + m4(Wd, 2, 2) * pz + m4(Wd, 2, 3);
     float vx = w0 - fp.cam_pos[0], vy = w1 - fp.cam_pos[1], vz = w2 - fp.cam_pos[2];
     normalize3(vx, vy, vz);
+    // PBR shading reads the first three float4s only; the colour is last
     float4 *a = attr + (size_t)g * 4;
-    a[0] = make_float4(u, vv, n0, n1);
-    a[1] = make_float4(n2, t0, t1, t2);
-    a[2] = make_float4(vx, vy, vz, cr);
-    a[3] = make_float4(cg, cb, 1.f / o3, 0.f);
-    if (density) {
-        // projected-vertex density feature of the capacity model (no parity role)
+    a[0] = make_float4(u, vv, 1.f / o3, n0);
+    a[1] = make_float4(n1, n2, t0, t1);
+    a[2] = make_float4(t2, vx, vy, vz);
+    a[3] = make_float4(cr, cg, cb, 0.f);
+    if (density && (v & 7u) == 0) {
+        // projected-vertex density feature of the capacity model (every 8th
+        // vertex; no parity role)
         const float sx = ((o0 + 1) / 2) * (float)fp.width;
         const float sy = ((1 - o1) / 2) * (float)fp.height;
-        if (o3 > 0.f && sx >= 0.f && sy >= 0.f && sx < (float)fp.width && sy < (float)fp.height) {
-            const uint32_t tile = ((uint32_t)sy / fp.tile) * fp.tiles_x + (uint32_t)sx / fp.tile;
-            agg_increment(density, tile);
-        }
+        if (o3 > 0.f && sx >= 0.f && sy >= 0.f && sx < (float)fp.width && sy < (float)fp.height)
+            count_tile(density, ((uint32_t)sy / fp.tile) * fp.tiles_x + (uint32_t)sx / fp.tile);
     }
 }
 
 // TA_Kernel + TB_Kernel (CCR/CUDARenderer.cu:702-796,903-988) on the CPU
-// path's rules (CCR/ERenderer.cpp:73-157, 603-624) with D4/D5.
+// path's rules (CCR/ERenderer.cpp:73-157, 603-624) with D4/D5, plus the
+// micro-triangle raster (RasterizeTriangle, CCR/CUDAROPs.cu:394-407, over the
+// whole bbox with the order-free key).
 template <bool SINGLE_PASS>
 __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshes,
                                                const uint32_t *__restrict__ block_mesh, FrameParams fp,
@@ -231,8 +335,12 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                                                uint32_t queue_cap, uint32_t *__restrict__ survivors,
                                                uint32_t *__restrict__ big, uint32_t big_cap,
                                                uint2 *__restrict__ ovf, uint32_t ovf_cap,
+                                               unsigned long long *__restrict__ keys,
                                                FrameCounters *__restrict__ ctr)
 {
+    __shared__ uint32_t lds_append[12];
+    __shared__ TriLds mstage[256];
+    __shared__ uint32_t moff[256];
     const uint32_t mi = block_mesh[blockIdx.x];
     const DevMesh &M = meshes[mi];
     const uint32_t k = (blockIdx.x - M.tri_block0) * 256u + threadIdx.x;
@@ -240,6 +348,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     const uint32_t gid = M.tri_base + k;
     bool ok = false;
     int xmin = 0, ymin = 0, xmax = 0, ymax = 0;
+    TriLds t;
     if (valid) {
         uint32_t i0, i1, i2;
         if (M.topology == 0) {
@@ -270,25 +379,26 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
         if (ok) {
             const float W = (float)fp.width, H = (float)fp.height;
             // NDCToScreenSpace (CCR/CUDAROPs.cu:88-97)
-            const float sx0 = ((P0.x + 1) / 2) * W, sy0 = ((1 - P0.y) / 2) * H;
-            const float sx1 = ((P1.x + 1) / 2) * W, sy1 = ((1 - P1.y) / 2) * H;
-            const float sx2 = ((P2.x + 1) / 2) * W, sy2 = ((1 - P2.y) / 2) * H;
+            t.sx0 = ((P0.x + 1) / 2) * W; t.sy0 = ((1 - P0.y) / 2) * H;
+            t.sx1 = ((P1.x + 1) / 2) * W; t.sy1 = ((1 - P1.y) / 2) * H;
+            t.sx2 = ((P2.x + 1) / 2) * W; t.sy2 = ((1 - P2.y) / 2) * H;
+            t.z0 = P0.z; t.z1 = P1.z; t.z2 = P2.z;
             // Cross(v0 - v1, v0 - v2) (CCR/CUDAROPs.cu:336)
-            const float ex = sx0 - sx1, ey = sy0 - sy1, fx2 = sx0 - sx2, fy2 = sy0 - sy2;
+            const float ex = t.sx0 - t.sx1, ey = t.sy0 - t.sy1, fx2 = t.sx0 - t.sx2, fy2 = t.sy0 - t.sy2;
             const float cr = ex * fy2 - ey * fx2;
             if (!(cr != 0.f) || !isfinite(cr)) ok = false;
             if (ok) {
-                const float area = fabsf(cr);
+                t.area = fabsf(cr);
                 // GetBoundingBox with D4 (CCR/CUDAROPs.cu:180-200)
-                float mnx = sx0, mxx = sx0, mny = sy0, mxy = sy0;
-                if (sx1 < mnx) mnx = sx1;
-                if (sx2 < mnx) mnx = sx2;
-                if (sx1 > mxx) mxx = sx1;
-                if (sx2 > mxx) mxx = sx2;
-                if (sy1 < mny) mny = sy1;
-                if (sy2 < mny) mny = sy2;
-                if (sy1 > mxy) mxy = sy1;
-                if (sy2 > mxy) mxy = sy2;
+                float mnx = t.sx0, mxx = t.sx0, mny = t.sy0, mxy = t.sy0;
+                if (t.sx1 < mnx) mnx = t.sx1;
+                if (t.sx2 < mnx) mnx = t.sx2;
+                if (t.sx1 > mxx) mxx = t.sx1;
+                if (t.sx2 > mxx) mxx = t.sx2;
+                if (t.sy1 < mny) mny = t.sy1;
+                if (t.sy2 < mny) mny = t.sy2;
+                if (t.sy1 > mxy) mxy = t.sy1;
+                if (t.sy2 > mxy) mxy = t.sy2;
                 xmin = clamp_trunc(mnx, W + 2.f) - 1;
                 ymin = clamp_trunc(mny, H + 2.f) - 1;
                 xmax = clamp_trunc(mxx, W + 2.f) + 1;
@@ -298,93 +408,161 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                 xmax = xmax > (int)fp.width ? (int)fp.width : xmax;
                 ymax = ymax > (int)fp.height ? (int)fp.height : ymax;
                 uint4 *r = reinterpret_cast<uint4 *>(trirec + (size_t)gid * kTriRecWords);
-                r[0] = make_uint4(__float_as_uint(sx0), __float_as_uint(sy0), __float_as_uint(sx1),
-                                  __float_as_uint(sy1));
-                r[1] = make_uint4(__float_as_uint(sx2), __float_as_uint(sy2), __float_as_uint(P0.z),
-                                  __float_as_uint(P1.z));
-                r[2] = make_uint4(__float_as_uint(P2.z), __float_as_uint(area),
+                r[0] = make_uint4(__float_as_uint(t.sx0), __float_as_uint(t.sy0), __float_as_uint(t.sx1),
+                                  __float_as_uint(t.sy1));
+                r[1] = make_uint4(__float_as_uint(t.sx2), __float_as_uint(t.sy2), __float_as_uint(t.z0),
+                                  __float_as_uint(t.z1));
+                r[2] = make_uint4(__float_as_uint(t.z2), __float_as_uint(t.area),
                                   (uint32_t)xmin | ((uint32_t)ymin << 16), (uint32_t)xmax | ((uint32_t)ymax << 16));
                 r[3] = make_uint4(i0, i1, i2, mi);
             }
         }
         alive[gid] = ok ? 1 : 0;
     }
-    const uint64_t am = __ballot(ok);
-    if (lane_id() == 0 && am) atomicAdd(&ctr->num_alive, (uint32_t)__popcll(am));
 
-    // binning: tile rectangle of the pixel range [xmin,xmax) x [ymin,ymax)
+    // classification by the pixel range [xmin,xmax) x [ymin,ymax)
     const bool nonempty = ok && xmin < xmax && ymin < ymax;
+    const uint32_t npix = nonempty ? (uint32_t)((xmax - xmin) * (ymax - ymin)) : 0u;
+    const bool micro = nonempty && npix <= fp.micro_limit;
     uint32_t tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0, span = 0;
-    if (nonempty) {
+    if (nonempty && !micro) {
         tx0 = (uint32_t)xmin / fp.tile; tx1 = (uint32_t)(xmax - 1) / fp.tile;
         ty0 = (uint32_t)ymin / fp.tile; ty1 = (uint32_t)(ymax - 1) / fp.tile;
         span = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
     }
-    const bool is_big = nonempty && span > fp.big_threshold;
-    const bool binned = nonempty && !is_big;
-    const uint32_t bslot = wave_append(&ctr->num_big, is_big);
-    if (is_big && bslot < big_cap) big[bslot] = gid;
-    if (is_big && bslot >= big_cap) ctr->storage_overflow = 1;
-    const uint32_t sslot = wave_append(&ctr->num_binned, binned);
-    if (binned) survivors[sslot] = gid;
+    const bool is_big = nonempty && !micro && span > fp.big_threshold;
+    const bool binned = nonempty && !micro && !is_big;
+
+    // Micro triangles: the wave rasterizes the union of their pixel ranges as
+    // one item list (prefix sum over the ranges), so a lane's work does not
+    // depend on its own triangle's size; fragments go to the key buffer with
+    // 64-bit atomic min.
+    {
+        const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+        const uint32_t area = micro ? npix : 0u;
+        uint32_t incl = area;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        if (total) {
+            TriLds *ws = mstage + wave * 64;
+            uint32_t *wo = moff + wave * 64;
+            if (micro) {
+                set_edges(t);
+                t.xr = (uint32_t)xmin | ((uint32_t)xmax << 16);
+                t.yr = (uint32_t)ymin | ((uint32_t)ymax << 16);
+                t.id = gid;
+                t.inv_w = 1.0f / (float)(xmax - xmin);
+                ws[lane] = t;
+            }
+            wo[lane] = incl - area;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t kk = lane; kk < total; kk += 64) {
+                const uint32_t j = find_item_owner(wo, 64, kk);
+                const TriLds &mt = ws[j];
+                int px, py;
+                item_pixel(mt, kk - wo[j], px, py);
+                float z;
+                if (fragment(mt, (float)px, (float)py, z))
+                    atomicMin(keys + (size_t)py * fp.width + px, (unsigned long long)make_key(z, mt.id));
+            }
+        }
+    }
+
+    uint32_t n_ok, bslot, sslot;
+    block_append3(&ctr->num_alive, ok, &ctr->num_big, is_big, &ctr->num_binned, binned, lds_append, n_ok, bslot,
+                  sslot);
+    (void)n_ok;
+    if (is_big) {
+        if (bslot < big_cap) big[bslot] = gid;
+        else ctr->storage_overflow = 1;
+    }
     if (binned) {
+        survivors[sslot] = gid;
         const uint32_t w = tx1 - tx0 + 1;
+        uint32_t tx = tx0, ty = ty0;
         for (uint32_t s = 0; s < span; ++s) {
-            const uint32_t t = (ty0 + s / w) * fp.tiles_x + tx0 + s % w;
-            const uint32_t slot = agg_increment(tile_count, t);
+            const uint32_t tl = ty * fp.tiles_x + tx;
             if (SINGLE_PASS) {
-                if (slot < tile_cap[t]) {
-                    const uint32_t pos = tile_off[t] + slot;
+                const uint32_t slot = claim_slot(tile_count, tl);
+                if (slot < tile_cap[tl]) {
+                    const uint32_t pos = tile_off[tl] + slot;
                     if (pos < queue_cap) queue[pos] = gid;
                 } else {
                     // capacity exceeded: keep the entry for the exact fallback
                     const uint32_t o = atomicAdd(&ctr->ovf_entries, 1u);
-                    if (o < ovf_cap) ovf[o] = make_uint2(t, gid);
+                    if (o < ovf_cap) ovf[o] = make_uint2(tl, gid);
                     else ctr->storage_overflow = 1;
                 }
+            } else {
+                count_tile(tile_count, tl);
             }
+            if (++tx == tx0 + w) { tx = tx0; ++ty; }
         }
     }
 }
 
-// Single-workgroup exclusive scan (n up to a few hundred thousand tiles).
-// out_off[n] = total.  is_capacity: in = capacities, the total is also the
+// Exclusive scan of n per-tile values by one 1024-thread workgroup:
+// out_off[n] = total.  is_capacity: in = capacities, the total is the
 // capacity sum; otherwise in = exact counts and the total is the pair count.
 __global__ __launch_bounds__(1024) void k_scan(uint32_t n, const uint32_t *__restrict__ in,
                                                uint32_t *__restrict__ out_off, uint32_t *__restrict__ zero_cursor,
                                                uint32_t limit, FrameCounters *__restrict__ ctr, int is_capacity)
 {
-    __shared__ uint32_t part[1024];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b = tid * per;
-    const uint32_t e = b + per < n ? b + per : n;
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += in[i];
-    part[tid] = s;
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid == 0) carry = 0;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = tid >= off ? part[tid - off] : 0;
+    // chunks of 4096 values: each thread owns 4 consecutive ones (one uint4)
+    for (uint32_t base = 0; base < n; base += 4096) {
+        const uint32_t i = base + tid * 4;
+        uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+        if (i + 3 < n) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(in + i);
+            v0 = q.x; v1 = q.y; v2 = q.z; v3 = q.w;
+        } else {
+            if (i < n) v0 = in[i];
+            if (i + 1 < n) v1 = in[i + 1];
+            if (i + 2 < n) v2 = in[i + 2];
+        }
+        const uint32_t s = v0 + v1 + v2 + v3;
+        uint32_t incl = s;  // inclusive wave prefix
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        if (lane == 63) wsum[wave] = incl;
         __syncthreads();
-        part[tid] += v;
+        uint32_t wbase = carry;
+        for (uint32_t w = 0; w < wave; ++w) wbase += wsum[w];
+        uint32_t run = wbase + incl - s;
+        if (i < n) { out_off[i] = run; run += v0; }
+        if (i + 1 < n) { out_off[i + 1] = run; run += v1; }
+        if (i + 2 < n) { out_off[i + 2] = run; run += v2; }
+        if (i + 3 < n) { out_off[i + 3] = run; }
+        if (zero_cursor) {
+            if (i + 3 < n) *reinterpret_cast<uint4 *>(zero_cursor + i) = make_uint4(0, 0, 0, 0);
+            else for (uint32_t q = i; q < n && q < i + 4; ++q) zero_cursor[q] = 0;
+        }
+        __syncthreads();
+        if (tid == 1023) carry = wbase + incl;
         __syncthreads();
     }
-    uint32_t run = tid ? part[tid - 1] : 0;
-    for (uint32_t i = b; i < e; ++i) {
-        out_off[i] = run;
-        run += in[i];
-        if (zero_cursor) zero_cursor[i] = 0;
-    }
-    if (tid == 1023) {
-        out_off[n] = part[1023];
-        if (is_capacity) ctr->capacity_total = part[1023];
-        else ctr->total_pairs = part[1023];
-        if (part[1023] > limit) ctr->storage_overflow = 1;
+    if (tid == 0) {
+        out_off[n] = carry;
+        if (is_capacity) ctr->capacity_total = carry;
+        else ctr->total_pairs = carry;
+        if (carry > limit) ctr->storage_overflow = 1;
     }
 }
 
-// Exact second pass over the binned survivors (the order inside a queue is
-// irrelevant: the raster resolves depth order-independently).
+// Exact second pass over the binned triangles (the order inside a queue is
+// irrelevant: depth is resolved order-independently).
 __global__ __launch_bounds__(256) void k_fill(FrameParams fp, const uint32_t *__restrict__ survivors,
                                               const uint32_t *__restrict__ trirec,
                                               uint32_t *__restrict__ cursor, const uint32_t *__restrict__ tile_off,
@@ -392,21 +570,18 @@ __global__ __launch_bounds__(256) void k_fill(FrameParams fp, const uint32_t *__
                                               const FrameCounters *__restrict__ ctr)
 {
     const uint32_t n = ctr->num_binned;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i - threadIdx.x < n; i += gridDim.x * 256u) {
-        if (i < n) {
-            const uint32_t gid = survivors[i];
-            const uint2 bb = *reinterpret_cast<const uint2 *>(trirec + (size_t)gid * kTriRecWords + 10);
-            const uint32_t xmin = bb.x & 0xFFFFu, ymin = bb.x >> 16, xmax = bb.y & 0xFFFFu, ymax = bb.y >> 16;
-            const uint32_t tx0 = xmin / fp.tile, tx1 = (xmax - 1) / fp.tile;
-            const uint32_t ty0 = ymin / fp.tile, ty1 = (ymax - 1) / fp.tile;
-            const uint32_t w = tx1 - tx0 + 1, span = w * (ty1 - ty0 + 1);
-            for (uint32_t s = 0; s < span; ++s) {
-                const uint32_t t = (ty0 + s / w) * fp.tiles_x + tx0 + s % w;
-                const uint32_t slot = agg_increment(cursor, t);
-                const uint32_t pos = tile_off[t] + slot;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const uint32_t gid = survivors[i];
+        const uint2 bb = *reinterpret_cast<const uint2 *>(trirec + (size_t)gid * kTriRecWords + 10);
+        const uint32_t xmin = bb.x & 0xFFFFu, ymin = bb.x >> 16, xmax = bb.y & 0xFFFFu, ymax = bb.y >> 16;
+        const uint32_t tx0 = xmin / fp.tile, tx1 = (xmax - 1) / fp.tile;
+        const uint32_t ty0 = ymin / fp.tile, ty1 = (ymax - 1) / fp.tile;
+        for (uint32_t ty = ty0; ty <= ty1; ++ty)
+            for (uint32_t tx = tx0; tx <= tx1; ++tx) {
+                const uint32_t t = ty * fp.tiles_x + tx;
+                const uint32_t pos = tile_off[t] + claim_slot(cursor, t);
                 if (pos < queue_cap) queue[pos] = gid;
             }
-        }
     }
 }
 
@@ -462,30 +637,54 @@ __global__ __launch_bounds__(256) void k_fallback_fill(const uint2 *__restrict__
     }
 }
 
+// Conservative depth bound of t: every computed fragment z of t exceeds the
+// returned value, so t loses at any pixel whose current depth d < bound.
+// Only for well-conditioned triangles (|area| >= 0.1 * longest edge^2, all
+// z > 0): each computed barycentric is then within ~62 ulp of the exact one
+// and every computed fragment z is >= min(z) * (1 - 3e-5); the 2e-3 margin
+// keeps the bound strictly below.  Returns -1 (no rejection) otherwise.
+__device__ __forceinline__ float cull_depth(const TriLds &t)
+{
+    const float zlo = fminf(t.z0, fminf(t.z1, t.z2));
+    if (!(zlo > 0.f)) return -1.f;
+    const float ea = t.ax * t.ax + t.ay * t.ay, eb = t.bx * t.bx + t.by * t.by, ec = t.cx * t.cx + t.cy * t.cy;
+    const float emax = fmaxf(ea, fmaxf(eb, ec));
+    if (!(t.area >= 0.1f * emax)) return -1.f;
+    return zlo * (1.0f - 2.0e-3f);
+}
+
 // RA_PerTileKernel (CCR/CUDARenderer.cu:1039-1129) redesigned: one workgroup
-// per tile, depth + visibility resolved in LDS with 64-bit (z, ~id) keys so
-// the queue order does not matter.  Small footprints are rasterized one
-// triangle per lane (LDS atomic min); larger ones are staged in LDS and
-// rasterized one pixel per lane.
+// per tile, each lane owning PPT pixels.  The lane loads its pixels' keys
+// (micro triangles, already rasterized at setup) and resets them for the
+// next frame.  The tile's queued triangles (queue region + fallback pool
+// region) are rasterized in LDS: a triangle covering at least small_limit
+// pixels of the tile one pixel per lane, the others as one compacted item
+// list (prefix sum over their clipped pixel ranges, LDS atomic min).  The few
+// screen-list triangles are then tested by every lane on its own pixels, with
+// a conservative per-pixel depth reject before the exact test.
 template <int T, int NT>
-__global__ __launch_bounds__(NT) void k_raster(FrameParams fp, const uint32_t *__restrict__ tile_count,
-                                               const uint32_t *__restrict__ tile_off,
-                                               const uint32_t *__restrict__ tile_cap,
-                                               const uint32_t *__restrict__ tile_pool,
-                                               const uint32_t *__restrict__ queue, uint32_t queue_cap,
-                                               const uint32_t *__restrict__ pool, uint32_t pool_cap,
-                                               const uint32_t *__restrict__ big, uint32_t big_cap,
-                                               const FrameCounters *__restrict__ ctr,
-                                               const uint32_t *__restrict__ trirec, uint32_t *__restrict__ vis,
-                                               float *__restrict__ depth, float *__restrict__ tile_min_depth,
-                                               uint32_t small_limit, uint32_t grid_tiles)
+__global__ __launch_bounds__(NT) void k_resolve(FrameParams fp, const uint32_t *__restrict__ tile_count,
+                                                const uint32_t *__restrict__ tile_off,
+                                                const uint32_t *__restrict__ tile_cap,
+                                                const uint32_t *__restrict__ tile_pool,
+                                                const uint32_t *__restrict__ queue, uint32_t queue_cap,
+                                                const uint32_t *__restrict__ pool, uint32_t pool_cap,
+                                                const uint32_t *__restrict__ big, uint32_t big_cap,
+                                                const FrameCounters *__restrict__ ctr,
+                                                const uint32_t *__restrict__ trirec,
+                                                unsigned long long *__restrict__ keys, uint32_t *__restrict__ vis,
+                                                float *__restrict__ depth, float *__restrict__ tile_min_depth,
+                                                uint32_t small_limit, uint32_t grid_tiles)
 {
     constexpr int NPIX = T * T;
     constexpr int PPT = NPIX / NT;
+    constexpr int NW = NT / 64;
     static_assert(NPIX % NT == 0, "tile pixels must be a multiple of the workgroup size");
     __shared__ uint64_t key[NPIX];
     __shared__ TriLds stage[NT];
-    __shared__ uint32_t nstage[2];
+    __shared__ uint32_t off[NT];
+    __shared__ uint32_t wlist[NT];
+    __shared__ uint32_t wred_u[NW], wwhole[NW];
 
     const uint32_t ntiles = fp.tiles_x * fp.tiles_y;
     // XCD-aware order: consecutive workgroups go round-robin over the 8 XCDs,
@@ -493,18 +692,33 @@ __global__ __launch_bounds__(NT) void k_raster(FrameParams fp, const uint32_t *_
     const uint32_t per_xcd = grid_tiles / 8u;
     const uint32_t tile = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
     if (tile >= ntiles) return;
-    const uint32_t tx = tile % fp.tiles_x, ty = tile / fp.tiles_x;
-    const int X0 = (int)(tx * T), Y0 = (int)(ty * T);
-    const uint32_t tid = threadIdx.x;
+    const uint32_t txi = tile % fp.tiles_x, tyi = tile / fp.tiles_x;
+    const int X0 = (int)(txi * T), Y0 = (int)(tyi * T);
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
 
-    for (int p = tid; p < NPIX; p += NT) key[p] = kClearKey;
-    if (tid == 0) { nstage[0] = 0; nstage[1] = 0; }
+    // the lane's pixels: p = tid + q * NT (rows of T pixels)
+    uint64_t best[PPT];
+    int pxs[PPT], pys[PPT];
+    bool inside[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int p = (int)tid + q * NT;
+        pxs[q] = X0 + p % T;
+        pys[q] = Y0 + p / T;
+        inside[q] = pxs[q] < (int)fp.width && pys[q] < (int)fp.height;
+        best[q] = kClearKey;
+        if (inside[q]) {
+            const size_t o = (size_t)pys[q] * fp.width + pxs[q];
+            best[q] = keys[o];
+            keys[o] = kClearKey;
+        }
+    }
 
     // list A: the tile's queue region; list B: its fallback pool region
     const uint32_t cnt = tile_count[tile];
     uint32_t n_a = cnt, n_b = 0;
-    const uint32_t off = tile_off[tile];
-    const uint32_t *list_a = queue + off;
+    const uint32_t qoff = tile_off[tile];
+    const uint32_t *list_a = queue + qoff;
     const uint32_t *list_b = pool;
     if (tile_cap) {
         const uint32_t cp = tile_cap[tile];
@@ -519,114 +733,161 @@ __global__ __launch_bounds__(NT) void k_raster(FrameParams fp, const uint32_t *_
             }
         }
     }
-    const uint32_t room_a = off < queue_cap ? queue_cap - off : 0;
+    const uint32_t room_a = qoff < queue_cap ? queue_cap - qoff : 0;
     n_a = n_a < room_a ? n_a : room_a;
     const uint32_t n_tile = n_a + n_b;
-    uint32_t n_big = ctr->num_big;
-    n_big = n_big < big_cap ? n_big : big_cap;
-    const uint32_t total = n_tile + n_big;
     const uint32_t num_tris = fp.num_tris;
-    __syncthreads();
 
-    // rows of the tile owned by this wave in the pixel-parallel phase
-    const int wave = (int)(tid / 64u);
-    uint32_t parity = 0;
-    for (uint32_t base = 0; base < total; base += NT, parity ^= 1u) {
-        const uint32_t i = base + tid;
-        if (i < total) {
-            const uint32_t id = i < n_a ? list_a[i] : (i < n_tile ? list_b[i - n_a] : big[i - n_tile]);
-            if (id < num_tris) {
-                TriLds t;
-                load_tri(trirec, id, t);
-                const int bx0 = (int)(t.xr & 0xFFFFu), by0 = (int)(t.xr >> 16);
-                const int bx1 = (int)(t.yr & 0xFFFFu), by1 = (int)(t.yr >> 16);
-                const int x0 = bx0 > X0 ? bx0 : X0, x1 = bx1 < X0 + T ? bx1 : X0 + T;
-                const int y0 = by0 > Y0 ? by0 : Y0, y1 = by1 < Y0 + T ? by1 : Y0 + T;
-                if (x0 < x1 && y0 < y1) {
-                    const uint32_t npix = (uint32_t)((x1 - x0) * (y1 - y0));
-                    if (npix <= small_limit) {
-                        for (int py = y0; py < y1; ++py)
-                            for (int px = x0; px < x1; ++px) {
-                                float z;
-                                if (fragment(t, (float)px, (float)py, z))
-                                    atomicMin((unsigned long long *)&key[(py - Y0) * T + (px - X0)],
-                                              (unsigned long long)make_key(z, id));
-                            }
-                    } else {
+    if (n_tile) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) key[(int)tid + q * NT] = best[q];
+        for (uint32_t base = 0; base < n_tile; base += NT) {
+            const uint32_t i = base + tid;
+            uint32_t area = 0;
+            bool whole = false;
+            TriLds t;
+            t.xr = 0; t.yr = 0; t.inv_w = 0.f;
+            if (i < n_tile) {
+                const uint32_t id = i < n_a ? list_a[i] : list_b[i - n_a];
+                if (id < num_tris) {
+                    load_tri(trirec, id, t);
+                    const int bx0 = (int)(t.xr & 0xFFFFu), by0 = (int)(t.xr >> 16);
+                    const int bx1 = (int)(t.yr & 0xFFFFu), by1 = (int)(t.yr >> 16);
+                    const int x0 = bx0 > X0 ? bx0 : X0, x1 = bx1 < X0 + T ? bx1 : X0 + T;
+                    const int y0 = by0 > Y0 ? by0 : Y0, y1 = by1 < Y0 + T ? by1 : Y0 + T;
+                    if (x0 < x1 && y0 < y1) {
                         t.xr = (uint32_t)x0 | ((uint32_t)x1 << 16);
                         t.yr = (uint32_t)y0 | ((uint32_t)y1 << 16);
-                        const uint32_t slot = atomicAdd(&nstage[parity], 1u);
-                        stage[slot] = t;
+                        t.inv_w = 1.0f / (float)(x1 - x0);
+                        area = (uint32_t)((x1 - x0) * (y1 - y0));
+                        if (area >= small_limit) { whole = true; area = 0; }
                     }
                 }
             }
-        }
-        __syncthreads();
-        const uint32_t ns = nstage[parity];
-        if (tid == 0) nstage[parity ^ 1u] = 0;
-        if (ns) {
-            uint64_t best[PPT];
-            int pxs[PPT], pys[PPT];
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int p = (int)tid + q * NT;
-                pxs[q] = X0 + p % T;
-                pys[q] = Y0 + p / T;
-                best[q] = key[p];
+            stage[tid] = t;
+            // block-wide exclusive scan of the item counts
+            uint32_t incl = area;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                if (lane >= (uint32_t)o) incl += y;
             }
-            // wave-uniform row range of this wave's pixels
-            const int wy0 = Y0 + (wave * 64) / T;
-            const int wy1 = Y0 + ((wave * 64) + (PPT - 1) * NT + 63) / T + 1;
-            for (uint32_t j = 0; j < ns; ++j) {
-                const TriLds &t = stage[j];
-                const int x0 = (int)(t.xr & 0xFFFFu), x1 = (int)(t.xr >> 16);
-                const int y0 = (int)(t.yr & 0xFFFFu), y1 = (int)(t.yr >> 16);
-                if (PPT == 1 && (y1 <= wy0 || y0 >= wy1)) continue;
+            if (lane == 63) wred_u[wave] = incl;
+            const uint64_t wm = __ballot(whole);
+            if (lane == 0) wwhole[wave] = (uint32_t)__popcll(wm);
+            __syncthreads();
+            uint32_t before = 0, nitems = 0, pbefore = 0, nwhole = 0;
+            for (int w = 0; w < NW; ++w) {
+                if (w < (int)wave) { before += wred_u[w]; pbefore += wwhole[w]; }
+                nitems += wred_u[w];
+                nwhole += wwhole[w];
+            }
+            off[tid] = before + incl - area;
+            if (whole) wlist[pbefore + popc_below(wm)] = tid;
+            __syncthreads();
+            // compacted items
+            for (uint32_t kk = tid; kk < nitems; kk += NT) {
+                const uint32_t j = find_item_owner(off, NT, kk);
+                const TriLds &mt = stage[j];
+                int px, py;
+                item_pixel(mt, kk - off[j], px, py);
+                float z;
+                if (fragment(mt, (float)px, (float)py, z))
+                    atomicMin((unsigned long long *)&key[(py - Y0) * T + (px - X0)],
+                              (unsigned long long)make_key(z, mt.id));
+            }
+            if (nwhole) {
+                __syncthreads();
 #pragma unroll
-                for (int q = 0; q < PPT; ++q) {
-                    const int px = pxs[q], py = pys[q];
-                    if (px >= x0 && px < x1 && py >= y0 && py < y1) {
-                        float z;
-                        if (fragment(t, (float)px, (float)py, z)) {
-                            const uint64_t kk = make_key(z, t.id);
-                            best[q] = kk < best[q] ? kk : best[q];
+                for (int q = 0; q < PPT; ++q) best[q] = key[(int)tid + q * NT];
+                // wave-uniform row range of this wave's pixels (PPT == 1)
+                const int wy0 = Y0 + (int)(wave * 64) / T;
+                const int wy1 = Y0 + ((int)(wave * 64) + 63) / T + 1;
+                for (uint32_t q2 = 0; q2 < nwhole; ++q2) {
+                    const TriLds &st = stage[wlist[q2]];
+                    const int x0 = (int)(st.xr & 0xFFFFu), x1 = (int)(st.xr >> 16);
+                    const int y0 = (int)(st.yr & 0xFFFFu), y1 = (int)(st.yr >> 16);
+                    if (PPT == 1 && (y1 <= wy0 || y0 >= wy1)) continue;
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) {
+                        const int px = pxs[q], py = pys[q];
+                        if (px >= x0 && px < x1 && py >= y0 && py < y1) {
+                            float z;
+                            if (fragment(st, (float)px, (float)py, z)) {
+                                const uint64_t kk = make_key(z, st.id);
+                                best[q] = kk < best[q] ? kk : best[q];
+                            }
                         }
                     }
                 }
-            }
 #pragma unroll
-            for (int q = 0; q < PPT; ++q) key[(int)tid + q * NT] = best[q];
+                for (int q = 0; q < PPT; ++q) key[(int)tid + q * NT] = best[q];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) best[q] = key[(int)tid + q * NT];
+    }
+
+    // screen-list triangles: staged in LDS with one parallel load per chunk,
+    // then every lane tests its own pixels
+    uint32_t n_big = ctr->num_big;
+    n_big = n_big < big_cap ? n_big : big_cap;
+    for (uint32_t base = 0; base < n_big; base += NT) {
+        const uint32_t i = base + tid;
+        if (i < n_big) {
+            TriLds t;
+            const uint32_t id = big[i];
+            if (id < num_tris) {
+                load_tri(trirec, id, t);
+                t.inv_w = cull_depth(t);
+            } else {
+                t.xr = 0; t.yr = 0;  // empty range
+            }
+            stage[tid] = t;
+        }
+        __syncthreads();
+        const uint32_t nb = n_big - base < (uint32_t)NT ? n_big - base : (uint32_t)NT;
+        for (uint32_t j = 0; j < nb; ++j) {
+            const TriLds &st = stage[j];
+            const int x0 = (int)(st.xr & 0xFFFFu), y0 = (int)(st.xr >> 16);
+            const int x1 = (int)(st.yr & 0xFFFFu), y1 = (int)(st.yr >> 16);
+            if (x1 <= X0 || x0 >= X0 + T || y1 <= Y0 || y0 >= Y0 + T) continue;
+            const float zc = st.inv_w;  // cull depth
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int px = pxs[q], py = pys[q];
+                if (px >= x0 && px < x1 && py >= y0 && py < y1 &&
+                    !(zc > __uint_as_float((uint32_t)(best[q] >> 32)))) {
+                    float z;
+                    if (fragment(st, (float)px, (float)py, z)) {
+                        const uint64_t kk = make_key(z, st.id);
+                        best[q] = kk < best[q] ? kk : best[q];
+                    }
+                }
+            }
         }
         __syncthreads();
     }
 
-    // write out (coalesced rows of T pixels)
+    // write out and the tile's min depth (a feature of the capacity model)
     float dmin = 1.0f;
-    for (int p = tid; p < NPIX; p += NT) {
-        const int px = X0 + p % T, py = Y0 + p / T;
-        const uint64_t kk = key[p];
-        const float d = __uint_as_float((uint32_t)(kk >> 32));
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const float d = __uint_as_float((uint32_t)(best[q] >> 32));
         dmin = d < dmin ? d : dmin;
-        if (px < (int)fp.width && py < (int)fp.height) {
-            const size_t o = (size_t)py * fp.width + px;
-            vis[o] = ~(uint32_t)kk - 1u;
+        if (inside[q]) {
+            const size_t o = (size_t)pys[q] * fp.width + pxs[q];
+            vis[o] = ~(uint32_t)best[q] - 1u;
             depth[o] = d;
         }
     }
     if (tile_min_depth) {
-        // per-tile min depth (feature of the capacity model); wave min then LDS
-        for (int off = 32; off > 0; off >>= 1) {
-            const float o = __shfl_xor(dmin, off);
-            dmin = o < dmin ? o : dmin;
+        for (int o = 32; o > 0; o >>= 1) {
+            const float y = __shfl_xor(dmin, o);
+            dmin = y < dmin ? y : dmin;
         }
-        __shared__ float wmin[NT / 64 > 0 ? NT / 64 : 1];
-        if ((tid & 63u) == 0) wmin[tid / 64u] = dmin;
-        __syncthreads();
-        if (tid == 0) {
-            float m = wmin[0];
-            for (int w = 1; w < NT / 64; ++w) m = wmin[w] < m ? wmin[w] : m;
-            tile_min_depth[tile] = m;
-        }
+        // depths are >= 0, so their bit patterns order like the values
+        if (lane == 0) atomicMin(reinterpret_cast<uint32_t *>(tile_min_depth) + tile, __float_as_uint(dmin));
     }
 }
 
@@ -656,8 +917,9 @@ __device__ __forceinline__ bool texel_coord(float uv, uint32_t n, int &out)
 }
 
 // CUDATextureSampler::SamplePoint / SampleLinear (CCR/CUDATextureSampler.cuh:86-190).
-__device__ __forceinline__ void sample(const DevTexture &t, float u, float v, int mode, float &r, float &g,
-                                       float &b)
+// unit[c] = c / 255.f (the same correctly rounded quotient as the reference).
+__device__ __forceinline__ void sample(const DevTexture &t, float u, float v, int mode, const float *unit, float &r,
+                                       float &g, float &b)
 {
     int x, y;
     if (!texel_coord(u, t.w, x) || !texel_coord(v, t.h, y)) { r = 1.f; g = 0.f; b = 1.f; return; }
@@ -665,9 +927,9 @@ __device__ __forceinline__ void sample(const DevTexture &t, float u, float v, in
     const int orig = x + y * w;
     if (mode == 0) {
         const uint32_t s = t.texels[orig];
-        r = (float)(s & 0xFFu) / 255.f;
-        g = (float)((s >> 8) & 0xFFu) / 255.f;
-        b = (float)((s >> 16) & 0xFFu) / 255.f;
+        r = unit[s & 0xFFu];
+        g = unit[(s >> 8) & 0xFFu];
+        b = unit[(s >> 16) & 0xFFu];
         return;
     }
     const int texSize = w * (int)t.h;
@@ -695,14 +957,21 @@ __device__ __forceinline__ void sample(const DevTexture &t, float u, float v, in
 
 // PS_Kernel + ShadePixelUnsafe (CCR/CUDARenderer.cu:833-897, CCR/CUDAROPs.cu:472-514)
 // with the attribute interpolation of RasterizePixelAtomic (CCR/CUDAROPs.cu:347-380).
+// The barycentrics, 1/w and the texture coordinates are computed exactly as
+// in the reference (so the same texels are fetched); the lighting terms use
+// the hardware reciprocal / rsqrt / exp2 / log2, which keeps RGBA8 within
+// 1/255 of the reference formula.  64x4-pixel blocks.
 __global__ __launch_bounds__(256) void k_shade(FrameParams fp, const DevMesh *__restrict__ meshes,
                                                const uint32_t *__restrict__ vis, const float *__restrict__ depth,
                                                const uint32_t *__restrict__ trirec,
                                                const float4 *__restrict__ attr, uint32_t *__restrict__ rgba)
 {
-    const uint32_t x = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t y = blockIdx.y;
-    if (x >= fp.width) return;
+    __shared__ float unit[256];
+    unit[threadIdx.x] = (float)threadIdx.x / 255.f;
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 64u + (threadIdx.x & 63u);
+    const uint32_t y = blockIdx.y * 4u + (threadIdx.x >> 6);
+    if (x >= fp.width || y >= fp.height) return;
     const size_t pid = (size_t)y * fp.width + x;
     const uint32_t id = vis[pid];
     if (id >= fp.num_tris) {
@@ -721,17 +990,16 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams fp, const DevMesh *__
     float w1 = (px - sx2) * (sy0 - sy2) - (py - sy2) * (sx0 - sx2);
     w0 = w0 / area; w1 = w1 / area; w2 = w2 / area;
     const float4 *A0 = attr + (size_t)q3.x * 4, *A1 = attr + (size_t)q3.y * 4, *A2 = attr + (size_t)q3.z * 4;
-    const float4 a00 = A0[0], a01 = A0[1], a02 = A0[2], a03 = A0[3];
-    const float4 a10 = A1[0], a11 = A1[1], a12 = A1[2], a13 = A1[3];
-    const float4 a20 = A2[0], a21 = A2[1], a22 = A2[2], a23 = A2[3];
-    const float i0 = a03.z, i1 = a13.z, i2 = a23.z;
+    const float4 a00 = A0[0], a10 = A1[0], a20 = A2[0];
+    const float4 a01 = A0[1], a11 = A1[1], a21 = A2[1];
+    const float i0 = a00.z, i1 = a10.z, i2 = a20.z;
     const float wI = 1.f / (i0 * w0 + i1 * w1 + i2 * w2);
 #define RAST_INTERP(c0, c1, c2) (w0 * ((c0) * i0) + w1 * ((c1) * i1) + w2 * ((c2) * i2))
     const float u = RAST_INTERP(a00.x, a10.x, a20.x) * wI;
     const float v = RAST_INTERP(a00.y, a10.y, a20.y) * wI;
-    const float nx = RAST_INTERP(a00.z, a10.z, a20.z) * wI;
-    const float ny = RAST_INTERP(a00.w, a10.w, a20.w) * wI;
-    const float nz = RAST_INTERP(a01.x, a11.x, a21.x) * wI;
+    const float nx = RAST_INTERP(a00.w, a10.w, a20.w) * wI;
+    const float ny = RAST_INTERP(a01.x, a11.x, a21.x) * wI;
+    const float nz = RAST_INTERP(a01.y, a11.y, a21.y) * wI;
     if (fp.visualisation == 1) {  // depth colour, Remap(z, 0.985, 1)
         rgba[pid] = pack_rgba((depth[pid] - 0.985f) / (1.f - 0.985f), 0.f, 0.f);
         return;
@@ -742,48 +1010,52 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams fp, const DevMesh *__
     }
     const DevMesh &M = meshes[q3.w < fp.num_meshes ? q3.w : 0];
     if (!M.has_diffuse) {  // vertex colour (perspective-weighted, not x w)
-        rgba[pid] = pack_rgba(RAST_INTERP(a02.w, a12.w, a22.w), RAST_INTERP(a03.x, a13.x, a23.x),
-                              RAST_INTERP(a03.y, a13.y, a23.y));
+        const float4 a03 = A0[3], a13 = A1[3], a23 = A2[3];
+        rgba[pid] = pack_rgba(RAST_INTERP(a03.x, a13.x, a23.x), RAST_INTERP(a03.y, a13.y, a23.y),
+                              RAST_INTERP(a03.z, a13.z, a23.z));
         return;
     }
-    const float tx = RAST_INTERP(a01.y, a11.y, a21.y);
-    const float ty = RAST_INTERP(a01.z, a11.z, a21.z);
-    const float tz = RAST_INTERP(a01.w, a11.w, a21.w);
-    float vx = RAST_INTERP(a02.x, a12.x, a22.x);
-    float vy = RAST_INTERP(a02.y, a12.y, a22.y);
-    float vz = RAST_INTERP(a02.z, a12.z, a22.z);
+    const float4 a02 = A0[2], a12 = A1[2], a22 = A2[2];
+    const float tx = RAST_INTERP(a01.z, a11.z, a21.z);
+    const float ty = RAST_INTERP(a01.w, a11.w, a21.w);
+    const float tz = RAST_INTERP(a02.x, a12.x, a22.x);
+    float vx = RAST_INTERP(a02.y, a12.y, a22.y);
+    float vy = RAST_INTERP(a02.z, a12.z, a22.z);
+    float vz = RAST_INTERP(a02.w, a12.w, a22.w);
 #undef RAST_INTERP
-    normalize3(vx, vy, vz);
+    const float vl = vx * vx + vy * vy + vz * vz;
+    const float vinv = vl > 0.f ? __builtin_amdgcn_rsqf(vl) : 0.f;
+    vx *= vinv; vy *= vinv; vz *= vinv;
     float dr, dg, db, nr, ng, nb, sr, sg, sb, gr, gg, gb;
-    sample(M.tex[0], u, v, fp.sample_state, dr, dg, db);
-    sample(M.tex[1], u, v, fp.sample_state, nr, ng, nb);
-    sample(M.tex[2], u, v, fp.sample_state, sr, sg, sb);
-    sample(M.tex[3], u, v, fp.sample_state, gr, gg, gb);
+    sample(M.tex[0], u, v, fp.sample_state, unit, dr, dg, db);
+    sample(M.tex[1], u, v, fp.sample_state, unit, nr, ng, nb);
+    sample(M.tex[2], u, v, fp.sample_state, unit, sr, sg, sb);
+    sample(M.tex[3], u, v, fp.sample_state, unit, gr, gg, gb);
     (void)gg; (void)gb;
     // binormal = -Cross(tan, n); N = {tan, binormal, n} * (2 * sample - 1)
     const float cx = ty * nz - tz * ny, cy = tz * nx - tx * nz, cz = tx * ny - ty * nx;
-    const float bx = -cx, by = -cy, bz = -cz;
     const float fx = 2.f * nr - 1.f, fy = 2.f * ng - 1.f, fz = 2.f * nb - 1.f;
-    const float Nx = tx * fx + bx * fy + nx * fz;
-    const float Ny = ty * fx + by * fy + ny * fz;
-    const float Nz = tz * fx + bz * fy + nz * fz;
+    const float Nx = tx * fx - cx * fy + nx * fz;
+    const float Ny = ty * fx - cy * fy + ny * fz;
+    const float Nz = tz * fx - cz * fy + nz * fz;
     const float Lx = 0.577f, Ly = -0.577f, Lz = -0.577f;
-    float oa = (-Nx) * Lx + (-Ny) * Ly + (-Nz) * Lz;  // ClampFast(x, 0, x) leaves x unchanged
-    oa /= 3.14159265358979323846f;
-    oa *= 7.0f;
+    // observed area (ClampFast(x, 0, x) leaves x unchanged) * intensity / pi
+    const float oa = ((-Nx) * Lx + (-Ny) * Ly + (-Nz) * Lz) * (7.0f / 3.14159265358979323846f);
     const float twoDot = 2.f * (Lx * Nx + Ly * Ny + Lz * Nz);
     const float rx = -Lx + twoDot * Nx, ry = -Ly + twoDot * Ny, rz = -Lz + twoDot * Nz;
     float angle = vx * rx + vy * ry + vz * rz;
     angle = angle < 0.f ? 0.f : angle;
     angle = angle > 1.f ? 1.f : angle;
-    angle = powf(angle, gr * 25.f);
+    // powf(angle, gloss * 25) via exp2(e * log2(angle)); pow(x, 0) = 1
+    const float e = gr * 25.f;
+    angle = e == 0.f ? 1.f : (angle > 0.f ? __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(angle)) : 0.f);
     float outr = 0.05f + dr * oa + sr * angle;
     float outg = 0.05f + dg * oa + sg * angle;
     float outb = 0.05f + db * oa + sb * angle;
     const float mgb = outg < outb ? outb : outg;
     const float mx = outr < mgb ? mgb : outr;
     if (mx > 1.f) {
-        const float rev = 1.0f / mx;
+        const float rev = __builtin_amdgcn_rcpf(mx);
         outr *= rev; outg *= rev; outb *= rev;
     }
     rgba[pid] = pack_rgba(outr, outg, outb);
@@ -798,7 +1070,13 @@ hipError_t launch_frame_init(const FrameParams &fp, const FrameBuffers &b, bool 
     const uint32_t n = fp.tiles_x * fp.tiles_y;
     const uint32_t nn = n > 64 ? n : 64;
     hipLaunchKernelGGL(k_frame_init, dim3(ceil_div(nn, 256)), dim3(256), 0, s, b.tile_count,
-                       single_pass ? b.tile_pool : nullptr, n, b.counters, b.density);
+                       single_pass ? b.tile_pool : nullptr, n, b.counters, b.density, b.tile_min_depth);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clear_keys, dim3(2048), dim3(256), 0, s, keys, n);
     return hipGetLastError();
 }
 
@@ -816,11 +1094,11 @@ hipError_t launch_setup(const FrameParams &fp, const FrameBuffers &b, bool singl
     if (single_pass)
         hipLaunchKernelGGL(k_setup<true>, dim3(b.tri_blocks), dim3(256), 0, s, b.meshes, b.tri_block_mesh, fp, b.pos4,
                            b.trirec, b.alive, b.tile_count, b.tile_off, b.tile_cap, b.queue, b.queue_cap,
-                           b.survivors, b.big, b.big_cap, b.ovf, b.ovf_cap, b.counters);
+                           b.survivors, b.big, b.big_cap, b.ovf, b.ovf_cap, b.keys, b.counters);
     else
         hipLaunchKernelGGL(k_setup<false>, dim3(b.tri_blocks), dim3(256), 0, s, b.meshes, b.tri_block_mesh, fp,
                            b.pos4, b.trirec, b.alive, b.tile_count, b.tile_off, b.tile_cap, b.queue, b.queue_cap,
-                           b.survivors, b.big, b.big_cap, b.ovf, b.ovf_cap, b.counters);
+                           b.survivors, b.big, b.big_cap, b.ovf, b.ovf_cap, b.keys, b.counters);
     return hipGetLastError();
 }
 
@@ -835,7 +1113,7 @@ hipError_t launch_scan(uint32_t n, const uint32_t *in, uint32_t *out_off, uint32
 hipError_t launch_fill(const FrameParams &fp, const FrameBuffers &b, hipStream_t s)
 {
     const uint32_t blocks = ceil_div(fp.num_tris > 0 ? fp.num_tris : 1, 256);
-    hipLaunchKernelGGL(k_fill, dim3(blocks < 2048 ? blocks : 2048), dim3(256), 0, s, fp, b.survivors, b.trirec,
+    hipLaunchKernelGGL(k_fill, dim3(blocks < 1024 ? blocks : 1024), dim3(256), 0, s, fp, b.survivors, b.trirec,
                        b.tile_cursor, b.tile_off, b.queue, b.queue_cap, b.counters);
     return hipGetLastError();
 }
@@ -856,31 +1134,22 @@ hipError_t launch_raster(const FrameParams &fp, const FrameBuffers &b, uint32_t 
 {
     const uint32_t n = fp.tiles_x * fp.tiles_y;
     const uint32_t grid = ceil_div(n, 8) * 8;
+#define RAST_RESOLVE_ARGS                                                                                        \
+    fp, b.tile_count, b.tile_off, b.tile_cap, b.tile_pool, b.queue, b.queue_cap, b.pool, b.pool_cap, b.big,     \
+        b.big_cap, b.counters, b.trirec, b.keys, b.vis, b.depth, b.tile_min_depth, small_limit, grid
     switch (fp.tile) {
-    case 8:
-        hipLaunchKernelGGL((k_raster<8, 64>), dim3(grid), dim3(64), 0, s, fp, b.tile_count, b.tile_off, b.tile_cap,
-                           b.tile_pool, b.queue, b.queue_cap, b.pool, b.pool_cap, b.big, b.big_cap, b.counters, b.trirec, b.vis,
-                           b.depth, b.tile_min_depth, small_limit, grid);
-        break;
-    case 16:
-        hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, s, fp, b.tile_count, b.tile_off,
-                           b.tile_cap, b.tile_pool, b.queue, b.queue_cap, b.pool, b.pool_cap, b.big, b.big_cap, b.counters,
-                           b.trirec, b.vis, b.depth, b.tile_min_depth, small_limit, grid);
-        break;
-    case 32:
-        hipLaunchKernelGGL((k_raster<32, 256>), dim3(grid), dim3(256), 0, s, fp, b.tile_count, b.tile_off,
-                           b.tile_cap, b.tile_pool, b.queue, b.queue_cap, b.pool, b.pool_cap, b.big, b.big_cap, b.counters,
-                           b.trirec, b.vis, b.depth, b.tile_min_depth, small_limit, grid);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    case 8: hipLaunchKernelGGL((k_resolve<8, 64>), dim3(grid), dim3(64), 0, s, RAST_RESOLVE_ARGS); break;
+    case 16: hipLaunchKernelGGL((k_resolve<16, 256>), dim3(grid), dim3(256), 0, s, RAST_RESOLVE_ARGS); break;
+    case 32: hipLaunchKernelGGL((k_resolve<32, 256>), dim3(grid), dim3(256), 0, s, RAST_RESOLVE_ARGS); break;
+    default: return hipErrorInvalidValue;
     }
+#undef RAST_RESOLVE_ARGS
     return hipGetLastError();
 }
 
 hipError_t launch_shade(const FrameParams &fp, const FrameBuffers &b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shade, dim3(ceil_div(fp.width, 256), fp.height), dim3(256), 0, s, fp, b.meshes, b.vis,
+    hipLaunchKernelGGL(k_shade, dim3(ceil_div(fp.width, 64), ceil_div(fp.height, 4)), dim3(256), 0, s, fp, b.meshes, b.vis,
                        b.depth, b.trirec, b.attr, b.rgba);
     return hipGetLastError();
 }

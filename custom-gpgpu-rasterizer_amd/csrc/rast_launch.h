@@ -33,12 +33,14 @@ struct FrameBuffers {
     FrameCounters *counters;
     uint32_t *density;      // per tile projected-vertex count (learned policy), may be null
     float *tile_min_depth;  // per tile min depth written by the raster, may be null
+    unsigned long long *keys;  // per pixel (z << 32 | ~(id + 1)), reset to depth 1 / none after each resolve
     uint32_t *vis;
     float *depth;
     uint32_t *rgba;
 };
 
 hipError_t launch_frame_init(const FrameParams &fp, const FrameBuffers &b, bool single_pass, hipStream_t s);
+hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s);
 hipError_t launch_vertex(const FrameParams &fp, const FrameBuffers &b, hipStream_t s);
 hipError_t launch_setup(const FrameParams &fp, const FrameBuffers &b, bool single_pass, hipStream_t s);
 hipError_t launch_scan(uint32_t n, const uint32_t *in, uint32_t *out_off, uint32_t *zero_cursor, uint32_t limit,

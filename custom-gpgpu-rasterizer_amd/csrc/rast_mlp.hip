@@ -52,13 +52,12 @@ __device__ __forceinline__ ParamView view(const float *p, int H)
 
 // ---------------------------------------------------------------- features
 
-__global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t *__restrict__ prev,
-                                                  const float *__restrict__ prev_depth,
-                                                  const uint32_t *__restrict__ density, float *__restrict__ out)
+// Per-tile features (inputs of the capacity model), shared by inference and
+// training-set recording.
+__device__ __forceinline__ void tile_features(const FrameParams &fp, uint32_t t, const uint32_t *__restrict__ prev,
+                                              const float *__restrict__ prev_depth,
+                                              const uint32_t *__restrict__ density, float *o)
 {
-    const uint32_t n = fp.tiles_x * fp.tiles_y;
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= n) return;
     const int tx = (int)(t % fp.tiles_x), ty = (int)(t / fp.tiles_x);
     float sum = 0.f, mx = 0.f, dsum = 0.f;
     int cnt = 0;
@@ -73,7 +72,6 @@ __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t
             dsum += density ? (float)density[q] : 0.f;
             ++cnt;
         }
-    float *o = out + (size_t)t * F;
     o[0] = log1pf((float)prev[t]);
     o[1] = log1pf(sum / (float)cnt);
     o[2] = log1pf(mx);
@@ -82,6 +80,67 @@ __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t
     o[5] = prev_depth[t];
     o[6] = ((float)ty + 0.5f) / (float)fp.tiles_y;
     o[7] = ((float)tx + 0.5f) / (float)fp.tiles_x;
+}
+
+__global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t *__restrict__ prev,
+                                                  const float *__restrict__ prev_depth,
+                                                  const uint32_t *__restrict__ density, float *__restrict__ out)
+{
+    const uint32_t n = fp.tiles_x * fp.tiles_y;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n) return;
+    float f[F];
+    tile_features(fp, t, prev, prev_depth, density, f);
+    for (int q = 0; q < F; ++q) out[(size_t)t * F + q] = f[q];
+}
+
+// Frame-time inference: a block serves 64 tiles (one per lane); each of its 4
+// waves computes a quarter of the hidden units, weights come through scalar
+// loads (uniform across the wave), layer-1 activations are shared in LDS.
+template <int H>
+__global__ __launch_bounds__(256) void k_mlp_capacity(FrameParams fp, const uint32_t *__restrict__ prev,
+                                                      const float *__restrict__ prev_depth,
+                                                      const uint32_t *__restrict__ density,
+                                                      const float *__restrict__ p, float margin, uint32_t slack,
+                                                      uint32_t *__restrict__ cap)
+{
+    constexpr int Q = H / 4;
+    __shared__ float h1s[H][65];
+    __shared__ float part[4][64];
+    const uint32_t n = fp.tiles_x * fp.tiles_y;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t t = blockIdx.x * 64u + lane;
+    const float *W1 = p, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
+    float x[F];
+    if (t < n) tile_features(fp, t, prev, prev_depth, density, x);
+    else for (int f = 0; f < F; ++f) x[f] = 0.f;
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+        const int k = (int)w * Q + u;
+        float a = b1[k];
+#pragma unroll
+        for (int f = 0; f < F; ++f) a += W1[k * F + f] * x[f];
+        h1s[k][lane] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    float h1[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) h1[k] = h1s[k][lane];
+    float o = 0.f;
+    for (int u = 0; u < Q; ++u) {
+        const int j = (int)w * Q + u;
+        float a = b2[j];
+#pragma unroll
+        for (int k = 0; k < H; ++k) a += W2[j * H + k] * h1[k];
+        o += W3[j] * (a > 0.f ? a : 0.f);
+    }
+    part[w][lane] = o;
+    __syncthreads();
+    if (w == 0 && t < n) {
+        const float out = ((b3[0] + part[0][lane]) + part[1][lane] + part[2][lane]) + part[3][lane];
+        const float c = ceilf(expm1f(out < 20.f ? out : 20.f) * margin);
+        cap[t] = (c > 0.f ? (uint32_t)c : 0u) + slack;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_targets(uint32_t n, const uint32_t *__restrict__ count,
@@ -386,10 +445,18 @@ int rast_model_capacities(rast_binmodel *m, const FrameParams &fp, const uint32_
                           const uint32_t *density, float *features, float margin, uint32_t slack, uint32_t *cap,
                           hipStream_t s)
 {
+    (void)features;
     if (!m) return rast_set_error(RAST_E_STATE, "learned binning without a model");
-    HIP_TRY(launch_tile_features(fp, prev_count, prev_depth, density, features, s));
-    return launch_infer(m->H, fp.tiles_x * fp.tiles_y, features, (const float *)m->params.p, margin, slack, cap, nullptr,
-                        s);
+    const uint32_t n = fp.tiles_x * fp.tiles_y;
+    const dim3 g(cdiv(n, 64)), b(256);
+    const float *p = (const float *)m->params.p;
+    switch (m->H) {
+    case 16: hipLaunchKernelGGL(k_mlp_capacity<16>, g, b, 0, s, fp, prev_count, prev_depth, density, p, margin, slack, cap); break;
+    case 32: hipLaunchKernelGGL(k_mlp_capacity<32>, g, b, 0, s, fp, prev_count, prev_depth, density, p, margin, slack, cap); break;
+    default: hipLaunchKernelGGL(k_mlp_capacity<64>, g, b, 0, s, fp, prev_count, prev_depth, density, p, margin, slack, cap); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return RAST_OK;
 }
 
 void rast_model_detach_renderer(rast_binmodel *m, rast_renderer *r)

@@ -76,7 +76,9 @@ struct rast_renderer {
     DBuf tile_count[2], tile_off, tile_cap, tile_pool, tile_cursor, density, tile_min_depth[2], features;
     DBuf queue, pool, big, ovf, counters, defaults;
     uint32_t queue_cap = 0, pool_cap = 0, big_cap = 0, ovf_cap = 0;
-    DBuf vis, depth, rgba;
+    DBuf vis, depth, rgba, keys;
+    bool keys_dirty = true;  // key buffer must be reset before the next frame
+    uint32_t micro_limit = 16;
     int cur = 0;  // index of the current frame's tile_count / tile_min_depth
     bool have_prev = false;
     rast::FrameCounters *h_counters = nullptr;  // pinned

@@ -28,7 +28,7 @@ DEBUG_VERTEX_POS, DEBUG_VERTEX_ATTR, DEBUG_TRI_RECORDS, DEBUG_TRI_ALIVE, DEBUG_T
 
 EXPORTS = [
     "rast_last_error", "rast_device_query", "rast_renderer_create", "rast_renderer_destroy", "rast_setup_bins",
-    "rast_resize", "rast_mesh_create", "rast_mesh_destroy", "rast_mesh_set_world", "rast_mesh_set_textures",
+    "rast_resize", "rast_set_tuning", "rast_mesh_create", "rast_mesh_destroy", "rast_mesh_set_world", "rast_mesh_set_textures",
     "rast_mesh_num_triangles", "rast_mesh_triangle_base", "rast_texture_create", "rast_texture_destroy",
     "rast_camera_frame", "rast_render", "rast_sync", "rast_readback", "rast_get_stats", "rast_frame_buffers",
     "rast_debug_read", "rast_binmodel_create", "rast_binmodel_destroy", "rast_binmodel_num_features",
@@ -95,6 +95,7 @@ def lib():
         L.rast_renderer_destroy.argtypes = [vp]
         L.rast_setup_bins.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.rast_resize.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.rast_set_tuning.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.rast_mesh_create.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.c_int, vp, C.POINTER(vp)]
         L.rast_mesh_destroy.argtypes = [vp]
         L.rast_mesh_set_world.argtypes = [vp, vp]
@@ -182,6 +183,9 @@ class Renderer:
     def setup_bins(self, tile: int, big_threshold: int = 256) -> None:
         check(lib().rast_setup_bins(self.h, tile, big_threshold), "rast_setup_bins")
         self.tile = tile
+
+    def set_tuning(self, micro_limit: int = 16, small_limit: int = 16) -> None:
+        check(lib().rast_set_tuning(self.h, micro_limit, small_limit), "rast_set_tuning")
 
     def texture(self, texels: np.ndarray) -> C.c_void_p:
         key = id(texels)

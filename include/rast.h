@@ -152,6 +152,14 @@ int rast_renderer_destroy(rast_renderer *r);
  * rectangle spans more than big_threshold tiles go to one screen-wide list
  * instead of the per-tile queues (0 = default 256). */
 int rast_setup_bins(rast_renderer *r, uint32_t tile_size, uint32_t big_threshold);
+/* Raster tuning (no reference counterpart; the reference derives its
+ * per-thread pixel coverage from the bin size, CCR/main.cpp:372-389):
+ * triangles whose bounding box has at most micro_limit pixels are rasterized
+ * directly at setup with 64-bit atomic min (0 = bin every triangle); inside a
+ * tile, triangles covering at most small_limit pixels of it are rasterized one
+ * per lane, larger ones one pixel per lane.  Defaults 16 / 16.  The output
+ * does not depend on either value. */
+int rast_set_tuning(rast_renderer *r, uint32_t micro_limit, uint32_t small_limit);
 /* Resize the framebuffer (a new Resolution in CCR/Resolution.cpp). */
 int rast_resize(rast_renderer *r, uint32_t width, uint32_t height);
 

@@ -83,8 +83,9 @@ def test_vertex_outputs_bitwise():
         attr = r.debug_read(R.DEBUG_VERTEX_ATTR, np.float32).reshape(-1, 16)
     opos, oattr = O.vertex_transform(w.scene, w.camera, w.width, w.height)
     assert_bitwise("pos4", pos, opos)
-    # attr layout on the GPU: uv n tan vd c 1/w in one 16-float record (same order as the oracle)
-    assert_bitwise("attr", attr[:, :15], oattr[:, :15])
+    # GPU record: u v 1/w n.x | n.y n.z t.x t.y | t.z vd.xyz | c.rgb 0; oracle: uv n tan vd c 1/w
+    gpu_in_oracle_order = attr[:, [0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 2]]
+    assert_bitwise("attr", np.ascontiguousarray(gpu_in_oracle_order), np.ascontiguousarray(oattr[:, :15]))
 
 
 def test_setup_records_bitwise():
@@ -139,3 +140,78 @@ def test_big_triangles_screen_list():
         vis, depth, _, st = gpu_render(scene, screen_camera(), S.Opts(), 128, 96, big_threshold=bt)
         assert_bitwise(f"vis bt={bt}", vis, ovis)
         assert_bitwise(f"depth bt={bt}", depth, odepth)
+
+
+@pytest.mark.parametrize("micro", [0, 4, 16, 64, 1024])
+def test_micro_limit_invariance(micro):
+    w = S.config1()
+    ovis, odepth, _, _ = O.render(w.scene, w.camera, w.opts, w.width, w.height, threads=4, want_rgba=False)
+    scene = random_triangle_scene(11, 400)
+    rvis, rdepth, _, _ = O.render(scene, screen_camera(), S.Opts(), 160, 96, threads=4, want_rgba=False)
+    for sc, cam, opts, W, H, ov, od in ((w.scene, w.camera, w.opts, w.width, w.height, ovis, odepth),
+                                         (scene, screen_camera(), S.Opts(), 160, 96, rvis, rdepth)):
+        with R.Renderer(W, H) as r:
+            r.set_tuning(micro, 16)
+            r.add_scene(sc)
+            r.render(cam, opts)
+            vis, depth, _ = r.readback()
+        assert_bitwise(f"vis micro={micro}", vis, ov)
+        assert_bitwise(f"depth micro={micro}", depth, od)
+
+
+@pytest.mark.parametrize("small", [0, 16, 256])
+def test_small_limit_invariance(small):
+    w = S.config1()
+    ovis, odepth, _, _ = O.render(w.scene, w.camera, w.opts, w.width, w.height, threads=4, want_rgba=False)
+    with R.Renderer(w.width, w.height) as r:
+        r.set_tuning(0, small)
+        r.add_scene(w.scene)
+        r.render(w.camera, w.opts)
+        vis, depth, _ = r.readback()
+    assert_bitwise("vis", vis, ovis)
+    assert_bitwise("depth", depth, odepth)
+
+
+def test_config2_icospheres_full_size():
+    w = S.config2()
+    st = check_frame(w.scene, w.camera, w.opts, w.width, w.height, threads=16)
+    assert st["num_triangles"] == 5_242_880
+
+
+def test_config3_terrain_4k_full_size():
+    w = S.config3()
+    st = check_frame(w.scene, w.camera, w.opts, w.width, w.height, threads=16)
+    assert st["num_triangles"] == 2_097_152 + 8
+
+
+def small_terrain_workload():
+    w = S.config3(n=192, tex_size=256, width=640, height=360)
+    return w
+
+
+@pytest.mark.parametrize("policy", [R.BIN_PREVFRAME, R.BIN_LEARNED])
+@pytest.mark.parametrize("margin,slack", [(1.0, 0), (1.25, 4), (4.0, 64)])
+def test_single_pass_policies_match_exact(policy, margin, slack):
+    """Single-pass binning with any capacities (including ones that overflow
+    on most tiles) renders exactly the frames of the exact two-pass binner."""
+    w = small_terrain_workload()
+    _, path = S.train_test_paths(4, 6)
+    with R.Renderer(w.width, w.height) as ex, R.Renderer(w.width, w.height) as sp:
+        ex.add_scene(w.scene)
+        sp.add_scene(w.scene)
+        sp.set_tuning(0, 16)  # bin every triangle so the queues carry the whole frame
+        model = R.BinModel(sp, hidden=16, seed=3)
+        model.attach(sp, margin, slack)
+        overflowed = 0
+        for cam in path:
+            ex.render(cam, w.opts, R.BIN_EXACT)
+            sp.render(cam, w.opts, policy)
+            a = ex.readback()
+            b = sp.readback()
+            overflowed += sp.stats()["overflow_tiles"]
+            assert_bitwise("vis", b[0], a[0])
+            assert_bitwise("depth", b[1], a[1])
+            np.testing.assert_array_equal(b[2], a[2])
+        model.close()
+    if margin == 1.0 and slack == 0:
+        assert overflowed > 0  # the fallback path really ran
