@@ -788,6 +788,7 @@ extern "C" int rast_get_stats(rast_renderer *r, rast_stats *out)
     s.num_bin_pairs = c.total_pairs;
     s.num_big = c.num_big;
     s.overflow_tiles = c.overflow_tiles;
+    s.overflow_entries = c.ovf_entries;
     s.capacity_total = r->last_single ? c.capacity_total : 0;
     s.rerenders = r->rerenders;
     s.ms_vertex = elapsed(r, EV_START, EV_VERTEX);

@@ -32,23 +32,7 @@ namespace {
 
 constexpr int F = kNumFeatures;
 
-struct ParamView {
-    const float *W1, *b1, *W2, *b2, *W3, *b3;
-};
-
 __host__ __device__ inline size_t num_params(int H) { return (size_t)H * F + H + (size_t)H * H + H + H + 1; }
-
-__device__ __forceinline__ ParamView view(const float *p, int H)
-{
-    ParamView v;
-    v.W1 = p;
-    v.b1 = v.W1 + H * F;
-    v.W2 = v.b1 + H;
-    v.b2 = v.W2 + H * H;
-    v.W3 = v.b2 + H;
-    v.b3 = v.W3 + H;
-    return v;
-}
 
 // ---------------------------------------------------------------- features
 
@@ -208,103 +192,121 @@ __global__ __launch_bounds__(256) void k_mlp_infer(uint32_t n, const float *__re
 
 // ----------------------------------------------------------------- training
 
-// Per sample: activations and deltas for the weight-gradient reduction.
+// One training step is split over blocks of 64 samples (split-K): wave 0
+// runs forward + backward with one sample per lane (weights in LDS) and keeps
+// activations / deltas in LDS; then all 256 threads form the block's partial
+// weight gradients, which k_reduce_partials sums in a fixed order, so a step
+// is deterministic.
 template <int H>
-__global__ __launch_bounds__(256) void k_mlp_fwd_bwd(uint32_t B, uint32_t N, const float *__restrict__ X,
-                                                     const float *__restrict__ Y, const uint32_t *__restrict__ idx,
-                                                     const float *__restrict__ p, float tau, float *__restrict__ Xb,
-                                                     float *__restrict__ H1, float *__restrict__ H2,
-                                                     float *__restrict__ D1, float *__restrict__ D2,
-                                                     float *__restrict__ dout, float *__restrict__ L)
+__global__ __launch_bounds__(256) void k_train_partial(uint32_t B, uint32_t N, const float *__restrict__ X,
+                                                       const float *__restrict__ Y, const uint32_t *__restrict__ idx,
+                                                       const float *__restrict__ p, float tau,
+                                                       float *__restrict__ partial)
 {
     constexpr int NP = H * F + H + H * H + H + H + 1;
     __shared__ float w[NP];
+    __shared__ float sx[64][F + 1];
+    __shared__ float sh1[64][H + 1], sh2[64][H + 1], sd1[64][H + 1], sd2[64][H + 1];
+    __shared__ float sdout[64], sloss[64];
     for (int i = threadIdx.x; i < NP; i += 256) w[i] = p[i];
     __syncthreads();
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
-    if (s >= B) return;
     const float *W1 = w, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
-    uint32_t src = idx ? idx[s] : s;
-    src = src < N ? src : N - 1;
-    float xv[F], h1[H], h2[H], d2[H];
+    const uint32_t s0 = blockIdx.x * 64u;
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x, s = s0 + lane;
+        float xv[F], h1[H], h2[H], d2[H];
+        float g = 0.f, l = 0.f;
+        if (s < B) {
+            uint32_t src = idx ? idx[s] : s;
+            src = src < N ? src : N - 1;
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-        xv[f] = X[(size_t)src * F + f];
-        Xb[(size_t)s * F + f] = xv[f];
+            for (int f = 0; f < F; ++f) xv[f] = X[(size_t)src * F + f];
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+                float a = b1[k];
+#pragma unroll
+                for (int f = 0; f < F; ++f) a += W1[k * F + f] * xv[f];
+                h1[k] = a > 0.f ? a : 0.f;
+            }
+            float out = b3[0];
+            for (int j = 0; j < H; ++j) {
+                float a = b2[j];
+#pragma unroll
+                for (int k = 0; k < H; ++k) a += W2[j * H + k] * h1[k];
+                h2[j] = a > 0.f ? a : 0.f;
+                out += W3[j] * h2[j];
+            }
+            const float r = Y[src] - out;
+            l = r > 0.f ? tau * r : (tau - 1.f) * r;
+            g = (r > 0.f ? -tau : 1.f - tau) / (float)B;
+        } else {
+#pragma unroll
+            for (int f = 0; f < F; ++f) xv[f] = 0.f;
+#pragma unroll
+            for (int k = 0; k < H; ++k) { h1[k] = 0.f; h2[k] = 0.f; }
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) d2[j] = h2[j] > 0.f ? g * W3[j] : 0.f;
+        for (int k = 0; k < H; ++k) {
+            float a = 0.f;
+#pragma unroll
+            for (int j = 0; j < H; ++j) a += W2[j * H + k] * d2[j];
+            sd1[lane][k] = h1[k] > 0.f ? a : 0.f;
+            sh1[lane][k] = h1[k];
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) { sd2[lane][j] = d2[j]; sh2[lane][j] = h2[j]; }
+#pragma unroll
+        for (int f = 0; f < F; ++f) sx[lane][f] = xv[f];
+        sdout[lane] = g;
+        sloss[lane] = l;
     }
-#pragma unroll
-    for (int k = 0; k < H; ++k) {
-        float a = b1[k];
-#pragma unroll
-        for (int f = 0; f < F; ++f) a += W1[k * F + f] * xv[f];
-        h1[k] = a > 0.f ? a : 0.f;
-    }
-    float out = b3[0];
-    for (int j = 0; j < H; ++j) {
-        float a = b2[j];
-#pragma unroll
-        for (int k = 0; k < H; ++k) a += W2[j * H + k] * h1[k];
-        h2[j] = a > 0.f ? a : 0.f;
-        out += W3[j] * h2[j];
-    }
-    const float r = Y[src] - out;
-    L[s] = r > 0.f ? tau * r : (tau - 1.f) * r;
-    const float g = (r > 0.f ? -tau : 1.f - tau) / (float)B;
-    dout[s] = g;
-#pragma unroll
-    for (int j = 0; j < H; ++j) d2[j] = h2[j] > 0.f ? g * W3[j] : 0.f;
-    for (int k = 0; k < H; ++k) {
+    __syncthreads();
+    float *out = partial + (size_t)blockIdx.x * (NP + 1);
+    float *gW1 = out, *gb1 = gW1 + H * F, *gW2 = gb1 + H, *gb2 = gW2 + H * H, *gW3 = gb2 + H, *gb3 = gW3 + H;
+    for (int o = threadIdx.x; o < H * H; o += 256) {
+        const int j = o / H, k = o % H;
         float a = 0.f;
-#pragma unroll
-        for (int j = 0; j < H; ++j) a += W2[j * H + k] * d2[j];
-        D1[(size_t)s * H + k] = h1[k] > 0.f ? a : 0.f;
-        H1[(size_t)s * H + k] = h1[k];
+        for (int s = 0; s < 64; ++s) a += sd2[s][j] * sh1[s][k];
+        gW2[o] = a;
     }
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-        D2[(size_t)s * H + j] = d2[j];
-        H2[(size_t)s * H + j] = h2[j];
+    for (int o = threadIdx.x; o < H * F; o += 256) {
+        const int j = o / F, f = o % F;
+        float a = 0.f;
+        for (int s = 0; s < 64; ++s) a += sd1[s][j] * sx[s][f];
+        gW1[o] = a;
+    }
+    if (threadIdx.x < H) {
+        const int j = threadIdx.x;
+        float a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        for (int s = 0; s < 64; ++s) {
+            a1 += sd1[s][j];
+            a2 += sd2[s][j];
+            a3 += sdout[s] * sh2[s][j];
+        }
+        gb1[j] = a1;
+        gb2[j] = a2;
+        gW3[j] = a3;
+    }
+    if (threadIdx.x == 255) {
+        float a = 0.f, lsum = 0.f;
+        for (int s = 0; s < 64; ++s) { a += sdout[s]; lsum += sloss[s]; }
+        gb3[0] = a;
+        out[NP] = lsum;  // loss partial
     }
 }
 
-// Weight gradients, one block per hidden unit j, fixed summation order.
-template <int H>
-__global__ __launch_bounds__(256) void k_mlp_wgrad(uint32_t B, const float *__restrict__ Xb,
-                                                   const float *__restrict__ H1, const float *__restrict__ H2,
-                                                   const float *__restrict__ D1, const float *__restrict__ D2,
-                                                   const float *__restrict__ dout, const float *__restrict__ L,
-                                                   float *__restrict__ grad, float *__restrict__ loss)
+// grad[i] = sum over blocks of partial[b][i] in block order; loss = mean.
+__global__ __launch_bounds__(256) void k_reduce_partials(uint32_t nblocks, uint32_t np, uint32_t B,
+                                                         const float *__restrict__ partial, float *__restrict__ grad,
+                                                         float *__restrict__ loss)
 {
-    const int j = blockIdx.x;
-    const int k = threadIdx.x & 63, part = threadIdx.x >> 6;
-    float aW2 = 0.f, aW1 = 0.f, ab1 = 0.f, ab2 = 0.f, aW3 = 0.f, ab3 = 0.f, aL = 0.f;
-    for (uint32_t s = part; s < B; s += 4) {
-        const float d2 = D2[(size_t)s * H + j];
-        const float d1 = D1[(size_t)s * H + j];
-        if (k < H) aW2 += d2 * H1[(size_t)s * H + k];
-        if (k < F) aW1 += d1 * Xb[(size_t)s * F + k];
-        if (k == 0) { ab1 += d1; ab2 += d2; }
-        if (k == 1) aW3 += dout[s] * H2[(size_t)s * H + j];
-        if (j == 0 && k == 2) ab3 += dout[s];
-        if (j == 0 && k == 3) aL += L[s];
-    }
-    __shared__ float red[7][256];
-    red[0][threadIdx.x] = aW2; red[1][threadIdx.x] = aW1; red[2][threadIdx.x] = ab1;
-    red[3][threadIdx.x] = ab2; red[4][threadIdx.x] = aW3; red[5][threadIdx.x] = ab3;
-    red[6][threadIdx.x] = aL;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        float v[7];
-        for (int q = 0; q < 7; ++q)
-            v[q] = ((red[q][threadIdx.x] + red[q][threadIdx.x + 64]) + red[q][threadIdx.x + 128]) + red[q][threadIdx.x + 192];
-        float *gW1 = grad, *gb1 = gW1 + H * F, *gW2 = gb1 + H, *gb2 = gW2 + H * H, *gW3 = gb2 + H, *gb3 = gW3 + H;
-        if (k < H) gW2[j * H + k] = v[0];
-        if (k < F) gW1[j * F + k] = v[1];
-        if (k == 0) { gb1[j] = v[2]; gb2[j] = v[3]; }
-        if (k == 1) gW3[j] = v[4];
-        if (j == 0 && k == 2) gb3[0] = v[5];
-        if (j == 0 && k == 3 && loss) loss[0] = v[6] / (float)B;
-    }
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i > np) return;
+    float a = 0.f;
+    for (uint32_t b = 0; b < nblocks; ++b) a += partial[(size_t)b * (np + 1) + i];
+    if (i < np) grad[i] = a;
+    else if (loss) loss[0] = a / (float)B;
 }
 
 __global__ __launch_bounds__(256) void k_adam(uint32_t n, float *__restrict__ p, const float *__restrict__ g,
@@ -343,7 +345,7 @@ struct rast_binmodel {
     DBuf params, grad, adam_m, adam_v;
     DBuf X, Y;            // recorded training set
     uint32_t n = 0, cap = 0;
-    DBuf idx, Xb, H1, H2, D1, D2, dout, L, loss, io_x, io_y, io_p;
+    DBuf idx, partial, loss, io_x, io_y, io_p;  // partial: per 64-sample block gradients + loss
     uint32_t batch_cap = 0;
     uint64_t adam_t = 0;
     std::vector<rast_renderer *> users;
@@ -374,18 +376,14 @@ int launch_infer(int H, uint32_t n, const float *x, const float *p, float margin
     return RAST_OK;
 }
 
+constexpr uint32_t kMaxBatch = 1u << 16;
+
 int ensure_batch(rast_binmodel *m, uint32_t B)
 {
     if (B <= m->batch_cap) return RAST_OK;
-    const size_t H = (size_t)m->H;
+    const size_t nb = (B + 63) / 64;
     RAST_TRY(ensure(m->idx, (size_t)B * 4));
-    RAST_TRY(ensure(m->Xb, (size_t)B * F * 4));
-    RAST_TRY(ensure(m->H1, (size_t)B * H * 4));
-    RAST_TRY(ensure(m->H2, (size_t)B * H * 4));
-    RAST_TRY(ensure(m->D1, (size_t)B * H * 4));
-    RAST_TRY(ensure(m->D2, (size_t)B * H * 4));
-    RAST_TRY(ensure(m->dout, (size_t)B * 4));
-    RAST_TRY(ensure(m->L, (size_t)B * 4));
+    RAST_TRY(ensure(m->partial, nb * (num_params(m->H) + 1) * 4));
     RAST_TRY(ensure(m->loss, 64));
     m->batch_cap = B;
     return RAST_OK;
@@ -394,28 +392,20 @@ int ensure_batch(rast_binmodel *m, uint32_t B)
 // One forward/backward over B samples (X, Y, optional index list) -> grad, loss.
 int loss_grad(rast_binmodel *m, uint32_t B, uint32_t N, const float *X, const float *Y, const uint32_t *idx, float tau)
 {
+    if (B == 0 || B > kMaxBatch) return rast_set_error(RAST_E_INVALID_ARG, "batch %u not in [1, %u]", B, kMaxBatch);
     RAST_TRY(ensure_batch(m, B));
-    const dim3 g(cdiv(B, 256)), b(256);
+    const uint32_t nblocks = (B + 63) / 64;
+    const uint32_t np = (uint32_t)num_params(m->H);
     const float *p = (const float *)m->params.p;
-    float *Xb = (float *)m->Xb.p, *H1 = (float *)m->H1.p, *H2 = (float *)m->H2.p, *D1 = (float *)m->D1.p,
-          *D2 = (float *)m->D2.p, *dout = (float *)m->dout.p, *L = (float *)m->L.p;
+    float *part = (float *)m->partial.p;
     switch (m->H) {
-    case 16:
-        hipLaunchKernelGGL(k_mlp_fwd_bwd<16>, g, b, 0, m->stream, B, N, X, Y, idx, p, tau, Xb, H1, H2, D1, D2, dout, L);
-        hipLaunchKernelGGL(k_mlp_wgrad<16>, dim3(16), b, 0, m->stream, B, Xb, H1, H2, D1, D2, dout, L,
-                           (float *)m->grad.p, (float *)m->loss.p);
-        break;
-    case 32:
-        hipLaunchKernelGGL(k_mlp_fwd_bwd<32>, g, b, 0, m->stream, B, N, X, Y, idx, p, tau, Xb, H1, H2, D1, D2, dout, L);
-        hipLaunchKernelGGL(k_mlp_wgrad<32>, dim3(32), b, 0, m->stream, B, Xb, H1, H2, D1, D2, dout, L,
-                           (float *)m->grad.p, (float *)m->loss.p);
-        break;
-    default:
-        hipLaunchKernelGGL(k_mlp_fwd_bwd<64>, g, b, 0, m->stream, B, N, X, Y, idx, p, tau, Xb, H1, H2, D1, D2, dout, L);
-        hipLaunchKernelGGL(k_mlp_wgrad<64>, dim3(64), b, 0, m->stream, B, Xb, H1, H2, D1, D2, dout, L,
-                           (float *)m->grad.p, (float *)m->loss.p);
-        break;
+    case 16: hipLaunchKernelGGL(k_train_partial<16>, dim3(nblocks), dim3(256), 0, m->stream, B, N, X, Y, idx, p, tau, part); break;
+    case 32: hipLaunchKernelGGL(k_train_partial<32>, dim3(nblocks), dim3(256), 0, m->stream, B, N, X, Y, idx, p, tau, part); break;
+    default: hipLaunchKernelGGL(k_train_partial<64>, dim3(nblocks), dim3(256), 0, m->stream, B, N, X, Y, idx, p, tau, part); break;
     }
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partials, dim3(cdiv(np + 1, 256)), dim3(256), 0, m->stream, nblocks, np, B, part,
+                       (float *)m->grad.p, (float *)m->loss.p);
     HIP_TRY(hipGetLastError());
     return RAST_OK;
 }
@@ -528,8 +518,8 @@ extern "C" int rast_binmodel_destroy(rast_binmodel *m)
             (void)hipStreamSynchronize(r->stream);
             if (r->model == m) r->model = nullptr;
         }
-    DBuf *bufs[] = { &m->params, &m->grad, &m->adam_m, &m->adam_v, &m->X, &m->Y, &m->idx, &m->Xb, &m->H1,
-                     &m->H2, &m->D1, &m->D2, &m->dout, &m->L, &m->loss, &m->io_x, &m->io_y, &m->io_p };
+    DBuf *bufs[] = { &m->params, &m->grad, &m->adam_m, &m->adam_v, &m->X, &m->Y, &m->idx, &m->partial, &m->loss,
+                     &m->io_x, &m->io_y, &m->io_p };
     for (DBuf *b : bufs) dfree(*b);
     delete m;
     return RAST_OK;
@@ -623,7 +613,7 @@ extern "C" int rast_binmodel_record(rast_binmodel *m, rast_renderer *r)
 extern "C" int rast_binmodel_train(rast_binmodel *m, uint32_t epochs, uint32_t batch, float lr, float tau,
                                    float *final_loss)
 {
-    if (!m || batch == 0 || batch > (1u << 22) || !(lr > 0.f) || !(tau > 0.f && tau < 1.f))
+    if (!m || batch == 0 || batch > kMaxBatch || !(lr > 0.f) || !(tau > 0.f && tau < 1.f))
         return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_train: bad arguments");
     if (m->n == 0) return rast_set_error(RAST_E_STATE, "rast_binmodel_train: no recorded data");
     HIP_TRY(hipSetDevice(m->device));
@@ -662,7 +652,7 @@ extern "C" int rast_binmodel_forward(rast_binmodel *m, const float *x, uint32_t 
 extern "C" int rast_binmodel_loss_grad(rast_binmodel *m, const float *x, const float *target, uint32_t n, float tau,
                                        float *loss, float *grad)
 {
-    if (!m || !x || !target || !grad || n == 0 || n > (1u << 22))
+    if (!m || !x || !target || !grad || n == 0 || n > kMaxBatch)
         return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_loss_grad: bad arguments");
     HIP_TRY(hipSetDevice(m->device));
     RAST_TRY(ensure(m->io_x, (size_t)n * F * 4));

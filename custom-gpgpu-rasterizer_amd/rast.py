@@ -56,7 +56,7 @@ class FrameOpts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("num_triangles", C.c_uint64), ("num_visible", C.c_uint64), ("num_bin_pairs", C.c_uint64),
                 ("num_big", C.c_uint32), ("num_tiles", C.c_uint32), ("overflow_tiles", C.c_uint32),
-                ("capacity_total", C.c_uint32), ("rerenders", C.c_uint32), ("reserved", C.c_uint32)] + [
+                ("capacity_total", C.c_uint32), ("rerenders", C.c_uint32), ("overflow_entries", C.c_uint32)] + [
         (n, C.c_float) for n in ("ms_vertex", "ms_setup", "ms_count", "ms_scan", "ms_fill", "ms_fallback",
                                  "ms_raster", "ms_shade", "ms_total", "ms_model")]
 

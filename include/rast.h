@@ -109,7 +109,7 @@ typedef struct {
     uint32_t overflow_tiles;     /* single-pass policies: tiles that fell back to exact binning */
     uint32_t capacity_total;     /* single-pass policies: entries allocated by the capacity model */
     uint32_t rerenders;          /* frames re-run after a queue-storage overflow (storage grown) */
-    uint32_t reserved;
+    uint32_t overflow_entries;   /* single-pass policies: queue entries placed by the exact fallback */
     float ms_vertex, ms_setup, ms_count, ms_scan, ms_fill, ms_fallback, ms_raster, ms_shade, ms_total;
     float ms_model;              /* capacity-model inference */
 } rast_stats;

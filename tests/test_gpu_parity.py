@@ -215,3 +215,38 @@ def test_single_pass_policies_match_exact(policy, margin, slack):
         model.close()
     if margin == 1.0 and slack == 0:
         assert overflowed > 0  # the fallback path really ran
+
+
+@pytest.mark.parametrize("vis_mode,sample", [(S.VIS_PBR, S.SAMPLE_LINEAR), (S.VIS_DEPTH, S.SAMPLE_POINT),
+                                             (S.VIS_NORMAL, S.SAMPLE_POINT)])
+def test_sampler_and_visualisation_modes(vis_mode, sample):
+    w = S.config1()
+    opts = S.Opts(visualisation=vis_mode, sample_state=sample)
+    check_frame(w.scene, w.camera, opts, w.width, w.height)
+
+
+def test_vertex_colour_path_without_diffuse_map():
+    m = S.uv_sphere(48, 24)
+    rng = np.random.default_rng(2)
+    m.verts[:, 11:14] = rng.uniform(0, 1, (m.verts.shape[0], 3)).astype(np.float32)
+    check_frame(S.Scene([m]), S.Camera(), S.Opts(), 320, 240)
+
+
+def test_triangle_strip_topology():
+    # a zig-zag strip across the screen: odd triangles swap their last two indices
+    pts = []
+    for i in range(40):
+        x = -1.4 + 2.8 * (i // 2) / 19.0
+        pts.append((x, 0.6 if i % 2 == 0 else -0.6, -2.0 - 0.01 * i))
+    strip = mesh(pts, list(range(40)), topology=S.TOPO_STRIP)
+    for cull in (S.CULL_BACK, S.CULL_NONE):
+        check_frame(S.Scene([strip]), screen_camera(), S.Opts(cull_mode=cull), 200, 120)
+
+
+def test_multi_mesh_ids_and_world_matrices():
+    a = random_triangle_scene(21, 60).meshes[0]
+    b = random_triangle_scene(22, 60).meshes[0]
+    b.world = S.world_matrix((0.2, -0.1, -0.3), (0.1, 0.2, 0.05), 0.9)
+    c = random_triangle_scene(23, 60).meshes[0]
+    scene = S.Scene([a, b, c])
+    check_frame(scene, screen_camera(), S.Opts(cull_mode=S.CULL_NONE), 160, 96)
