@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -186,6 +187,9 @@ extern "C" int rast_camera_frame(const rast_camera *cam, uint32_t width, uint32_
     return RAST_OK;
 }
 
+// Timing events of one frame, recorded between the stages when RAST_FLAG_TIMING is set.
+enum { EV_START, EV_VERTEX, EV_MODEL, EV_SETUP, EV_SCAN, EV_FILL, EV_FALLBACK, EV_RASTER, EV_SHADE, EV_COUNT };
+
 static int alloc_framebuffers(rast_renderer *r)
 {
     bool synced = false;
@@ -196,10 +200,11 @@ static int alloc_framebuffers(rast_renderer *r)
     HIP_TRY(grow(r, r->keys, P * 8, &synced));
     r->keys_dirty = true;
     r->frame_valid = false;
+    ++r->generation;
     return RAST_OK;
 }
 
-extern "C" int rast_renderer_create(int device, uint32_t width, uint32_t height, rast_renderer **out)
+static int new_renderer(int device, uint32_t width, uint32_t height, rast_renderer *scene_src, rast_renderer **out)
 {
     if (!out) return rast_set_error(RAST_E_INVALID_ARG, "rast_renderer_create: null output");
     *out = nullptr;
@@ -216,8 +221,8 @@ extern "C" int rast_renderer_create(int device, uint32_t width, uint32_t height,
     r->H = height;
     int rc = RAST_OK;
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) rc = RAST_E_HIP;
-    if (rc == RAST_OK && hipEventCreateWithFlags(&r->staging_done, hipEventDisableTiming) != hipSuccess) rc = RAST_E_HIP;
-    for (int i = 0; rc == RAST_OK && i < 12; ++i)
+    if (rc == RAST_OK && hipEventCreateWithFlags(&r->join, hipEventDisableTiming) != hipSuccess) rc = RAST_E_HIP;
+    for (int i = 0; rc == RAST_OK && i < EV_COUNT; ++i)
         if (hipEventCreate(&r->ev[i]) != hipSuccess) rc = RAST_E_HIP;
     if (rc == RAST_OK && hipHostMalloc((void **)&r->h_counters, sizeof(FrameCounters)) != hipSuccess) rc = RAST_E_HIP;
     if (rc == RAST_OK && hipHostMalloc((void **)&r->h_defaults, 16) != hipSuccess) rc = RAST_E_HIP;
@@ -227,8 +232,24 @@ extern "C" int rast_renderer_create(int device, uint32_t width, uint32_t height,
         return rast_set_error(rc, "rast_renderer_create: HIP resource allocation failed");
     }
     memset(r->h_counters, 0, sizeof(FrameCounters));
+    if (scene_src) {
+        r->scene_src = scene_src;
+        scene_src->views.push_back(r);
+    }
     *out = r;
     return RAST_OK;
+}
+
+extern "C" int rast_renderer_create(int device, uint32_t width, uint32_t height, rast_renderer **out)
+{
+    return new_renderer(device, width, height, nullptr, out);
+}
+
+extern "C" int rast_renderer_create_view(rast_renderer *owner, uint32_t width, uint32_t height, rast_renderer **out)
+{
+    if (!owner) return rast_set_error(RAST_E_INVALID_ARG, "rast_renderer_create_view: null owner");
+    if (owner->scene_src) owner = owner->scene_src;  // a view of a view draws the same scene
+    return new_renderer(owner->device, width, height, owner, out);
 }
 
 static void free_mesh(rast_mesh *m)
@@ -241,8 +262,18 @@ static void free_mesh(rast_mesh *m)
 extern "C" int rast_renderer_destroy(rast_renderer *r)
 {
     if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_renderer_destroy: null renderer");
+    if (!r->views.empty())
+        return rast_set_error(RAST_E_STATE, "rast_renderer_destroy: %zu views still draw this renderer's meshes",
+                              r->views.size());
     (void)hipSetDevice(r->device);
     if (r->stream) (void)hipStreamSynchronize(r->stream);
+    rast_forget_renderer(r);
+    if (r->scene_src) {
+        std::vector<rast_renderer *> &v = r->scene_src->views;
+        v.erase(std::remove(v.begin(), v.end(), r), v.end());
+    }
+    for (FrameGraph &g : r->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     rast_model_detach_renderer(r->model, r);
     for (rast_mesh *m : r->meshes) free_mesh(m);
     for (rast_texture *t : r->textures) {
@@ -260,7 +291,7 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
     if (r->h_defaults) (void)hipHostFree(r->h_defaults);
     for (hipEvent_t &e : r->ev)
         if (e) (void)hipEventDestroy(e);
-    if (r->staging_done) (void)hipEventDestroy(r->staging_done);
+    if (r->join) (void)hipEventDestroy(r->join);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return RAST_OK;
@@ -309,6 +340,7 @@ extern "C" int rast_mesh_create(rast_renderer *r, const rast_ivertex *verts, uin
     *out = nullptr;
     if (!r || !verts || !indices || num_verts == 0 || num_indices == 0)
         return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_create: empty or null mesh data");
+    if (r && r->scene_src) return rast_set_error(RAST_E_STATE, "rast_mesh_create: a view has no meshes of its own");
     if (topology != RAST_TOPOLOGY_LIST && topology != RAST_TOPOLOGY_STRIP)
         return rast_set_error(RAST_E_INVALID_ARG, "unknown topology %d", topology);
     if (r->meshes.size() >= (size_t)kMaxMeshes) return rast_set_error(RAST_E_INVALID_ARG, "too many meshes");
@@ -342,7 +374,7 @@ extern "C" int rast_mesh_create(rast_renderer *r, const rast_ivertex *verts, uin
                     hipGetErrorString(e));
     }
     r->meshes.push_back(m);
-    r->scene_dirty = true;
+    ++r->scene_version;
     r->have_prev = false;
     *out = m;
     return RAST_OK;
@@ -354,9 +386,13 @@ extern "C" int rast_mesh_destroy(rast_mesh *m)
     rast_renderer *r = m->r;
     (void)hipSetDevice(r->device);
     HIP_TRY(hipStreamSynchronize(r->stream));
+    for (rast_renderer *v : r->views) {
+        HIP_TRY(hipStreamSynchronize(v->stream));
+        v->have_prev = false;
+    }
     r->meshes.erase(std::remove(r->meshes.begin(), r->meshes.end(), m), r->meshes.end());
     free_mesh(m);
-    r->scene_dirty = true;
+    ++r->scene_version;
     r->have_prev = false;
     r->frame_valid = false;
     return RAST_OK;
@@ -434,6 +470,7 @@ extern "C" int rast_texture_destroy(rast_texture *t)
     rast_renderer *r = t->r;
     (void)hipSetDevice(r->device);
     HIP_TRY(hipStreamSynchronize(r->stream));
+    for (rast_renderer *v : r->views) HIP_TRY(hipStreamSynchronize(v->stream));
     for (rast_mesh *m : r->meshes)
         for (auto &slot : m->tex)
             if (slot == t) slot = nullptr;
@@ -445,13 +482,19 @@ extern "C" int rast_texture_destroy(rast_texture *t)
 
 // ------------------------------------------------------------------ frame
 
+// Renderers created with rast_renderer_create_view draw the meshes of their
+// owner; everything per frame (buffers, staging, graphs) is their own.
+static std::vector<rast_mesh *> &scene_meshes(rast_renderer *r) { return r->scene_src ? r->scene_src->meshes : r->meshes; }
+static rast_renderer *scene_owner(rast_renderer *r) { return r->scene_src ? r->scene_src : r; }
+
 static int rebuild_layout(rast_renderer *r)
 {
+    std::vector<rast_mesh *> &meshes = scene_meshes(r);
     uint64_t vb = 0, tb = 0;
     r->vs_block_mesh.clear();
     r->tri_block_mesh.clear();
-    for (size_t i = 0; i < r->meshes.size(); ++i) {
-        rast_mesh *m = r->meshes[i];
+    for (size_t i = 0; i < meshes.size(); ++i) {
+        rast_mesh *m = meshes[i];
         m->tri_base = tb;
         const uint32_t nvb = ceil_div(m->nv, 256), ntb = ceil_div(m->ntri, 256);
         r->vs_block_mesh.insert(r->vs_block_mesh.end(), nvb, (uint32_t)i);
@@ -473,7 +516,9 @@ static int rebuild_layout(rast_renderer *r)
     if (!r->tri_block_mesh.empty())
         HIP_TRY(hipMemcpy(r->d_tri_block_mesh.p, r->tri_block_mesh.data(), r->tri_block_mesh.size() * 4,
                           hipMemcpyHostToDevice));
-    r->scene_dirty = false;
+    r->layout_version = scene_owner(r)->scene_version;
+    ++r->generation;
+    r->have_prev = false;
     return RAST_OK;
 }
 
@@ -507,13 +552,19 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->queue, (size_t)r->queue_cap * 4, &synced));
     HIP_TRY(grow(r, r->pool, (size_t)r->pool_cap * 4, &synced));
     HIP_TRY(grow(r, r->big, (size_t)r->big_cap * 4, &synced));
-    if (synced) r->have_prev = false;
+    if (synced) {  // some buffer moved: captured graphs hold stale pointers
+        r->have_prev = false;
+        ++r->generation;
+    }
     return RAST_OK;
 }
 
-static int stage_meshes(rast_renderer *r, const rast_frame_consts &fc)
+// Writes the frame's mesh table (matrices for this camera) and map stand-ins
+// into pinned staging memory; the device copies are enqueued by launch_all.
+static int stage_frame(rast_renderer *r, const rast_frame_consts &fc, const rast_frame_opts &o)
 {
-    const size_t n = r->meshes.size();
+    std::vector<rast_mesh *> &meshes = scene_meshes(r);
+    const size_t n = meshes.size();
     if (n > r->h_meshes_cap) {
         HIP_TRY(hipStreamSynchronize(r->stream));
         if (r->h_meshes) (void)hipHostFree(r->h_meshes);
@@ -523,14 +574,17 @@ static int stage_meshes(rast_renderer *r, const rast_frame_consts &fc)
         r->h_meshes_cap = cap;
         bool synced = true;
         HIP_TRY(grow(r, r->d_meshes, cap * sizeof(DevMesh), &synced));
-    } else {
-        // the previous frame's upload must have left the staging buffer
-        HIP_TRY(hipEventSynchronize(r->staging_done));
+        ++r->generation;
     }
+    // 1x1 stand-ins for missing maps (deviation D8), slot 0 unused
+    r->h_defaults[0] = 0xFFFFFFFFu;
+    r->h_defaults[1] = o.default_normal;
+    r->h_defaults[2] = o.default_specular;
+    r->h_defaults[3] = o.default_gloss;
     uint32_t vb = 0, vsb = 0, tsb = 0;
     const uint32_t *dflt = (const uint32_t *)r->defaults.p;
     for (size_t i = 0; i < n; ++i) {
-        const rast_mesh *m = r->meshes[i];
+        const rast_mesh *m = meshes[i];
         DevMesh &d = r->h_meshes[i];
         d.ivert = m->vert16;
         d.idx = m->idx;
@@ -544,16 +598,20 @@ static int stage_meshes(rast_renderer *r, const rast_frame_consts &fc)
         d.has_diffuse = m->tex[0] != nullptr;
         mat4_mul(fc.viewproj, m->world, d.wvp);  // CCR/CUDARenderer.cu:241
         memcpy(d.world, m->world, sizeof d.world);
-        for (int s = 0; s < 4; ++s) {
-            if (m->tex[s]) d.tex[s] = { m->tex[s]->texels, m->tex[s]->w, m->tex[s]->h };
-            else d.tex[s] = { dflt + s, 1u, 1u };
+        for (int q = 0; q < 3; ++q) {
+            d.cam_pos[q] = fc.cam_pos[q];
+            d.cam_fwd[q] = fc.cam_fwd[q];
+        }
+        d.cam_pos[3] = d.cam_fwd[3] = 0.f;
+        for (int q = 0; q < 4; ++q) {
+            if (m->tex[q]) d.tex[q] = { m->tex[q]->texels, m->tex[q]->w, m->tex[q]->h };
+            else d.tex[q] = { dflt + q, 1u, 1u };
         }
         vb += m->nv;
         vsb += ceil_div(m->nv, 256);
         tsb += ceil_div(m->ntri, 256);
     }
-    if (n) HIP_TRY(hipMemcpyAsync(r->d_meshes.p, r->h_meshes, n * sizeof(DevMesh), hipMemcpyHostToDevice, r->stream));
-    HIP_TRY(hipEventRecord(r->staging_done, r->stream));
+    r->staged_meshes = n;
     return RAST_OK;
 }
 
@@ -593,11 +651,9 @@ static FrameBuffers frame_buffers(rast_renderer *r)
     return b;
 }
 
-enum { EV_START, EV_VERTEX, EV_MODEL, EV_SETUP, EV_SCAN, EV_FILL, EV_FALLBACK, EV_RASTER, EV_SHADE, EV_COUNT };
-
-static int mark(rast_renderer *r, int which)
+static int mark(rast_renderer *r, int which, bool timing)
 {
-    if (!r->timing) return RAST_OK;
+    if (!timing) return RAST_OK;
     HIP_TRY(hipEventRecord(r->ev[which], r->stream));
     r->ev_used[which] = 1;
     return RAST_OK;
@@ -606,35 +662,35 @@ static int mark(rast_renderer *r, int which)
 static int validate_opts(const rast_frame_opts *o)
 {
     if (o->cull_mode < 0 || o->cull_mode > 2) return rast_set_error(RAST_E_INVALID_ARG, "cull_mode %d", o->cull_mode);
-    if (o->sample_state < 0 || o->sample_state > 1) return rast_set_error(RAST_E_INVALID_ARG, "sample_state %d", o->sample_state);
+    if (o->sample_state < 0 || o->sample_state > 1)
+        return rast_set_error(RAST_E_INVALID_ARG, "sample_state %d", o->sample_state);
     if (o->visualisation < 0 || o->visualisation > 2)
         return rast_set_error(RAST_E_INVALID_ARG, "visualisation %d", o->visualisation);
     if (o->bin_policy < 0 || o->bin_policy > 2) return rast_set_error(RAST_E_INVALID_ARG, "bin_policy %d", o->bin_policy);
     return RAST_OK;
 }
 
-static int enqueue_frame(rast_renderer *r)
+static int validate_camera(const rast_camera *cam)
 {
-    const rast_camera &cam = r->last_cam;
+    if (!(cam->near_z > 0.f) || !(cam->far_z > cam->near_z) || !(cam->fov_deg > 0.f && cam->fov_deg < 180.f))
+        return rast_set_error(RAST_E_INVALID_ARG, "camera near/far/fov out of range");
+    return RAST_OK;
+}
+
+// Host part of a frame: layout, storage, staging and the launch parameters.
+static int prepare_frame(rast_renderer *r)
+{
     const rast_frame_opts &o = r->last_opts;
     HIP_TRY(hipSetDevice(r->device));
-    if (r->scene_dirty) RAST_TRY(rebuild_layout(r));
+    if (scene_meshes(r).empty()) return rast_set_error(RAST_E_STATE, "rast_render: no meshes loaded");
+    if (r->layout_version != scene_owner(r)->scene_version) RAST_TRY(rebuild_layout(r));
     const uint32_t tx = ceil_div(r->W, r->tile), ty = ceil_div(r->H, r->tile), ntiles = tx * ty;
     RAST_TRY(ensure_frame_storage(r, ntiles));
     rast_frame_consts fc;
-    camera_frame(cam, r->W, r->H, fc);
-    // 1x1 stand-ins for missing maps (deviation D8), slot 0 unused
-    HIP_TRY(hipEventSynchronize(r->staging_done));
-    r->h_defaults[0] = 0xFFFFFFFFu;
-    r->h_defaults[1] = o.default_normal;
-    r->h_defaults[2] = o.default_specular;
-    r->h_defaults[3] = o.default_gloss;
-    HIP_TRY(hipMemcpyAsync(r->defaults.p, r->h_defaults, 16, hipMemcpyHostToDevice, r->stream));
-    RAST_TRY(stage_meshes(r, fc));
-
-    FrameParams fp{};
-    memcpy(fp.cam_pos, fc.cam_pos, sizeof fp.cam_pos);
-    memcpy(fp.cam_fwd, fc.cam_fwd, sizeof fp.cam_fwd);
+    camera_frame(r->last_cam, r->W, r->H, fc);
+    RAST_TRY(stage_frame(r, fc, o));
+    FrameParams &fp = r->fp;
+    fp = FrameParams{};
     fp.width = r->W;
     fp.height = r->H;
     fp.cull_mode = o.cull_mode;
@@ -648,66 +704,144 @@ static int enqueue_frame(rast_renderer *r)
     fp.micro_limit = r->micro_limit;
     fp.num_tris = r->total_tris;
     fp.num_verts = r->total_verts;
-    fp.num_meshes = (uint32_t)r->meshes.size();
-
-    r->timing = (o.flags & RAST_FLAG_TIMING) != 0;
-    memset(r->ev_used, 0, sizeof r->ev_used);
-    const bool had_prev = r->have_prev;
-    bool single = o.bin_policy != RAST_BIN_EXACT && had_prev;
+    fp.num_meshes = (uint32_t)scene_meshes(r).size();
+    r->frame_had_prev = r->have_prev;
+    bool single = o.bin_policy != RAST_BIN_EXACT && r->have_prev;
     if (o.bin_policy == RAST_BIN_LEARNED && !r->model) single = false;
-    FrameBuffers b = frame_buffers(r);
+    r->last_single = single;
+    r->fb = frame_buffers(r);
+    if (r->model) r->fb.density = (uint32_t *)r->density.p;  // feature of the capacity model
+    if (!single) {
+        r->fb.tile_pool = nullptr;
+        r->fb.tile_cap = nullptr;
+    }
+    return RAST_OK;
+}
+
+// Everything a frame puts on `s`: staging uploads, kernels, counters readback.
+// Graph-capturable (no memset, no host waits, events only when timing).
+static int launch_all(rast_renderer *r, hipStream_t s, bool timing)
+{
+    const rast_frame_opts &o = r->last_opts;
+    const FrameParams &fp = r->fp;
+    const FrameBuffers &b = r->fb;
+    const bool single = r->last_single;
+    const uint32_t ntiles = fp.tiles_x * fp.tiles_y;
     const uint32_t *prev_count = (const uint32_t *)r->tile_count[r->cur ^ 1].p;
     const float *prev_depth = (const float *)r->tile_min_depth[r->cur ^ 1].p;
-    if (r->model) b.density = (uint32_t *)r->density.p;  // feature of the capacity model
-
-    RAST_TRY(mark(r, EV_START));
-    const bool was_dirty = r->keys_dirty;
-    r->keys_dirty = true;  // until the frame's resolve pass has been enqueued
-    if (was_dirty) HIP_TRY(launch_clear_keys(b.keys, (size_t)r->W * r->H, r->stream));
-    HIP_TRY(launch_frame_init(fp, b, single, r->stream));
-    HIP_TRY(launch_vertex(fp, b, r->stream));
-    RAST_TRY(mark(r, EV_VERTEX));
+    HIP_TRY(hipMemcpyAsync(r->defaults.p, r->h_defaults, 16, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(r->d_meshes.p, r->h_meshes, r->staged_meshes * sizeof(DevMesh), hipMemcpyHostToDevice, s));
+    RAST_TRY(mark(r, EV_START, timing));
+    HIP_TRY(launch_frame_init(fp, b, single, s));
+    HIP_TRY(launch_vertex(fp, b, s));
+    RAST_TRY(mark(r, EV_VERTEX, timing));
     if (single) {
         if (o.bin_policy == RAST_BIN_LEARNED) {
             RAST_TRY(rast_model_capacities(r->model, fp, prev_count, prev_depth, b.density, (float *)r->features.p,
-                                           r->margin, r->slack, b.tile_cap, r->stream));
+                                           r->margin, r->slack, b.tile_cap, s));
         } else {
-            HIP_TRY(launch_prevframe_capacity(ntiles, prev_count, r->margin, r->slack, b.tile_cap, r->stream));
+            HIP_TRY(launch_prevframe_capacity(ntiles, prev_count, r->margin, r->slack, b.tile_cap, s));
         }
-        RAST_TRY(mark(r, EV_MODEL));
-        HIP_TRY(launch_scan(ntiles, b.tile_cap, b.tile_off, nullptr, r->queue_cap, b.counters, true, r->stream));
-        RAST_TRY(mark(r, EV_SCAN));
-        HIP_TRY(launch_setup(fp, b, true, r->stream));
-        RAST_TRY(mark(r, EV_SETUP));
-        HIP_TRY(launch_fallback(fp, b, r->stream));
-        RAST_TRY(mark(r, EV_FALLBACK));
+        RAST_TRY(mark(r, EV_MODEL, timing));
+        HIP_TRY(launch_scan(ntiles, b.tile_cap, b.tile_off, nullptr, r->queue_cap, b.counters, true, s));
+        RAST_TRY(mark(r, EV_SCAN, timing));
+        HIP_TRY(launch_setup(fp, b, true, s));
+        RAST_TRY(mark(r, EV_SETUP, timing));
+        HIP_TRY(launch_fallback(fp, b, s));
+        RAST_TRY(mark(r, EV_FALLBACK, timing));
     } else {
-        b.tile_pool = nullptr;
-        b.tile_cap = nullptr;
-        HIP_TRY(launch_setup(fp, b, false, r->stream));
-        RAST_TRY(mark(r, EV_SETUP));
-        HIP_TRY(launch_scan(ntiles, b.tile_count, b.tile_off, b.tile_cursor, r->queue_cap, b.counters, false,
-                            r->stream));
-        RAST_TRY(mark(r, EV_SCAN));
-        HIP_TRY(launch_fill(fp, b, r->stream));
-        RAST_TRY(mark(r, EV_FILL));
+        HIP_TRY(launch_setup(fp, b, false, s));
+        RAST_TRY(mark(r, EV_SETUP, timing));
+        HIP_TRY(launch_scan(ntiles, b.tile_count, b.tile_off, b.tile_cursor, r->queue_cap, b.counters, false, s));
+        RAST_TRY(mark(r, EV_SCAN, timing));
+        HIP_TRY(launch_fill(fp, b, s));
+        RAST_TRY(mark(r, EV_FILL, timing));
     }
-    HIP_TRY(launch_raster(fp, b, r->small_limit, r->stream));
-    r->keys_dirty = false;
-    RAST_TRY(mark(r, EV_RASTER));
+    HIP_TRY(launch_raster(fp, b, r->small_limit, s));
+    RAST_TRY(mark(r, EV_RASTER, timing));
     if (!(o.flags & RAST_FLAG_NO_SHADE)) {
-        HIP_TRY(launch_shade(fp, b, r->stream));
-        RAST_TRY(mark(r, EV_SHADE));
+        HIP_TRY(launch_shade(fp, b, s));
+        RAST_TRY(mark(r, EV_SHADE, timing));
     }
-    HIP_TRY(hipMemcpyAsync(r->h_counters, b.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipMemcpyAsync(r->h_counters, b.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, s));
+    return RAST_OK;
+}
+
+// Identity of a captured frame: everything launch_all bakes into the graph.
+static uint64_t frame_key(rast_renderer *r)
+{
+    const rast_frame_opts &o = r->last_opts;
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+    mix((uint64_t)o.cull_mode | (uint64_t)o.sample_state << 4 | (uint64_t)o.visualisation << 8 |
+        (uint64_t)o.bin_policy << 12 | (uint64_t)(o.flags & RAST_FLAG_NO_SHADE) << 16 | (uint64_t)r->last_single << 20);
+    mix(o.clear_rgba);
+    mix((uint64_t)r->W << 32 | r->H);
+    mix((uint64_t)r->tile << 32 | r->big_threshold);
+    mix((uint64_t)r->micro_limit << 32 | r->small_limit);
+    mix(r->generation);
+    mix((uint64_t)r->cur);
+    mix(r->staged_meshes);
+    mix((uint64_t)(uintptr_t)r->model);
+    mix((uint64_t)(r->margin * 65536.f) << 32 | r->slack);
+    return h;
+}
+
+static void finish_enqueue(rast_renderer *r)
+{
     r->stats.num_triangles = r->total_tris;
-    r->stats.num_tiles = ntiles;
+    r->stats.num_tiles = r->fp.tiles_x * r->fp.tiles_y;
     r->frame_pending = true;
     r->frame_valid = false;
     r->have_prev = true;
-    r->frame_had_prev = had_prev;
     r->cur ^= 1;
-    r->last_single = single;
+}
+
+static int clear_keys_if_dirty(rast_renderer *r, bool wait)
+{
+    if (!r->keys_dirty) return RAST_OK;
+    HIP_TRY(launch_clear_keys((unsigned long long *)r->keys.p, (size_t)r->W * r->H, r->stream));
+    if (wait) HIP_TRY(hipStreamSynchronize(r->stream));
+    r->keys_dirty = false;
+    return RAST_OK;
+}
+
+static int enqueue_frame(rast_renderer *r)
+{
+    const rast_frame_opts &o = r->last_opts;
+    RAST_TRY(prepare_frame(r));
+    const bool timing = (o.flags & RAST_FLAG_TIMING) != 0;
+    const bool graph = (o.flags & RAST_FLAG_GRAPH) != 0 && !timing;
+    r->timing = timing;
+    memset(r->ev_used, 0, sizeof r->ev_used);
+    RAST_TRY(clear_keys_if_dirty(r, false));
+    r->keys_dirty = true;  // until the frame (whose resolve pass resets the keys) is enqueued
+    if (graph) {
+        const uint64_t key = frame_key(r);
+        FrameGraph &g = r->graphs[r->cur];
+        if (!g.exec || g.key != key) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_all(r, r->stream, false);
+            hipGraph_t cap = nullptr;
+            const hipError_t e = hipStreamEndCapture(r->stream, &cap);
+            if (rc != RAST_OK) {
+                if (cap) (void)hipGraphDestroy(cap);
+                return rc;
+            }
+            HIP_TRY(e);
+            const hipError_t ie = hipGraphInstantiate(&g.exec, cap, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(cap);
+            HIP_TRY(ie);
+            g.key = key;
+        }
+        HIP_TRY(hipGraphLaunch(g.exec, r->stream));
+    } else {
+        RAST_TRY(launch_all(r, r->stream, timing));
+    }
+    r->keys_dirty = false;
+    finish_enqueue(r);
     return RAST_OK;
 }
 
@@ -739,13 +873,11 @@ extern "C" int rast_render(rast_renderer *r, const rast_camera *cam, const rast_
 {
     if (!r || !cam || !opts) return rast_set_error(RAST_E_INVALID_ARG, "rast_render: null argument");
     RAST_TRY(validate_opts(opts));
-    if (!(cam->near_z > 0.f) || !(cam->far_z > cam->near_z) || !(cam->fov_deg > 0.f && cam->fov_deg < 180.f))
-        return rast_set_error(RAST_E_INVALID_ARG, "camera near/far/fov out of range");
+    RAST_TRY(validate_camera(cam));
     if (r->frame_pending) RAST_TRY(check_frame(r));
     r->last_cam = *cam;
     r->last_opts = *opts;
     r->prev_valid_before = r->have_prev;
-    if (r->meshes.empty()) return rast_set_error(RAST_E_STATE, "rast_render: no meshes loaded");
     return enqueue_frame(r);
 }
 
@@ -859,16 +991,138 @@ extern "C" int rast_renderer_set_binmodel(rast_renderer *r, rast_binmodel *m, fl
     r->model = m;
     r->margin = margin;
     r->slack = slack;
+    ++r->generation;  // captured frames hold the previous model's parameter pointer
+    return RAST_OK;
+}
+
+// ----------------------------------------------------------------- batching
+
+// Cached batch graphs, each keyed by its jobs' renderers and frame keys.
+// Renderers alternate between two count buffers, so a repeated batch needs
+// two entries; a few more cover alternating job lists.  Renderer destruction
+// drops the entries that use it (a graph holds its buffers).
+struct BatchGraph {
+    std::vector<std::pair<rast_renderer *, uint64_t>> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+};
+struct BatchCache {
+    BatchGraph entries[4];
+    uint64_t clock = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;
+    int device = -1;
+};
+static BatchCache g_batch;
+static std::mutex g_batch_mutex;
+
+static void drop_batch_graph(BatchGraph &g)
+{
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.key.clear();
+}
+
+void rast_forget_renderer(rast_renderer *r)
+{
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    for (BatchGraph &g : g_batch.entries)
+        for (auto &kv : g.key)
+            if (kv.first == r) {
+                if (g_batch.stream) (void)hipStreamSynchronize(g_batch.stream);
+                drop_batch_graph(g);
+                break;
+            }
+}
+
+static int capture_batch(const rast_job *jobs, uint32_t n, hipGraphExec_t *exec)
+{
+    HIP_TRY(hipStreamBeginCapture(g_batch.stream, hipStreamCaptureModeThreadLocal));
+    int rc = RAST_OK;
+    hipError_t e = hipEventRecord(g_batch.fork, g_batch.stream);
+    for (uint32_t i = 0; i < n && rc == RAST_OK && e == hipSuccess; ++i) {
+        rast_renderer *t = jobs[i].target;  // one branch per job, forked from and joined to the batch stream
+        e = hipStreamWaitEvent(t->stream, g_batch.fork, 0);
+        if (e == hipSuccess) rc = launch_all(t, t->stream, false);
+        if (rc == RAST_OK && e == hipSuccess) e = hipEventRecord(t->join, t->stream);
+        if (rc == RAST_OK && e == hipSuccess) e = hipStreamWaitEvent(g_batch.stream, t->join, 0);
+    }
+    hipGraph_t cap = nullptr;
+    const hipError_t ee = hipStreamEndCapture(g_batch.stream, &cap);
+    if (rc != RAST_OK || e != hipSuccess || ee != hipSuccess) {
+        if (cap) (void)hipGraphDestroy(cap);
+        if (rc != RAST_OK) return rc;
+        HIP_TRY(e != hipSuccess ? e : ee);
+    }
+    const hipError_t ie = hipGraphInstantiate(exec, cap, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(cap);
+    HIP_TRY(ie);
     return RAST_OK;
 }
 
 extern "C" int rast_render_batch(const rast_job *jobs, uint32_t n)
 {
-    if (!jobs && n) return rast_set_error(RAST_E_INVALID_ARG, "rast_render_batch: null jobs");
+    if (!jobs || n == 0) return rast_set_error(RAST_E_INVALID_ARG, "rast_render_batch: no jobs");
+    const int device = jobs[0].target ? jobs[0].target->device : 0;
     for (uint32_t i = 0; i < n; ++i) {
-        if (!jobs[i].target) return rast_set_error(RAST_E_INVALID_ARG, "job %u has no target renderer", i);
-        RAST_TRY(rast_render(jobs[i].target, &jobs[i].camera, &jobs[i].opts));
+        rast_renderer *t = jobs[i].target;
+        if (!t) return rast_set_error(RAST_E_INVALID_ARG, "job %u has no target renderer", i);
+        if (t->device != device) return rast_set_error(RAST_E_INVALID_ARG, "job %u is on another device", i);
+        for (uint32_t j = 0; j < i; ++j)
+            if (jobs[j].target == t) return rast_set_error(RAST_E_INVALID_ARG, "jobs %u and %u share a renderer", j, i);
+        RAST_TRY(validate_opts(&jobs[i].opts));
+        RAST_TRY(validate_camera(&jobs[i].camera));
     }
-    for (uint32_t i = 0; i < n; ++i) RAST_TRY(rast_sync(jobs[i].target));
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    HIP_TRY(hipSetDevice(device));
+    if (g_batch.device != device) {
+        if (g_batch.stream) {
+            (void)hipStreamSynchronize(g_batch.stream);
+            for (BatchGraph &g : g_batch.entries) drop_batch_graph(g);
+            (void)hipStreamDestroy(g_batch.stream);
+            (void)hipEventDestroy(g_batch.fork);
+            (void)hipEventDestroy(g_batch.done);
+            g_batch.stream = nullptr;
+        }
+        HIP_TRY(hipStreamCreateWithFlags(&g_batch.stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&g_batch.fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&g_batch.done, hipEventDisableTiming));
+        g_batch.device = device;
+    }
+    // host part of every job; each renderer's previous frame is checked first
+    std::vector<std::pair<rast_renderer *, uint64_t>> key(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        rast_renderer *t = jobs[i].target;
+        if (t->frame_pending) RAST_TRY(check_frame(t));
+        t->last_cam = jobs[i].camera;
+        t->last_opts = jobs[i].opts;
+        t->last_opts.flags &= ~RAST_FLAG_TIMING;
+        t->prev_valid_before = t->have_prev;
+        RAST_TRY(prepare_frame(t));
+        RAST_TRY(clear_keys_if_dirty(t, false));
+        HIP_TRY(hipEventRecord(t->join, t->stream));  // the batch follows earlier work on t
+        HIP_TRY(hipStreamWaitEvent(g_batch.stream, t->join, 0));
+        key[i] = { t, frame_key(t) };
+    }
+    BatchGraph *hit = nullptr, *victim = &g_batch.entries[0];
+    for (BatchGraph &g : g_batch.entries) {
+        if (g.exec && g.key == key) hit = &g;
+        if (!g.exec || (victim->exec && g.last_use < victim->last_use)) victim = &g;
+    }
+    if (!hit) {
+        hit = victim;
+        drop_batch_graph(*hit);
+        RAST_TRY(capture_batch(jobs, n, &hit->exec));
+        hit->key = key;
+    }
+    hit->last_use = ++g_batch.clock;
+    HIP_TRY(hipGraphLaunch(hit->exec, g_batch.stream));
+    HIP_TRY(hipEventRecord(g_batch.done, g_batch.stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        rast_renderer *t = jobs[i].target;
+        HIP_TRY(hipStreamWaitEvent(t->stream, g_batch.done, 0));  // later work on t follows the batch
+        memset(t->ev_used, 0, sizeof t->ev_used);
+        finish_enqueue(t);
+    }
     return RAST_OK;
 }

@@ -47,12 +47,14 @@ struct DevMesh {
     int32_t has_diffuse;
     float wvp[16];            // data[col][row]
     float world[16];
+    float cam_pos[4];         // camera position (xyz) and forward (xyz) of the frame
+    float cam_fwd[4];
     DevTexture tex[4];        // diffuse, normal, specular, gloss (stand-ins filled in)
 };
 
+// Per-frame values that do not depend on the camera (the camera travels in
+// DevMesh), so a captured frame graph stays valid while the camera moves.
 struct FrameParams {
-    float cam_pos[3];
-    float cam_fwd[3];
     uint32_t width, height;
     int32_t cull_mode;
     int32_t sample_state;

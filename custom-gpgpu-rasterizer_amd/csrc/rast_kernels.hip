@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
     const float w0 = m4(Wd, 0, 0) * px + m4(Wd, 0, 1) * py + m4(Wd, 0, 2) * pz + m4(Wd, 0, 3);
     const float w1 = m4(Wd, 1, 0) * px + m4(Wd, 1, 1) * py + m4(Wd, 1, 2) * pz + m4(Wd, 1, 3);
     const float w2 = m4(Wd, 2, 0) * px + m4(Wd, 2, 1) * py + m4(Wd, 2, 2) * pz + m4(Wd, 2, 3);
-    float vx = w0 - fp.cam_pos[0], vy = w1 - fp.cam_pos[1], vz = w2 - fp.cam_pos[2];
+    float vx = w0 - M.cam_pos[0], vy = w1 - M.cam_pos[1], vz = w2 - M.cam_pos[2];
     normalize3(vx, vy, vz);
     // PBR shading reads the first three float4s only; the colour is last
     float4 *a = attr + (size_t)g * 4;
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
             float fy = e1z * e2x - e1x * e2z;
             float fz = e1x * e2y - e1y * e2x;
             normalize3(fx, fy, fz);
-            const float cv = fp.cam_fwd[0] * fx + fp.cam_fwd[1] * fy + fp.cam_fwd[2] * fz;
+            const float cv = M.cam_fwd[0] * fx + M.cam_fwd[1] * fy + M.cam_fwd[2] * fz;
             ok = fp.cull_mode == 0 ? !(cv <= 0.f) : !(cv >= 0.f);
         }
         // IsTriangleVisible (CCR/CUDAROPs.cu:147-178)

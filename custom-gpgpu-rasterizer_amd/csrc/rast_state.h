@@ -8,6 +8,7 @@
 
 #include "rast.h"
 #include "rast_internal.h"
+#include "rast_launch.h"
 
 // Records the calling thread's last error message (rast_last_error) and
 // returns `code`.
@@ -26,6 +27,11 @@ int rast_set_error(int code, const char *fmt, ...);
         int rc_ = (expr);               \
         if (rc_ != RAST_OK) return rc_; \
     } while (0)
+
+using rast::FrameBuffers;
+
+// Drops cached batch graphs that refer to r (before r is destroyed).
+void rast_forget_renderer(rast_renderer *r);
 
 struct DBuf {
     void *p = nullptr;
@@ -56,6 +62,11 @@ struct rast_mesh {
     uint64_t tri_base = 0;
 };
 
+struct FrameGraph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t key = 0;
+};
+
 struct rast_renderer {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -63,14 +74,22 @@ struct rast_renderer {
     uint32_t tile = 16, big_threshold = 256, small_limit = 16;
     std::vector<rast_mesh *> meshes;
     std::vector<rast_texture *> textures;
-    bool scene_dirty = true;
+    rast_renderer *scene_src = nullptr;   // set for views: the renderer whose meshes they draw
+    std::vector<rast_renderer *> views;    // views drawing this renderer's meshes
+    uint64_t scene_version = 1;            // bumped when the mesh list changes
+    uint64_t layout_version = 0;           // scene version the block tables were built for
+    uint64_t generation = 0;               // bumped whenever a device buffer moves
+    FrameGraph graphs[2];                  // captured frame per count-buffer parity
+    hipEvent_t join = nullptr;             // batch capture: end of this renderer's branch
+    rast::FrameParams fp{};
+    FrameBuffers fb{};
+    size_t staged_meshes = 0;
     // scene layout
     uint32_t total_verts = 0, total_tris = 0, vs_blocks = 0, tri_blocks = 0;
     std::vector<uint32_t> vs_block_mesh, tri_block_mesh;
     DBuf d_vs_block_mesh, d_tri_block_mesh, d_meshes;
     rast::DevMesh *h_meshes = nullptr;  // pinned staging
     size_t h_meshes_cap = 0;
-    hipEvent_t staging_done = nullptr;
     // frame buffers
     DBuf pos4, attr, trirec, alive, survivors;
     DBuf tile_count[2], tile_off, tile_cap, tile_pool, tile_cursor, density, tile_min_depth[2], features;

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "librast.so")
 RAST_OK = 0
 VIS_NONE = 0xFFFFFFFF
 BIN_EXACT, BIN_LEARNED, BIN_PREVFRAME = 0, 1, 2
-FLAG_TIMING, FLAG_NO_SHADE = 1, 2
+FLAG_TIMING, FLAG_NO_SHADE, FLAG_GRAPH = 1, 2, 4
 DEBUG_VERTEX_POS, DEBUG_VERTEX_ATTR, DEBUG_TRI_RECORDS, DEBUG_TRI_ALIVE, DEBUG_TILE_COUNTS, DEBUG_BIG_LIST = range(6)
 
 EXPORTS = [
@@ -34,7 +34,7 @@ EXPORTS = [
     "rast_debug_read", "rast_binmodel_create", "rast_binmodel_destroy", "rast_binmodel_num_features",
     "rast_renderer_set_binmodel", "rast_binmodel_record", "rast_binmodel_clear_data", "rast_binmodel_train",
     "rast_binmodel_get_params", "rast_binmodel_set_params", "rast_binmodel_num_params", "rast_binmodel_forward",
-    "rast_binmodel_loss_grad", "rast_render_batch",
+    "rast_binmodel_loss_grad", "rast_render_batch", "rast_renderer_create_view",
 ]
 
 
@@ -93,6 +93,7 @@ def lib():
         L.rast_device_query.argtypes = [C.c_int, C.POINTER(DeviceInfo)]
         L.rast_renderer_create.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(vp)]
         L.rast_renderer_destroy.argtypes = [vp]
+        L.rast_renderer_create_view.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]
         L.rast_setup_bins.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.rast_resize.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.rast_set_tuning.argtypes = [vp, C.c_uint32, C.c_uint32]
@@ -173,12 +174,22 @@ def device_info(device: int = 0) -> dict:
 class Renderer:
     """CUDARenderer equivalent over the C ABI."""
 
-    def __init__(self, width: int, height: int, device: int = 0, tile: int = 16, big_threshold: int = 256):
+    def __init__(self, width: int, height: int, device: int = 0, tile: int = 16, big_threshold: int = 256,
+                 owner: Optional["Renderer"] = None):
         self.h = C.c_void_p()
-        check(lib().rast_renderer_create(device, width, height, C.byref(self.h)), "rast_renderer_create")
+        if owner is None:
+            check(lib().rast_renderer_create(device, width, height, C.byref(self.h)), "rast_renderer_create")
+        else:
+            check(lib().rast_renderer_create_view(owner.h, width, height, C.byref(self.h)),
+                  "rast_renderer_create_view")
         self.width, self.height = width, height
         self.meshes, self.textures = [], {}
+        self.owner = owner
         self.setup_bins(tile, big_threshold)
+
+    def view(self, width: int, height: int, tile: int = 16, big_threshold: int = 256) -> "Renderer":
+        """Another framebuffer drawing this renderer's meshes."""
+        return Renderer(width, height, tile=tile, big_threshold=big_threshold, owner=self)
 
     def setup_bins(self, tile: int, big_threshold: int = 256) -> None:
         check(lib().rast_setup_bins(self.h, tile, big_threshold), "rast_setup_bins")
@@ -312,6 +323,16 @@ class BinModel:
         if self.h:
             check(lib().rast_binmodel_destroy(self.h), "rast_binmodel_destroy")
             self.h = C.c_void_p()
+
+
+def render_batch(jobs) -> None:
+    """Enqueue [(renderer, camera, opts, policy, flags), ...] as one graph launch."""
+    arr = (Job * len(jobs))()
+    for j, (r, cam, opts, policy, flags) in enumerate(jobs):
+        arr[j].target = r.h
+        arr[j].camera = camera_struct(cam)
+        arr[j].opts = opts_struct(opts, policy, flags)
+    check(lib().rast_render_batch(arr, len(jobs)), "rast_render_batch")
 
 
 def render_workload(w: S.Workload, policy: int = BIN_EXACT, tile: Optional[int] = None):

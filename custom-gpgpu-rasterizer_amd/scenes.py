@@ -8,7 +8,9 @@ that the GPU path and the CPU oracle consume byte-identical inputs.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -243,3 +245,117 @@ def config3(n=1024, tex_size=2048, width=3840, height=2160, camera: Optional[Cam
 def train_test_paths(train=200, test=100):
     """Disjoint seeded camera paths over the terrain (different seeds)."""
     return (camera_path(SEED_BASE + 0x300, train), camera_path(SEED_BASE + 0x301, test))
+
+
+@dataclass
+class Job:
+    """One render request of a batched workload."""
+    name: str
+    scene: Scene
+    camera: Camera
+    width: int
+    height: int
+    opts: Opts = field(default_factory=Opts)
+
+    @property
+    def pixels(self) -> int:
+        return self.width * self.height
+
+
+def orbit_camera(yaw: float, elevation: float, distance: float, fov_deg=45.0) -> Camera:
+    """Camera on a sphere around the origin, looking at it (the reference's
+    forward points from the target to the eye)."""
+    ce = lib().rsc_cos(elevation)
+    d = (ce * lib().rsc_sin(yaw), lib().rsc_sin(elevation), ce * lib().rsc_cos(yaw))
+    return Camera(position=tuple(float(distance * q) for q in d), forward=tuple(float(q) for q in d),
+                  fov_deg=fov_deg, near_z=0.1, far_z=100.0)
+
+
+def config4(jobs=256, tex_size=512):
+    """256 uneven render requests: resolution per axis log-uniform in
+    [64, 1024]; each job one seeded icosphere (subdivision 3-6, 1,280-81,920
+    tris) or terrain (2n^2 tris log-uniform in [1k, 200k]) with its own
+    camera and a tex_size^2 RGBA8 noise texture."""
+    seed = SEED_BASE + 4
+    # the generators release the GIL inside ctypes calls: build textures in parallel
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        texs = list(ex.map(lambda j: noise_texture(tex_size, tex_size, seed + 1000 + j, 0), range(jobs)))
+    ico = {}
+    out = []
+    for j in range(jobs):
+        def u(q, lo, hi):
+            return uniform(seed, 16 * j + q, lo, hi)
+        W = int(round(math.exp(u(0, math.log(64), math.log(1024)))))
+        H = int(round(math.exp(u(1, math.log(64), math.log(1024)))))
+        tex = texs[j]
+        if u(2, 0.0, 1.0) < 0.5:
+            sub = min(3 + int(u(3, 0.0, 4.0)), 6)
+            if sub not in ico:
+                ico[sub] = icosphere(sub, 1.0)
+            base = ico[sub]
+            m = Mesh(verts=base.verts, indices=base.indices, textures=[tex, None, None, None])
+            cam = orbit_camera(u(4, 0.0, 6.2831853), u(5, -0.9, 0.9), u(6, 2.2, 6.0))
+            name = f"ico{sub}"
+        else:
+            tris = math.exp(u(3, math.log(1000), math.log(200000)))
+            n = max(int(round(math.sqrt(tris / 2))), 23)
+            m = terrain(n, 80.0, 6.0, 6, seed + 2000 + j, 1.0)
+            m.textures = [tex, None, None, None]
+            cam = camera_path(seed + 3000 + j, 1)[0]
+            name = f"terrain{n}"
+        out.append(Job(f"{name}_{W}x{H}", Scene([m]), cam, W, H))
+    return out
+
+
+def cube_directions():
+    """Face directions of a cube map whose body diagonal is vertical, so no
+    face looks straight up or down (the reference camera builds its frame from
+    the world up axis, which a face direction must not be parallel to)."""
+    a = math.atan(1.0 / math.sqrt(2.0))  # 35.26 deg: elevation of each face normal
+    dirs = []
+    for k in range(3):
+        yaw = 2.0 * math.pi * k / 3.0
+        ce, se = math.cos(a), math.sin(a)
+        d = (ce * math.sin(yaw), se, ce * math.cos(yaw))
+        dirs.append(d)
+        dirs.append(tuple(-q for q in d))
+    return dirs
+
+
+@dataclass
+class Flythrough:
+    scene: Scene
+    path: List[Camera]
+    main: tuple        # (width, height) of the main view
+    face: int          # cube face size
+    opts: Opts = field(default_factory=Opts)
+
+    def views(self, frame: int):
+        """(camera, width, height) of the main view and the six cube faces."""
+        cam = self.path[frame]
+        out = [(cam, self.main[0], self.main[1])]
+        for d in cube_directions():
+            out.append((Camera(position=cam.position, forward=d, fov_deg=90.0, near_z=0.1, far_z=100.0),
+                        self.face, self.face))
+        return out
+
+
+def config5(frames=300, n=1160, spheres=64, subdiv=5, tex_size=2048, main=(3840, 2160), face=1024) -> Flythrough:
+    """Terrain of n^2 quads (1160^2: 2,691,200 tris) plus 64 instanced
+    subdivision-5 icospheres (1,310,720 tris), 4,001,920 tris in all, with
+    four tex_size^2 maps; a seeded camera path; per frame one main view and
+    six cube-map faces."""
+    seed = SEED_BASE + 5
+    maps = noise_maps(tex_size, seed)
+    ter = terrain(n, 80.0, 6.0, 6, seed, 1.0)
+    ter.textures = maps
+    base = icosphere(subdiv, 1.0)
+    meshes = [ter]
+    for k in range(spheres):
+        t = (uniform(seed, 4 * k, -34.0, 34.0), uniform(seed, 4 * k + 1, 7.5, 14.0),
+             uniform(seed, 4 * k + 2, -34.0, 34.0))
+        e = tuple(uniform(seed, 1000 + 3 * k + q, 0.0, 6.2831853) for q in range(3))
+        sc = uniform(seed, 4 * k + 3, 1.0, 2.5)
+        meshes.append(Mesh(verts=base.verts, indices=base.indices, world=world_matrix(t, e, sc), textures=maps))
+    path = camera_path(seed + 0x100, frames)
+    return Flythrough(Scene(meshes), path, main, face)

@@ -97,6 +97,11 @@ typedef struct {
 
 #define RAST_FLAG_TIMING 1u   /* record per-stage HIP events (read with rast_get_stats) */
 #define RAST_FLAG_NO_SHADE 2u /* stop after visibility + depth */
+/* Replay the frame from a captured HIP graph (captured on first use, kept per
+ * options, sizes and tuning, re-captured when a device buffer moves).  Has no
+ * reference counterpart: it replaces the per-stage launches of
+ * CCR/CUDARenderer.cu:237-330 with one graph launch.  Ignored with TIMING. */
+#define RAST_FLAG_GRAPH 4u
 
 /* Frame statistics: CCR/CUDARenderer.h:63-64 (visible / total triangles) and
  * the per-stage milliseconds of the BENCHMARK build (CCR/CUDARenderer.cu:344-353). */
@@ -144,6 +149,12 @@ int rast_device_query(int device, rast_device_info *out);
  * (CCR/CUDARenderer.h:34, CCR/CUDARenderer.cu:32-45,413-475): a renderer for a
  * width x height framebuffer on HIP device `device`. */
 int rast_renderer_create(int device, uint32_t width, uint32_t height, rast_renderer **out);
+/* A second framebuffer over the meshes of `owner` (no reference counterpart:
+ * the reference renders one view per renderer; cf. LoadScene
+ * CCR/CUDARenderer.cu:57-75).  Views own their frame buffers, bins, capacity
+ * model and captured graphs; meshes and textures stay with the owner, which
+ * cannot be destroyed while views exist.  Creating meshes on a view fails. */
+int rast_renderer_create_view(rast_renderer *owner, uint32_t width, uint32_t height, rast_renderer **out);
 /* ~CUDARenderer (CCR/CUDARenderer.cu:47-52): waits for the stream, frees all
  * meshes, textures and buffers of the renderer. */
 int rast_renderer_destroy(rast_renderer *r);
@@ -254,7 +265,12 @@ int rast_binmodel_loss_grad(rast_binmodel *m, const float *x, const float *targe
 /* ----------------------------------------------------- batched rendering */
 /* One render job of a batch: no reference counterpart; it realises the
  * "persistent kernel approach" TODO of CCR/CUDARenderer.cu:284-286 and the
- * per-mesh host loop of CCR/CUDARenderer.cu:237-330 as one launch per stage. */
+ * per-mesh host loop of CCR/CUDARenderer.cu:237-330.  rast_render_batch
+ * enqueues all jobs as one captured HIP graph (one independent branch per
+ * job, so small frames overlap on the device); the graph is cached per job
+ * list and options.  Every job needs its own renderer (views of one scene
+ * are fine); timing flags are ignored.  Results are read per renderer with
+ * rast_readback / rast_get_stats as after rast_render. */
 typedef struct {
     rast_renderer *target;   /* renderer providing the framebuffer and meshes */
     rast_camera camera;
