@@ -188,7 +188,7 @@ extern "C" int rast_camera_frame(const rast_camera *cam, uint32_t width, uint32_
 }
 
 // Timing events of one frame, recorded between the stages when RAST_FLAG_TIMING is set.
-enum { EV_START, EV_VERTEX, EV_MODEL, EV_SETUP, EV_SCAN, EV_FILL, EV_FALLBACK, EV_RASTER, EV_SHADE, EV_COUNT };
+enum { EV_START, EV_VERTEX, EV_MODEL, EV_SCANCAP, EV_SETUP, EV_SCANCNT, EV_FILL, EV_FALLBACK, EV_RASTER, EV_SHADE, EV_COUNT };
 
 static int alloc_framebuffers(rast_renderer *r)
 {
@@ -200,7 +200,6 @@ static int alloc_framebuffers(rast_renderer *r)
     HIP_TRY(grow(r, r->keys, P * 8, &synced));
     r->keys_dirty = true;
     r->frame_valid = false;
-    ++r->generation;
     return RAST_OK;
 }
 
@@ -225,7 +224,8 @@ static int new_renderer(int device, uint32_t width, uint32_t height, rast_render
     for (int i = 0; rc == RAST_OK && i < EV_COUNT; ++i)
         if (hipEventCreate(&r->ev[i]) != hipSuccess) rc = RAST_E_HIP;
     if (rc == RAST_OK && hipHostMalloc((void **)&r->h_counters, sizeof(FrameCounters)) != hipSuccess) rc = RAST_E_HIP;
-    if (rc == RAST_OK && hipHostMalloc((void **)&r->h_defaults, 16) != hipSuccess) rc = RAST_E_HIP;
+    if (rc == RAST_OK && hipHostGetDevicePointer((void **)&r->d_counters_host, r->h_counters, 0) != hipSuccess)
+        rc = RAST_E_HIP;
     if (rc == RAST_OK) rc = alloc_framebuffers(r);
     if (rc != RAST_OK) {
         rast_renderer_destroy(r);
@@ -267,28 +267,24 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
                               r->views.size());
     (void)hipSetDevice(r->device);
     if (r->stream) (void)hipStreamSynchronize(r->stream);
-    rast_forget_renderer(r);
     if (r->scene_src) {
         std::vector<rast_renderer *> &v = r->scene_src->views;
         v.erase(std::remove(v.begin(), v.end(), r), v.end());
     }
-    for (FrameGraph &g : r->graphs)
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    plan_free(r->plan);
     rast_model_detach_renderer(r->model, r);
     for (rast_mesh *m : r->meshes) free_mesh(m);
     for (rast_texture *t : r->textures) {
         if (t->texels) (void)hipFree(t->texels);
         delete t;
     }
-    DBuf *bufs[] = { &r->d_vs_block_mesh, &r->d_tri_block_mesh, &r->d_meshes, &r->pos4, &r->attr, &r->trirec,
+    DBuf *bufs[] = { &r->pos4, &r->attr, &r->trirec,
                      &r->alive, &r->survivors, &r->tile_count[0], &r->tile_count[1], &r->tile_off, &r->tile_cap,
                      &r->tile_pool, &r->tile_cursor, &r->density, &r->tile_min_depth[0], &r->tile_min_depth[1], &r->keys,
-                     &r->features, &r->queue, &r->pool, &r->big, &r->ovf, &r->counters, &r->defaults, &r->vis, &r->depth,
+                     &r->queue, &r->pool, &r->big, &r->ovf, &r->counters, &r->vis, &r->depth,
                      &r->rgba };
     for (DBuf *b : bufs) dfree(*b);
-    if (r->h_meshes) (void)hipHostFree(r->h_meshes);
     if (r->h_counters) (void)hipHostFree(r->h_counters);
-    if (r->h_defaults) (void)hipHostFree(r->h_defaults);
     for (hipEvent_t &e : r->ev)
         if (e) (void)hipEventDestroy(e);
     if (r->join) (void)hipEventDestroy(r->join);
@@ -483,43 +479,22 @@ extern "C" int rast_texture_destroy(rast_texture *t)
 // ------------------------------------------------------------------ frame
 
 // Renderers created with rast_renderer_create_view draw the meshes of their
-// owner; everything per frame (buffers, staging, graphs) is their own.
+// owner; everything per frame (buffers, launch plan) is their own.
 static std::vector<rast_mesh *> &scene_meshes(rast_renderer *r) { return r->scene_src ? r->scene_src->meshes : r->meshes; }
 static rast_renderer *scene_owner(rast_renderer *r) { return r->scene_src ? r->scene_src : r; }
 
-static int rebuild_layout(rast_renderer *r)
+static void refresh_layout(rast_renderer *r)
 {
-    std::vector<rast_mesh *> &meshes = scene_meshes(r);
     uint64_t vb = 0, tb = 0;
-    r->vs_block_mesh.clear();
-    r->tri_block_mesh.clear();
-    for (size_t i = 0; i < meshes.size(); ++i) {
-        rast_mesh *m = meshes[i];
+    for (rast_mesh *m : scene_meshes(r)) {
         m->tri_base = tb;
-        const uint32_t nvb = ceil_div(m->nv, 256), ntb = ceil_div(m->ntri, 256);
-        r->vs_block_mesh.insert(r->vs_block_mesh.end(), nvb, (uint32_t)i);
-        r->tri_block_mesh.insert(r->tri_block_mesh.end(), ntb, (uint32_t)i);
         vb += m->nv;
         tb += m->ntri;
     }
     r->total_verts = (uint32_t)vb;
     r->total_tris = (uint32_t)tb;
-    r->vs_blocks = (uint32_t)r->vs_block_mesh.size();
-    r->tri_blocks = (uint32_t)r->tri_block_mesh.size();
-    bool synced = false;
-    HIP_TRY(grow(r, r->d_vs_block_mesh, (r->vs_block_mesh.size() + 1) * 4, &synced));
-    HIP_TRY(grow(r, r->d_tri_block_mesh, (r->tri_block_mesh.size() + 1) * 4, &synced));
-    if (!synced) HIP_TRY(hipStreamSynchronize(r->stream));
-    if (!r->vs_block_mesh.empty())
-        HIP_TRY(hipMemcpy(r->d_vs_block_mesh.p, r->vs_block_mesh.data(), r->vs_block_mesh.size() * 4,
-                          hipMemcpyHostToDevice));
-    if (!r->tri_block_mesh.empty())
-        HIP_TRY(hipMemcpy(r->d_tri_block_mesh.p, r->tri_block_mesh.data(), r->tri_block_mesh.size() * 4,
-                          hipMemcpyHostToDevice));
     r->layout_version = scene_owner(r)->scene_version;
-    ++r->generation;
     r->have_prev = false;
-    return RAST_OK;
 }
 
 static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
@@ -541,9 +516,7 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->tile_pool, tb, &synced));
     HIP_TRY(grow(r, r->tile_cursor, tb, &synced));
     HIP_TRY(grow(r, r->density, tb, &synced));
-    HIP_TRY(grow(r, r->features, ((size_t)ntiles + 64) * kNumFeatures * 4, &synced));
     HIP_TRY(grow(r, r->counters, sizeof(FrameCounters) + 64, &synced));
-    HIP_TRY(grow(r, r->defaults, 64, &synced));
     if (r->queue_cap == 0) r->queue_cap = std::max<uint32_t>(1u << 20, std::min<uint64_t>(4ull * nt, 0x7FFFFFFFull));
     if (r->pool_cap == 0) r->pool_cap = 1u << 20;
     if (r->big_cap == 0) r->big_cap = 1u << 16;
@@ -552,110 +525,7 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->queue, (size_t)r->queue_cap * 4, &synced));
     HIP_TRY(grow(r, r->pool, (size_t)r->pool_cap * 4, &synced));
     HIP_TRY(grow(r, r->big, (size_t)r->big_cap * 4, &synced));
-    if (synced) {  // some buffer moved: captured graphs hold stale pointers
-        r->have_prev = false;
-        ++r->generation;
-    }
-    return RAST_OK;
-}
-
-// Writes the frame's mesh table (matrices for this camera) and map stand-ins
-// into pinned staging memory; the device copies are enqueued by launch_all.
-static int stage_frame(rast_renderer *r, const rast_frame_consts &fc, const rast_frame_opts &o)
-{
-    std::vector<rast_mesh *> &meshes = scene_meshes(r);
-    const size_t n = meshes.size();
-    if (n > r->h_meshes_cap) {
-        HIP_TRY(hipStreamSynchronize(r->stream));
-        if (r->h_meshes) (void)hipHostFree(r->h_meshes);
-        r->h_meshes = nullptr;
-        const size_t cap = std::max<size_t>(n, 16);
-        HIP_TRY(hipHostMalloc((void **)&r->h_meshes, cap * sizeof(DevMesh)));
-        r->h_meshes_cap = cap;
-        bool synced = true;
-        HIP_TRY(grow(r, r->d_meshes, cap * sizeof(DevMesh), &synced));
-        ++r->generation;
-    }
-    // 1x1 stand-ins for missing maps (deviation D8), slot 0 unused
-    r->h_defaults[0] = 0xFFFFFFFFu;
-    r->h_defaults[1] = o.default_normal;
-    r->h_defaults[2] = o.default_specular;
-    r->h_defaults[3] = o.default_gloss;
-    uint32_t vb = 0, vsb = 0, tsb = 0;
-    const uint32_t *dflt = (const uint32_t *)r->defaults.p;
-    for (size_t i = 0; i < n; ++i) {
-        const rast_mesh *m = meshes[i];
-        DevMesh &d = r->h_meshes[i];
-        d.ivert = m->vert16;
-        d.idx = m->idx;
-        d.num_verts = m->nv;
-        d.num_tris = m->ntri;
-        d.vert_base = vb;
-        d.tri_base = (uint32_t)m->tri_base;
-        d.vs_block0 = vsb;
-        d.tri_block0 = tsb;
-        d.topology = m->topology;
-        d.has_diffuse = m->tex[0] != nullptr;
-        mat4_mul(fc.viewproj, m->world, d.wvp);  // CCR/CUDARenderer.cu:241
-        memcpy(d.world, m->world, sizeof d.world);
-        for (int q = 0; q < 3; ++q) {
-            d.cam_pos[q] = fc.cam_pos[q];
-            d.cam_fwd[q] = fc.cam_fwd[q];
-        }
-        d.cam_pos[3] = d.cam_fwd[3] = 0.f;
-        for (int q = 0; q < 4; ++q) {
-            if (m->tex[q]) d.tex[q] = { m->tex[q]->texels, m->tex[q]->w, m->tex[q]->h };
-            else d.tex[q] = { dflt + q, 1u, 1u };
-        }
-        vb += m->nv;
-        vsb += ceil_div(m->nv, 256);
-        tsb += ceil_div(m->ntri, 256);
-    }
-    r->staged_meshes = n;
-    return RAST_OK;
-}
-
-static FrameBuffers frame_buffers(rast_renderer *r)
-{
-    FrameBuffers b{};
-    b.meshes = (const DevMesh *)r->d_meshes.p;
-    b.vs_block_mesh = (const uint32_t *)r->d_vs_block_mesh.p;
-    b.tri_block_mesh = (const uint32_t *)r->d_tri_block_mesh.p;
-    b.vs_blocks = r->vs_blocks;
-    b.tri_blocks = r->tri_blocks;
-    b.pos4 = (float4 *)r->pos4.p;
-    b.attr = (float4 *)r->attr.p;
-    b.trirec = (uint32_t *)r->trirec.p;
-    b.alive = (uint8_t *)r->alive.p;
-    b.tile_count = (uint32_t *)r->tile_count[r->cur].p;
-    b.tile_off = (uint32_t *)r->tile_off.p;
-    b.tile_cap = (uint32_t *)r->tile_cap.p;
-    b.tile_pool = (uint32_t *)r->tile_pool.p;
-    b.tile_cursor = (uint32_t *)r->tile_cursor.p;
-    b.queue = (uint32_t *)r->queue.p;
-    b.queue_cap = r->queue_cap;
-    b.pool = (uint32_t *)r->pool.p;
-    b.pool_cap = r->pool_cap;
-    b.ovf = (uint2 *)r->ovf.p;
-    b.ovf_cap = r->ovf_cap;
-    b.survivors = (uint32_t *)r->survivors.p;
-    b.big = (uint32_t *)r->big.p;
-    b.big_cap = r->big_cap;
-    b.counters = (FrameCounters *)r->counters.p;
-    b.density = nullptr;
-    b.tile_min_depth = (float *)r->tile_min_depth[r->cur].p;
-    b.keys = (unsigned long long *)r->keys.p;
-    b.vis = (uint32_t *)r->vis.p;
-    b.depth = (float *)r->depth.p;
-    b.rgba = (uint32_t *)r->rgba.p;
-    return b;
-}
-
-static int mark(rast_renderer *r, int which, bool timing)
-{
-    if (!timing) return RAST_OK;
-    HIP_TRY(hipEventRecord(r->ev[which], r->stream));
-    r->ev_used[which] = 1;
+    if (synced) r->have_prev = false;  // the previous frame's tile counts may be gone
     return RAST_OK;
 }
 
@@ -677,169 +547,401 @@ static int validate_camera(const rast_camera *cam)
     return RAST_OK;
 }
 
-// Host part of a frame: layout, storage, staging and the launch parameters.
-static int prepare_frame(rast_renderer *r)
+// Host part of a renderer's next frame (r->last_cam / last_opts): scene
+// layout, storage and the binning mode.
+static int prepare_job(rast_renderer *r)
 {
-    const rast_frame_opts &o = r->last_opts;
     HIP_TRY(hipSetDevice(r->device));
     if (scene_meshes(r).empty()) return rast_set_error(RAST_E_STATE, "rast_render: no meshes loaded");
-    if (r->layout_version != scene_owner(r)->scene_version) RAST_TRY(rebuild_layout(r));
-    const uint32_t tx = ceil_div(r->W, r->tile), ty = ceil_div(r->H, r->tile), ntiles = tx * ty;
-    RAST_TRY(ensure_frame_storage(r, ntiles));
-    rast_frame_consts fc;
-    camera_frame(r->last_cam, r->W, r->H, fc);
-    RAST_TRY(stage_frame(r, fc, o));
-    FrameParams &fp = r->fp;
-    fp = FrameParams{};
-    fp.width = r->W;
-    fp.height = r->H;
-    fp.cull_mode = o.cull_mode;
-    fp.sample_state = o.sample_state;
-    fp.visualisation = o.visualisation;
-    fp.clear_rgba = o.clear_rgba;
-    fp.tile = r->tile;
-    fp.tiles_x = tx;
-    fp.tiles_y = ty;
-    fp.big_threshold = r->big_threshold;
-    fp.micro_limit = r->micro_limit;
-    fp.num_tris = r->total_tris;
-    fp.num_verts = r->total_verts;
-    fp.num_meshes = (uint32_t)scene_meshes(r).size();
+    if (r->layout_version != scene_owner(r)->scene_version) refresh_layout(r);
+    RAST_TRY(ensure_frame_storage(r, ceil_div(r->W, r->tile) * ceil_div(r->H, r->tile)));
     r->frame_had_prev = r->have_prev;
-    bool single = o.bin_policy != RAST_BIN_EXACT && r->have_prev;
-    if (o.bin_policy == RAST_BIN_LEARNED && !r->model) single = false;
+    bool single = r->last_opts.bin_policy != RAST_BIN_EXACT && r->have_prev;
+    if (r->last_opts.bin_policy == RAST_BIN_LEARNED && !r->model) single = false;
     r->last_single = single;
-    r->fb = frame_buffers(r);
-    if (r->model) r->fb.density = (uint32_t *)r->density.p;  // feature of the capacity model
-    if (!single) {
-        r->fb.tile_pool = nullptr;
-        r->fb.tile_cap = nullptr;
-    }
     return RAST_OK;
 }
 
-// Everything a frame puts on `s`: staging uploads, kernels, counters readback.
-// Graph-capturable (no memset, no host waits, events only when timing).
-static int launch_all(rast_renderer *r, hipStream_t s, bool timing)
-{
-    const rast_frame_opts &o = r->last_opts;
-    const FrameParams &fp = r->fp;
-    const FrameBuffers &b = r->fb;
-    const bool single = r->last_single;
-    const uint32_t ntiles = fp.tiles_x * fp.tiles_y;
-    const uint32_t *prev_count = (const uint32_t *)r->tile_count[r->cur ^ 1].p;
-    const float *prev_depth = (const float *)r->tile_min_depth[r->cur ^ 1].p;
-    HIP_TRY(hipMemcpyAsync(r->defaults.p, r->h_defaults, 16, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(r->d_meshes.p, r->h_meshes, r->staged_meshes * sizeof(DevMesh), hipMemcpyHostToDevice, s));
-    RAST_TRY(mark(r, EV_START, timing));
-    HIP_TRY(launch_frame_init(fp, b, single, s));
-    HIP_TRY(launch_vertex(fp, b, s));
-    RAST_TRY(mark(r, EV_VERTEX, timing));
-    if (single) {
-        if (o.bin_policy == RAST_BIN_LEARNED) {
-            RAST_TRY(rast_model_capacities(r->model, fp, prev_count, prev_depth, b.density, (float *)r->features.p,
-                                           r->margin, r->slack, b.tile_cap, s));
-        } else {
-            HIP_TRY(launch_prevframe_capacity(ntiles, prev_count, r->margin, r->slack, b.tile_cap, s));
-        }
-        RAST_TRY(mark(r, EV_MODEL, timing));
-        HIP_TRY(launch_scan(ntiles, b.tile_cap, b.tile_off, nullptr, r->queue_cap, b.counters, true, s));
-        RAST_TRY(mark(r, EV_SCAN, timing));
-        HIP_TRY(launch_setup(fp, b, true, s));
-        RAST_TRY(mark(r, EV_SETUP, timing));
-        HIP_TRY(launch_fallback(fp, b, s));
-        RAST_TRY(mark(r, EV_FALLBACK, timing));
-    } else {
-        HIP_TRY(launch_setup(fp, b, false, s));
-        RAST_TRY(mark(r, EV_SETUP, timing));
-        HIP_TRY(launch_scan(ntiles, b.tile_count, b.tile_off, b.tile_cursor, r->queue_cap, b.counters, false, s));
-        RAST_TRY(mark(r, EV_SCAN, timing));
-        HIP_TRY(launch_fill(fp, b, s));
-        RAST_TRY(mark(r, EV_FILL, timing));
-    }
-    HIP_TRY(launch_raster(fp, b, r->small_limit, s));
-    RAST_TRY(mark(r, EV_RASTER, timing));
-    if (!(o.flags & RAST_FLAG_NO_SHADE)) {
-        HIP_TRY(launch_shade(fp, b, s));
-        RAST_TRY(mark(r, EV_SHADE, timing));
-    }
-    HIP_TRY(hipMemcpyAsync(r->h_counters, b.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, s));
-    return RAST_OK;
-}
+static bool job_learned(const rast_renderer *r) { return r->last_single && r->last_opts.bin_policy == RAST_BIN_LEARNED; }
+static bool job_shades(const rast_renderer *r) { return !(r->last_opts.flags & RAST_FLAG_NO_SHADE); }
 
-// Identity of a captured frame: everything launch_all bakes into the graph.
-static uint64_t frame_key(rast_renderer *r)
+// Structure of a launch: whatever fixes its grids and block tables.
+static uint64_t plan_key(rast_renderer *const *rs, uint32_t n, int hidden)
 {
-    const rast_frame_opts &o = r->last_opts;
     uint64_t h = 1469598103934665603ull;
     auto mix = [&h](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
-    mix((uint64_t)o.cull_mode | (uint64_t)o.sample_state << 4 | (uint64_t)o.visualisation << 8 |
-        (uint64_t)o.bin_policy << 12 | (uint64_t)(o.flags & RAST_FLAG_NO_SHADE) << 16 | (uint64_t)r->last_single << 20);
-    mix(o.clear_rgba);
-    mix((uint64_t)r->W << 32 | r->H);
-    mix((uint64_t)r->tile << 32 | r->big_threshold);
-    mix((uint64_t)r->micro_limit << 32 | r->small_limit);
-    mix(r->generation);
-    mix((uint64_t)r->cur);
-    mix(r->staged_meshes);
-    mix((uint64_t)(uintptr_t)r->model);
-    mix((uint64_t)(r->margin * 65536.f) << 32 | r->slack);
+    mix(n);
+    mix((uint64_t)hidden);
+    for (uint32_t j = 0; j < n; ++j) {
+        const rast_renderer *r = rs[j];
+        mix((uint64_t)r->W << 32 | r->H);
+        mix((uint64_t)r->tile << 8 | (uint64_t)r->last_single << 4 | (uint64_t)job_learned(r) << 2 | job_shades(r));
+        const std::vector<rast_mesh *> &ms = scene_meshes(const_cast<rast_renderer *>(r));
+        mix(ms.size());
+        for (const rast_mesh *m : ms) mix((uint64_t)m->nv << 32 | m->ntri);
+    }
     return h;
+}
+
+void plan_free(LaunchPlan &p)
+{
+    if (p.exec) (void)hipGraphExecDestroy(p.exec);
+    p.exec = nullptr;
+    if (p.done) (void)hipEventDestroy(p.done);
+    p.done = nullptr;
+    p.done_pending = false;
+    if (p.h_stage) (void)hipHostFree(p.h_stage);
+    p.h_stage = nullptr;
+    dfree(p.d_block_mesh);
+    dfree(p.d_tables);
+    dfree(p.d_stage);
+    p.built = false;
+    p.key = 0;
+}
+
+// The plan's previous launch must be done before its staging or tables change.
+static int plan_wait(LaunchPlan &P)
+{
+    if (P.done_pending) {
+        HIP_TRY(hipEventSynchronize(P.done));
+        P.done_pending = false;
+    }
+    return RAST_OK;
+}
+
+static int dev_alloc(DBuf &b, size_t bytes)
+{
+    if (b.p && b.n >= bytes) return RAST_OK;
+    dfree(b);
+    HIP_TRY(hipMalloc(&b.p, bytes < 256 ? 256 : bytes));
+    b.n = bytes < 256 ? 256 : bytes;
+    return RAST_OK;
+}
+
+// Grids, block -> job tables and block -> mesh tables of a launch structure.
+static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int hidden, uint64_t key)
+{
+    RAST_TRY(plan_wait(P));
+    if (P.exec) (void)hipGraphExecDestroy(P.exec);
+    P.exec = nullptr;
+    P.built = false;
+    if (!P.done) HIP_TRY(hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
+    if (n > 65535) return rast_set_error(RAST_E_INVALID_ARG, "at most 65535 jobs per launch");
+    Launch &L = P.L;
+    L = Launch{};
+    L.njobs = n;
+    L.model_hidden = hidden;
+    P.block0.assign((size_t)n * ST_COUNT, 0);
+    P.nblocks.assign((size_t)n * ST_COUNT, 0);
+    std::vector<uint16_t> tables[ST_COUNT];
+    std::vector<uint32_t> vs_mesh, tri_mesh;
+    P.vs_block0.clear();
+    P.tri_block0.clear();
+    uint64_t grid[ST_COUNT] = {};
+    uint32_t mesh_entry = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        rast_renderer *r = rs[j];
+        const uint32_t ntiles = ceil_div(r->W, r->tile) * ceil_div(r->H, r->tile);
+        const uint32_t ntris = std::max<uint32_t>(r->total_tris, 1);
+        const bool single = r->last_single;
+        uint32_t nb[ST_COUNT] = {};
+        nb[ST_INIT] = ceil_div(ntiles, 256);
+        nb[ST_CAP] = single ? ceil_div(ntiles, 64) : 0;
+        nb[ST_FILL] = single ? 0 : std::min<uint32_t>(ceil_div(ntris, 256), 1024);
+        nb[ST_FBALLOC] = single ? ceil_div(ntiles, 256) : 0;
+        nb[ST_FBFILL] = single ? std::min<uint32_t>(ceil_div(ntris, 256), 256) : 0;
+        const int res = r->tile == 8 ? ST_RES8 : (r->tile == 16 ? ST_RES16 : ST_RES32);
+        nb[res] = ceil_div(ntiles, 8) * 8;  // multiple of 8: the XCD order holds inside the job
+        nb[ST_SHADE] = job_shades(r) ? ceil_div(r->W, 64) * ceil_div(r->H, 4) : 0;
+        for (int st = 0; st < ST_COUNT; ++st) {
+            P.block0[(size_t)j * ST_COUNT + st] = (uint32_t)grid[st];
+            P.nblocks[(size_t)j * ST_COUNT + st] = nb[st];
+            grid[st] += nb[st];
+            if (n > 1) tables[st].insert(tables[st].end(), nb[st], (uint16_t)j);
+        }
+        if (single) ++L.cap_jobs;
+        else ++L.exact_jobs;
+        for (rast_mesh *m : scene_meshes(r)) {
+            P.vs_block0.push_back((uint32_t)vs_mesh.size());
+            P.tri_block0.push_back((uint32_t)tri_mesh.size());
+            vs_mesh.insert(vs_mesh.end(), ceil_div(m->nv, 256), mesh_entry);
+            tri_mesh.insert(tri_mesh.end(), ceil_div(m->ntri, 256), mesh_entry);
+            ++mesh_entry;
+        }
+    }
+    // a grid's threads must stay below 2^32 (256 threads per block in every stage but the scan)
+    for (int st = 0; st < ST_COUNT; ++st)
+        if (grid[st] >= (1ull << 24)) return rast_set_error(RAST_E_INVALID_ARG, "launch too large (stage %d)", st);
+    if (vs_mesh.size() >= (1ull << 24) || tri_mesh.size() >= (1ull << 24))
+        return rast_set_error(RAST_E_INVALID_ARG, "launch too large (meshes)");
+    for (int st = 0; st < ST_COUNT; ++st) L.grid[st] = (uint32_t)grid[st];
+    P.njobs = n;
+    P.nmeshes = mesh_entry;
+    L.vs_blocks = (uint32_t)vs_mesh.size();
+    L.tri_blocks = (uint32_t)tri_mesh.size();
+    // device tables: [vs block -> mesh][tri block -> mesh], [stage tables]
+    RAST_TRY(dev_alloc(P.d_block_mesh, (vs_mesh.size() + tri_mesh.size() + 2) * 4));
+    uint32_t *bm = (uint32_t *)P.d_block_mesh.p;
+    if (!vs_mesh.empty()) HIP_TRY(hipMemcpy(bm, vs_mesh.data(), vs_mesh.size() * 4, hipMemcpyHostToDevice));
+    if (!tri_mesh.empty())
+        HIP_TRY(hipMemcpy(bm + vs_mesh.size(), tri_mesh.data(), tri_mesh.size() * 4, hipMemcpyHostToDevice));
+    L.vs_block_mesh = bm;
+    L.tri_block_mesh = bm + vs_mesh.size();
+    if (n > 1) {
+        size_t total = 0;
+        for (int st = 0; st < ST_COUNT; ++st) total += tables[st].size();
+        RAST_TRY(dev_alloc(P.d_tables, (total + 8) * 2));
+        uint16_t *tb = (uint16_t *)P.d_tables.p;
+        size_t off = 0;
+        for (int st = 0; st < ST_COUNT; ++st) {
+            if (!tables[st].empty())
+                HIP_TRY(hipMemcpy(tb + off, tables[st].data(), tables[st].size() * 2, hipMemcpyHostToDevice));
+            L.table[st] = tables[st].empty() ? nullptr : tb + off;
+            off += tables[st].size();
+        }
+    }
+    // staging block: JobDesc[n] then DevMesh[meshes]
+    const size_t bytes = (size_t)n * sizeof(JobDesc) + (size_t)P.nmeshes * sizeof(DevMesh);
+    if (bytes > P.stage_bytes || !P.h_stage) {
+        if (P.h_stage) (void)hipHostFree(P.h_stage);
+        P.h_stage = nullptr;
+        HIP_TRY(hipHostMalloc(&P.h_stage, bytes));
+    }
+    RAST_TRY(dev_alloc(P.d_stage, bytes));
+    P.stage_bytes = bytes;
+    L.jobs = (const JobDesc *)P.d_stage.p;
+    L.meshes = (const DevMesh *)((char *)P.d_stage.p + (size_t)n * sizeof(JobDesc));
+    P.key = key;
+    P.built = true;
+    return RAST_OK;
+}
+
+// Writes every job's descriptor and mesh entries (this frame's cameras,
+// options and buffer parities) into the plan's pinned staging block.
+static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
+{
+    RAST_TRY(plan_wait(P));
+    JobDesc *jd = (JobDesc *)P.h_stage;
+    DevMesh *dm = (DevMesh *)((char *)P.h_stage + (size_t)n * sizeof(JobDesc));
+    uint32_t me = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        rast_renderer *r = rs[j];
+        const rast_frame_opts &o = r->last_opts;
+        const std::vector<rast_mesh *> &meshes = scene_meshes(r);
+        JobDesc &J = jd[j];
+        memset(&J, 0, sizeof J);
+        FrameParams &fp = J.fp;
+        fp.width = r->W;
+        fp.height = r->H;
+        fp.cull_mode = o.cull_mode;
+        fp.sample_state = o.sample_state;
+        fp.visualisation = o.visualisation;
+        fp.clear_rgba = o.clear_rgba;
+        fp.tile = r->tile;
+        fp.tiles_x = ceil_div(r->W, r->tile);
+        fp.tiles_y = ceil_div(r->H, r->tile);
+        fp.big_threshold = r->big_threshold;
+        fp.micro_limit = r->micro_limit;
+        fp.num_tris = r->total_tris;
+        fp.num_verts = r->total_verts;
+        fp.num_meshes = (uint32_t)meshes.size();
+        memcpy(J.block0, &P.block0[(size_t)j * ST_COUNT], sizeof J.block0);
+        memcpy(J.nblocks, &P.nblocks[(size_t)j * ST_COUNT], sizeof J.nblocks);
+        J.mesh0 = me;
+        J.single = r->last_single;
+        J.learned = job_learned(r);
+        J.small_limit = r->small_limit;
+        J.margin = r->margin;
+        J.slack = r->slack;
+        J.queue_cap = r->queue_cap;
+        J.pool_cap = r->pool_cap;
+        J.big_cap = r->big_cap;
+        J.ovf_cap = r->ovf_cap;
+        // 1x1 stand-ins for missing maps (deviation D8); word 0 unused
+        J.defaults[0] = 0xFFFFFFFFu;
+        J.defaults[1] = o.default_normal;
+        J.defaults[2] = o.default_specular;
+        J.defaults[3] = o.default_gloss;
+        J.model = J.learned ? rast_model_params(r->model) : nullptr;
+        J.pos4 = (float4 *)r->pos4.p;
+        J.attr = (float4 *)r->attr.p;
+        J.trirec = (uint32_t *)r->trirec.p;
+        J.alive = (uint8_t *)r->alive.p;
+        J.tile_count = (uint32_t *)r->tile_count[r->cur].p;
+        J.tile_off = (uint32_t *)r->tile_off.p;
+        J.tile_cap = (uint32_t *)r->tile_cap.p;
+        J.tile_pool = (uint32_t *)r->tile_pool.p;
+        J.tile_cursor = (uint32_t *)r->tile_cursor.p;
+        J.prev_count = (const uint32_t *)r->tile_count[r->cur ^ 1].p;
+        J.prev_depth = (const float *)r->tile_min_depth[r->cur ^ 1].p;
+        J.queue = (uint32_t *)r->queue.p;
+        J.pool = (uint32_t *)r->pool.p;
+        J.ovf = (uint2 *)r->ovf.p;
+        J.survivors = (uint32_t *)r->survivors.p;
+        J.big = (uint32_t *)r->big.p;
+        J.counters = (FrameCounters *)r->counters.p;
+        J.host_counters = r->d_counters_host;
+        J.density = r->model ? (uint32_t *)r->density.p : nullptr;  // feature of the capacity model
+        J.tile_min_depth = (float *)r->tile_min_depth[r->cur].p;
+        J.keys = (unsigned long long *)r->keys.p;
+        J.vis = (uint32_t *)r->vis.p;
+        J.depth = (float *)r->depth.p;
+        J.rgba = (uint32_t *)r->rgba.p;
+        const uint32_t *dflt = (const uint32_t *)((const char *)P.d_stage.p + (size_t)j * sizeof(JobDesc) +
+                                                  offsetof(JobDesc, defaults));
+        rast_frame_consts fc;
+        camera_frame(r->last_cam, r->W, r->H, fc);
+        uint32_t vb = 0;
+        for (size_t i = 0; i < meshes.size(); ++i, ++me) {
+            const rast_mesh *m = meshes[i];
+            DevMesh &d = dm[me];
+            memset(&d, 0, sizeof d);
+            d.ivert = m->vert16;
+            d.idx = m->idx;
+            d.num_verts = m->nv;
+            d.num_tris = m->ntri;
+            d.vert_base = vb;
+            d.tri_base = (uint32_t)m->tri_base;
+            d.vs_block0 = P.vs_block0[me];
+            d.tri_block0 = P.tri_block0[me];
+            d.topology = m->topology;
+            d.has_diffuse = m->tex[0] != nullptr;
+            d.job = j;
+            mat4_mul(fc.viewproj, m->world, d.wvp);  // CCR/CUDARenderer.cu:241
+            memcpy(d.world, m->world, sizeof d.world);
+            for (int q = 0; q < 3; ++q) {
+                d.cam_pos[q] = fc.cam_pos[q];
+                d.cam_fwd[q] = fc.cam_fwd[q];
+            }
+            for (int q = 0; q < 4; ++q) {
+                if (m->tex[q]) d.tex[q] = { m->tex[q]->texels, m->tex[q]->w, m->tex[q]->h };
+                else d.tex[q] = { dflt + q, 1u, 1u };
+            }
+            vb += m->nv;
+        }
+    }
+    return RAST_OK;
+}
+
+// Timing events of one frame, recorded between the stages when RAST_FLAG_TIMING is set.
+static int mark(rast_renderer *r, int which, hipStream_t s)
+{
+    if (!r) return RAST_OK;
+    HIP_TRY(hipEventRecord(r->ev[which], s));
+    r->ev_used[which] = 1;
+    return RAST_OK;
+}
+
+// Everything a launch puts on a stream: the staging upload, one kernel per
+// stage, the counters publish.  Graph-capturable (no memset, no host waits;
+// events only when `timed` is set).
+static int enqueue_launch(LaunchPlan &P, hipStream_t s, rast_renderer *timed)
+{
+    const Launch &L = P.L;
+    HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage, P.stage_bytes, hipMemcpyHostToDevice, s));
+    RAST_TRY(mark(timed, EV_START, s));
+    HIP_TRY(launch_frame_init(L, s));
+    HIP_TRY(launch_vertex(L, s));
+    RAST_TRY(mark(timed, EV_VERTEX, s));
+    HIP_TRY(launch_capacity(L, s));
+    RAST_TRY(mark(timed, EV_MODEL, s));
+    HIP_TRY(launch_scan(L, true, s));
+    RAST_TRY(mark(timed, EV_SCANCAP, s));
+    HIP_TRY(launch_setup(L, s));
+    RAST_TRY(mark(timed, EV_SETUP, s));
+    HIP_TRY(launch_scan(L, false, s));
+    RAST_TRY(mark(timed, EV_SCANCNT, s));
+    HIP_TRY(launch_fill(L, s));
+    RAST_TRY(mark(timed, EV_FILL, s));
+    HIP_TRY(launch_fallback(L, s));
+    RAST_TRY(mark(timed, EV_FALLBACK, s));
+    HIP_TRY(launch_resolve(L, s));
+    RAST_TRY(mark(timed, EV_RASTER, s));
+    HIP_TRY(launch_shade(L, s));
+    RAST_TRY(mark(timed, EV_SHADE, s));
+    HIP_TRY(launch_publish(L, s));
+    return RAST_OK;
+}
+
+static int capture_launch(LaunchPlan &P, hipStream_t s)
+{
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_launch(P, s, nullptr);
+    hipGraph_t cap = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &cap);
+    if (rc != RAST_OK) {
+        if (cap) (void)hipGraphDestroy(cap);
+        return rc;
+    }
+    HIP_TRY(e);
+    const hipError_t ie = hipGraphInstantiate(&P.exec, cap, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(cap);
+    if (ie != hipSuccess) P.exec = nullptr;
+    HIP_TRY(ie);
+    return RAST_OK;
+}
+
+// Runs a built and staged plan on s (replaying its graph in graph mode).
+static int run_plan(LaunchPlan &P, hipStream_t s, bool graph, rast_renderer *timed)
+{
+    if (graph) {
+        if (!P.exec) RAST_TRY(capture_launch(P, s));
+        HIP_TRY(hipGraphLaunch(P.exec, s));
+    } else {
+        RAST_TRY(enqueue_launch(P, s, timed));
+    }
+    HIP_TRY(hipEventRecord(P.done, s));
+    P.done_pending = true;
+    return RAST_OK;
 }
 
 static void finish_enqueue(rast_renderer *r)
 {
     r->stats.num_triangles = r->total_tris;
-    r->stats.num_tiles = r->fp.tiles_x * r->fp.tiles_y;
+    r->stats.num_tiles = ceil_div(r->W, r->tile) * ceil_div(r->H, r->tile);
     r->frame_pending = true;
     r->frame_valid = false;
     r->have_prev = true;
     r->cur ^= 1;
 }
 
-static int clear_keys_if_dirty(rast_renderer *r, bool wait)
+static int clear_keys_if_dirty(rast_renderer *r)
 {
     if (!r->keys_dirty) return RAST_OK;
     HIP_TRY(launch_clear_keys((unsigned long long *)r->keys.p, (size_t)r->W * r->H, r->stream));
-    if (wait) HIP_TRY(hipStreamSynchronize(r->stream));
     r->keys_dirty = false;
+    return RAST_OK;
+}
+
+static int job_hidden(rast_renderer *const *rs, uint32_t n, int *out)
+{
+    int h = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        if (!job_learned(rs[j])) continue;
+        const int hj = rast_model_hidden(rs[j]->model);
+        if (h && hj != h) return rast_set_error(RAST_E_INVALID_ARG, "learned jobs of one launch must share the model width");
+        h = hj;
+    }
+    *out = h;
     return RAST_OK;
 }
 
 static int enqueue_frame(rast_renderer *r)
 {
     const rast_frame_opts &o = r->last_opts;
-    RAST_TRY(prepare_frame(r));
+    RAST_TRY(prepare_job(r));
     const bool timing = (o.flags & RAST_FLAG_TIMING) != 0;
     const bool graph = (o.flags & RAST_FLAG_GRAPH) != 0 && !timing;
     r->timing = timing;
     memset(r->ev_used, 0, sizeof r->ev_used);
-    RAST_TRY(clear_keys_if_dirty(r, false));
+    int hidden = 0;
+    RAST_TRY(job_hidden(&r, 1, &hidden));
+    const uint64_t key = plan_key(&r, 1, hidden);
+    if (!r->plan.built || r->plan.key != key) RAST_TRY(build_plan(r->plan, &r, 1, hidden, key));
+    RAST_TRY(stage_launch(r->plan, &r, 1));
+    RAST_TRY(clear_keys_if_dirty(r));
     r->keys_dirty = true;  // until the frame (whose resolve pass resets the keys) is enqueued
-    if (graph) {
-        const uint64_t key = frame_key(r);
-        FrameGraph &g = r->graphs[r->cur];
-        if (!g.exec || g.key != key) {
-            if (g.exec) (void)hipGraphExecDestroy(g.exec);
-            g.exec = nullptr;
-            HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-            const int rc = launch_all(r, r->stream, false);
-            hipGraph_t cap = nullptr;
-            const hipError_t e = hipStreamEndCapture(r->stream, &cap);
-            if (rc != RAST_OK) {
-                if (cap) (void)hipGraphDestroy(cap);
-                return rc;
-            }
-            HIP_TRY(e);
-            const hipError_t ie = hipGraphInstantiate(&g.exec, cap, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(cap);
-            HIP_TRY(ie);
-            g.key = key;
-        }
-        HIP_TRY(hipGraphLaunch(g.exec, r->stream));
-    } else {
-        RAST_TRY(launch_all(r, r->stream, timing));
-    }
+    RAST_TRY(run_plan(r->plan, r->stream, graph, timing ? r : nullptr));
     r->keys_dirty = false;
     finish_enqueue(r);
     return RAST_OK;
@@ -924,21 +1026,15 @@ extern "C" int rast_get_stats(rast_renderer *r, rast_stats *out)
     s.capacity_total = r->last_single ? c.capacity_total : 0;
     s.rerenders = r->rerenders;
     s.ms_vertex = elapsed(r, EV_START, EV_VERTEX);
-    if (r->last_single) {
-        s.ms_model = elapsed(r, EV_VERTEX, EV_MODEL);
-        s.ms_scan = elapsed(r, EV_MODEL, EV_SCAN);
-        s.ms_setup = elapsed(r, EV_SCAN, EV_SETUP);
-        s.ms_fallback = elapsed(r, EV_SETUP, EV_FALLBACK);
-        s.ms_raster = elapsed(r, EV_FALLBACK, EV_RASTER);
-    } else {
-        s.ms_setup = elapsed(r, EV_VERTEX, EV_SETUP);
-        s.ms_scan = elapsed(r, EV_SETUP, EV_SCAN);
-        s.ms_fill = elapsed(r, EV_SCAN, EV_FILL);
-        s.ms_raster = elapsed(r, EV_FILL, EV_RASTER);
-    }
+    s.ms_model = elapsed(r, EV_VERTEX, EV_MODEL);
+    s.ms_scan = elapsed(r, EV_MODEL, EV_SCANCAP) + elapsed(r, EV_SETUP, EV_SCANCNT);
+    s.ms_setup = elapsed(r, EV_SCANCAP, EV_SETUP);
     s.ms_count = s.ms_setup;
+    s.ms_fill = elapsed(r, EV_SCANCNT, EV_FILL);
+    s.ms_fallback = elapsed(r, EV_FILL, EV_FALLBACK);
+    s.ms_raster = elapsed(r, EV_FALLBACK, EV_RASTER);
     s.ms_shade = elapsed(r, EV_RASTER, EV_SHADE);
-    s.ms_total = r->ev_used[EV_SHADE] ? elapsed(r, EV_START, EV_SHADE) : elapsed(r, EV_START, EV_RASTER);
+    s.ms_total = elapsed(r, EV_START, EV_SHADE);
     *out = s;
     return RAST_OK;
 }
@@ -991,137 +1087,93 @@ extern "C" int rast_renderer_set_binmodel(rast_renderer *r, rast_binmodel *m, fl
     r->model = m;
     r->margin = margin;
     r->slack = slack;
-    ++r->generation;  // captured frames hold the previous model's parameter pointer
     return RAST_OK;
 }
 
 // ----------------------------------------------------------------- batching
 
-// Cached batch graphs, each keyed by its jobs' renderers and frame keys.
-// Renderers alternate between two count buffers, so a repeated batch needs
-// two entries; a few more cover alternating job lists.  Renderer destruction
-// drops the entries that use it (a graph holds its buffers).
-struct BatchGraph {
-    std::vector<std::pair<rast_renderer *, uint64_t>> key;
-    hipGraphExec_t exec = nullptr;
-    uint64_t last_use = 0;
-};
+// Launch plans of recent batch shapes (a few, least recently used replaced),
+// and the stream the batches run on.
 struct BatchCache {
-    BatchGraph entries[4];
+    LaunchPlan plans[4];
+    uint64_t last_use[4] = {};
     uint64_t clock = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t fork = nullptr, done = nullptr;
+    hipEvent_t done = nullptr;
     int device = -1;
 };
 static BatchCache g_batch;
 static std::mutex g_batch_mutex;
 
-static void drop_batch_graph(BatchGraph &g)
-{
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    g.exec = nullptr;
-    g.key.clear();
-}
-
-void rast_forget_renderer(rast_renderer *r)
-{
-    std::lock_guard<std::mutex> lock(g_batch_mutex);
-    for (BatchGraph &g : g_batch.entries)
-        for (auto &kv : g.key)
-            if (kv.first == r) {
-                if (g_batch.stream) (void)hipStreamSynchronize(g_batch.stream);
-                drop_batch_graph(g);
-                break;
-            }
-}
-
-static int capture_batch(const rast_job *jobs, uint32_t n, hipGraphExec_t *exec)
-{
-    HIP_TRY(hipStreamBeginCapture(g_batch.stream, hipStreamCaptureModeThreadLocal));
-    int rc = RAST_OK;
-    hipError_t e = hipEventRecord(g_batch.fork, g_batch.stream);
-    for (uint32_t i = 0; i < n && rc == RAST_OK && e == hipSuccess; ++i) {
-        rast_renderer *t = jobs[i].target;  // one branch per job, forked from and joined to the batch stream
-        e = hipStreamWaitEvent(t->stream, g_batch.fork, 0);
-        if (e == hipSuccess) rc = launch_all(t, t->stream, false);
-        if (rc == RAST_OK && e == hipSuccess) e = hipEventRecord(t->join, t->stream);
-        if (rc == RAST_OK && e == hipSuccess) e = hipStreamWaitEvent(g_batch.stream, t->join, 0);
-    }
-    hipGraph_t cap = nullptr;
-    const hipError_t ee = hipStreamEndCapture(g_batch.stream, &cap);
-    if (rc != RAST_OK || e != hipSuccess || ee != hipSuccess) {
-        if (cap) (void)hipGraphDestroy(cap);
-        if (rc != RAST_OK) return rc;
-        HIP_TRY(e != hipSuccess ? e : ee);
-    }
-    const hipError_t ie = hipGraphInstantiate(exec, cap, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(cap);
-    HIP_TRY(ie);
-    return RAST_OK;
-}
-
 extern "C" int rast_render_batch(const rast_job *jobs, uint32_t n)
 {
     if (!jobs || n == 0) return rast_set_error(RAST_E_INVALID_ARG, "rast_render_batch: no jobs");
+    if (n > 65535) return rast_set_error(RAST_E_INVALID_ARG, "rast_render_batch: at most 65535 jobs");
     const int device = jobs[0].target ? jobs[0].target->device : 0;
     for (uint32_t i = 0; i < n; ++i) {
         rast_renderer *t = jobs[i].target;
         if (!t) return rast_set_error(RAST_E_INVALID_ARG, "job %u has no target renderer", i);
         if (t->device != device) return rast_set_error(RAST_E_INVALID_ARG, "job %u is on another device", i);
-        for (uint32_t j = 0; j < i; ++j)
-            if (jobs[j].target == t) return rast_set_error(RAST_E_INVALID_ARG, "jobs %u and %u share a renderer", j, i);
         RAST_TRY(validate_opts(&jobs[i].opts));
         RAST_TRY(validate_camera(&jobs[i].camera));
+    }
+    std::vector<rast_renderer *> rs(n);
+    for (uint32_t i = 0; i < n; ++i) rs[i] = jobs[i].target;
+    {
+        std::vector<rast_renderer *> sorted(rs);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return rast_set_error(RAST_E_INVALID_ARG, "two jobs of a batch share a renderer");
     }
     std::lock_guard<std::mutex> lock(g_batch_mutex);
     HIP_TRY(hipSetDevice(device));
     if (g_batch.device != device) {
         if (g_batch.stream) {
             (void)hipStreamSynchronize(g_batch.stream);
-            for (BatchGraph &g : g_batch.entries) drop_batch_graph(g);
+            for (LaunchPlan &p : g_batch.plans) plan_free(p);
             (void)hipStreamDestroy(g_batch.stream);
-            (void)hipEventDestroy(g_batch.fork);
             (void)hipEventDestroy(g_batch.done);
             g_batch.stream = nullptr;
         }
         HIP_TRY(hipStreamCreateWithFlags(&g_batch.stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&g_batch.fork, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&g_batch.done, hipEventDisableTiming));
         g_batch.device = device;
     }
     // host part of every job; each renderer's previous frame is checked first
-    std::vector<std::pair<rast_renderer *, uint64_t>> key(n);
     for (uint32_t i = 0; i < n; ++i) {
-        rast_renderer *t = jobs[i].target;
+        rast_renderer *t = rs[i];
         if (t->frame_pending) RAST_TRY(check_frame(t));
         t->last_cam = jobs[i].camera;
         t->last_opts = jobs[i].opts;
         t->last_opts.flags &= ~RAST_FLAG_TIMING;
         t->prev_valid_before = t->have_prev;
-        RAST_TRY(prepare_frame(t));
-        RAST_TRY(clear_keys_if_dirty(t, false));
+        RAST_TRY(prepare_job(t));
+        memset(t->ev_used, 0, sizeof t->ev_used);
+    }
+    int hidden = 0;
+    RAST_TRY(job_hidden(rs.data(), n, &hidden));
+    const uint64_t key = plan_key(rs.data(), n, hidden);
+    int pick = -1, victim = 0;
+    for (int k = 0; k < 4; ++k) {
+        if (g_batch.plans[k].built && g_batch.plans[k].key == key) pick = k;
+        if (g_batch.last_use[k] < g_batch.last_use[victim]) victim = k;
+    }
+    if (pick < 0) {
+        pick = victim;
+        RAST_TRY(build_plan(g_batch.plans[pick], rs.data(), n, hidden, key));
+    }
+    LaunchPlan &P = g_batch.plans[pick];
+    g_batch.last_use[pick] = ++g_batch.clock;
+    RAST_TRY(stage_launch(P, rs.data(), n));
+    for (rast_renderer *t : rs) {
+        RAST_TRY(clear_keys_if_dirty(t));
         HIP_TRY(hipEventRecord(t->join, t->stream));  // the batch follows earlier work on t
         HIP_TRY(hipStreamWaitEvent(g_batch.stream, t->join, 0));
-        key[i] = { t, frame_key(t) };
     }
-    BatchGraph *hit = nullptr, *victim = &g_batch.entries[0];
-    for (BatchGraph &g : g_batch.entries) {
-        if (g.exec && g.key == key) hit = &g;
-        if (!g.exec || (victim->exec && g.last_use < victim->last_use)) victim = &g;
-    }
-    if (!hit) {
-        hit = victim;
-        drop_batch_graph(*hit);
-        RAST_TRY(capture_batch(jobs, n, &hit->exec));
-        hit->key = key;
-    }
-    hit->last_use = ++g_batch.clock;
-    HIP_TRY(hipGraphLaunch(hit->exec, g_batch.stream));
+    RAST_TRY(run_plan(P, g_batch.stream, true, nullptr));
     HIP_TRY(hipEventRecord(g_batch.done, g_batch.stream));
-    for (uint32_t i = 0; i < n; ++i) {
-        rast_renderer *t = jobs[i].target;
+    for (rast_renderer *t : rs) {
         HIP_TRY(hipStreamWaitEvent(t->stream, g_batch.done, 0));  // later work on t follows the batch
-        memset(t->ev_used, 0, sizeof t->ev_used);
         finish_enqueue(t);
     }
     return RAST_OK;

@@ -18,6 +18,8 @@
 //   vis, depth  u32 / f32 per pixel, rgba u32 per pixel
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace rast {
@@ -45,6 +47,8 @@ struct DevMesh {
     uint32_t tri_block0;      // first block of this mesh in the triangle launch
     int32_t topology;
     int32_t has_diffuse;
+    uint32_t job;             // index of the frame (job) this mesh entry belongs to
+    uint32_t pad_;
     float wvp[16];            // data[col][row]
     float world[16];
     float cam_pos[4];         // camera position (xyz) and forward (xyz) of the frame
@@ -69,6 +73,21 @@ struct FrameParams {
     uint32_t num_meshes;
 };
 
+// Launch stages whose grids are shared by all jobs of a launch: block b of
+// stage s serves job table_s[b], as its block b - JobDesc::block0[s].
+enum Stage : int {
+    ST_INIT,      // per-tile reset (256 tiles per block)
+    ST_CAP,       // single-pass capacities (64 tiles per block)
+    ST_FILL,      // exact second pass (grid-stride over binned triangles)
+    ST_FBALLOC,   // single-pass fallback allocation (256 tiles per block)
+    ST_FBFILL,    // single-pass fallback placement (grid-stride over entries)
+    ST_RES8,      // resolve, 8x8 tiles (one block per tile, XCD-ordered)
+    ST_RES16,     // resolve, 16x16 tiles
+    ST_RES32,     // resolve, 32x32 tiles
+    ST_SHADE,     // 64x4-pixel blocks
+    ST_COUNT
+};
+
 // Device-side frame counters, read back once per frame.
 struct FrameCounters {
     uint32_t num_alive;       // survivors of culling
@@ -80,6 +99,49 @@ struct FrameCounters {
     uint32_t pool_used;       // fallback pool entries
     uint32_t storage_overflow;// queue / pool / big list storage exhausted -> frame invalid
     uint32_t capacity_total;
+};
+
+// One frame of a launch: its parameters and the device buffers of the
+// renderer that owns it.  Uploaded with the mesh table for every launch, so
+// a captured launch graph replays with new cameras, options and buffer
+// parities; only the grid shapes are fixed by the graph.
+struct JobDesc {
+    FrameParams fp;
+    uint32_t block0[ST_COUNT];   // first block of this job in each stage's grid
+    uint32_t nblocks[ST_COUNT];  // its block count there (0: not part of that stage)
+    uint32_t mesh0;              // first entry of the job in the mesh table
+    uint32_t single;             // single-pass binning (capacities known before setup)
+    uint32_t learned;            // capacities from the MLP (else previous count * margin + slack)
+    uint32_t small_limit;        // tile-clipped pixel count from which resolve goes pixel-parallel
+    float margin;
+    uint32_t slack;
+    uint32_t queue_cap, pool_cap, big_cap, ovf_cap;
+    uint32_t defaults[4];        // 1x1 stand-in maps: [1] normal, [2] specular, [3] gloss
+    const float *model;          // MLP parameters (learned capacities)
+    float4 *pos4;
+    float4 *attr;
+    uint32_t *trirec;
+    uint8_t *alive;
+    uint32_t *tile_count;        // exact: counts; single pass: cursors (end = exact counts)
+    uint32_t *tile_off;          // num_tiles + 1
+    uint32_t *tile_cap;          // single pass capacities
+    uint32_t *tile_pool;         // per tile: fallback pool offset, or ~0u if the tile did not overflow
+    uint32_t *tile_cursor;       // exact fill cursors / fallback cursors
+    const uint32_t *prev_count;  // previous frame's exact per-tile counts
+    const float *prev_depth;     // previous frame's per-tile min depth
+    uint32_t *queue;
+    uint32_t *pool;
+    uint2 *ovf;                  // single pass: (tile, id) entries beyond capacity
+    uint32_t *survivors;
+    uint32_t *big;
+    FrameCounters *counters;
+    FrameCounters *host_counters;  // pinned host copy, written at the end of the launch
+    uint32_t *density;           // per tile projected-vertex count (capacity model), may be null
+    float *tile_min_depth;       // per tile min depth written by the resolve pass
+    unsigned long long *keys;    // per pixel (z << 32 | ~(id + 1)), reset to depth 1 / none after each resolve
+    uint32_t *vis;
+    float *depth;
+    uint32_t *rgba;
 };
 
 }  // namespace rast

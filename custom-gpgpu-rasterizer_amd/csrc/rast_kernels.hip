@@ -11,7 +11,7 @@
 // sequential "reject if z > depth, later triangle wins ties" rule of
 // CCR/ERenderer.cpp:544-554 applied in ascending triangle order.
 //
-// Pipeline (one launch per stage, all meshes at once):
+// Pipeline (one launch per stage for all jobs and meshes of a launch):
 //   k_vertex   per vertex: clip/NDC position + shading attributes
 //   k_setup    per triangle: assembly, cull, screen setup record; triangles
 //              whose bounding box has at most micro_limit pixels are
@@ -248,33 +248,39 @@ __device__ __forceinline__ void load_tri(const uint32_t *__restrict__ rec, uint3
 
 // ---------------------------------------------------------------- kernels
 
-__global__ void k_frame_init(uint32_t *tile_count, uint32_t *tile_aux, uint32_t n, FrameCounters *ctr,
-                             uint32_t *density, float *tile_min_depth)
+__global__ __launch_bounds__(256) void k_frame_init(const JobDesc *__restrict__ jobs,
+                                                    const uint16_t *__restrict__ table)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, ST_INIT, local)];
+    const uint32_t n = J.fp.tiles_x * J.fp.tiles_y;
+    const uint32_t i = local * 256u + threadIdx.x;
     if (i < n) {
-        tile_count[i] = 0;
-        if (tile_min_depth) tile_min_depth[i] = 1.0f;
-        if (tile_aux) tile_aux[i] = 0xFFFFFFFFu;
-        if (density) density[i] = 0;
+        J.tile_count[i] = 0;
+        J.tile_min_depth[i] = 1.0f;
+        if (J.single) J.tile_pool[i] = 0xFFFFFFFFu;
+        if (J.density) J.density[i] = 0;
     }
-    if (i < sizeof(FrameCounters) / 4) reinterpret_cast<uint32_t *>(ctr)[i] = 0;
+    if (i < sizeof(FrameCounters) / 4) reinterpret_cast<uint32_t *>(J.counters)[i] = 0;
 }
 
-__global__ void k_clear_keys(unsigned long long *keys, size_t n)
+__global__ __launch_bounds__(256) void k_clear_keys(unsigned long long *__restrict__ keys, size_t n)
 {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        keys[i] = kClearKey;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u) keys[i] = kClearKey;
 }
 
 // VS_Kernel / GetNDCVertex (CCR/CUDARenderer.cu:673-700, CCR/CUDAROPs.cu:99-118).
 __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ meshes,
-                                                const uint32_t *__restrict__ block_mesh, FrameParams fp,
-                                                float4 *__restrict__ pos4, float4 *__restrict__ attr,
-                                                uint32_t *__restrict__ density)
+                                                const uint32_t *__restrict__ block_mesh,
+                                                const JobDesc *__restrict__ jobs)
 {
     const uint32_t mi = block_mesh[blockIdx.x];
     const DevMesh &M = meshes[mi];
+    const JobDesc &J = jobs[M.job];
+    const FrameParams &fp = J.fp;
+    float4 *__restrict__ pos4 = J.pos4;
+    float4 *__restrict__ attr = J.attr;
+    uint32_t *__restrict__ density = J.density;
     const uint32_t v = (blockIdx.x - M.vs_block0) * 256u + threadIdx.x;
     if (v >= M.num_verts) return;
     const float4 *iv = reinterpret_cast<const float4 *>(M.ivert + (size_t)v * 16);
@@ -325,24 +331,23 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
 // path's rules (CCR/ERenderer.cpp:73-157, 603-624) with D4/D5, plus the
 // micro-triangle raster (RasterizeTriangle, CCR/CUDAROPs.cu:394-407, over the
 // whole bbox with the order-free key).
-template <bool SINGLE_PASS>
 __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshes,
-                                               const uint32_t *__restrict__ block_mesh, FrameParams fp,
-                                               const float4 *__restrict__ pos4, uint32_t *__restrict__ trirec,
-                                               uint8_t *__restrict__ alive, uint32_t *__restrict__ tile_count,
-                                               const uint32_t *__restrict__ tile_off,
-                                               const uint32_t *__restrict__ tile_cap, uint32_t *__restrict__ queue,
-                                               uint32_t queue_cap, uint32_t *__restrict__ survivors,
-                                               uint32_t *__restrict__ big, uint32_t big_cap,
-                                               uint2 *__restrict__ ovf, uint32_t ovf_cap,
-                                               unsigned long long *__restrict__ keys,
-                                               FrameCounters *__restrict__ ctr)
+                                               const uint32_t *__restrict__ block_mesh,
+                                               const JobDesc *__restrict__ jobs)
 {
     __shared__ uint32_t lds_append[12];
     __shared__ TriLds mstage[256];
     __shared__ uint32_t moff[256];
     const uint32_t mi = block_mesh[blockIdx.x];
     const DevMesh &M = meshes[mi];
+    const JobDesc &J = jobs[M.job];
+    const FrameParams &fp = J.fp;
+    const float4 *__restrict__ pos4 = J.pos4;
+    uint32_t *__restrict__ trirec = J.trirec;
+    uint32_t *__restrict__ tile_count = J.tile_count;
+    unsigned long long *__restrict__ keys = J.keys;
+    FrameCounters *__restrict__ ctr = J.counters;
+    const bool single_pass = J.single != 0;
     const uint32_t k = (blockIdx.x - M.tri_block0) * 256u + threadIdx.x;
     const bool valid = k < M.num_tris;
     const uint32_t gid = M.tri_base + k;
@@ -417,7 +422,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                 r[3] = make_uint4(i0, i1, i2, mi);
             }
         }
-        alive[gid] = ok ? 1 : 0;
+        J.alive[gid] = ok ? 1 : 0;
     }
 
     // classification by the pixel range [xmin,xmax) x [ymin,ymax)
@@ -478,24 +483,24 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                   sslot);
     (void)n_ok;
     if (is_big) {
-        if (bslot < big_cap) big[bslot] = gid;
+        if (bslot < J.big_cap) J.big[bslot] = gid;
         else ctr->storage_overflow = 1;
     }
     if (binned) {
-        survivors[sslot] = gid;
+        J.survivors[sslot] = gid;
         const uint32_t w = tx1 - tx0 + 1;
         uint32_t tx = tx0, ty = ty0;
         for (uint32_t s = 0; s < span; ++s) {
             const uint32_t tl = ty * fp.tiles_x + tx;
-            if (SINGLE_PASS) {
+            if (single_pass) {
                 const uint32_t slot = claim_slot(tile_count, tl);
-                if (slot < tile_cap[tl]) {
-                    const uint32_t pos = tile_off[tl] + slot;
-                    if (pos < queue_cap) queue[pos] = gid;
+                if (slot < J.tile_cap[tl]) {
+                    const uint32_t pos = J.tile_off[tl] + slot;
+                    if (pos < J.queue_cap) J.queue[pos] = gid;
                 } else {
                     // capacity exceeded: keep the entry for the exact fallback
                     const uint32_t o = atomicAdd(&ctr->ovf_entries, 1u);
-                    if (o < ovf_cap) ovf[o] = make_uint2(tl, gid);
+                    if (o < J.ovf_cap) J.ovf[o] = make_uint2(tl, gid);
                     else ctr->storage_overflow = 1;
                 }
             } else {
@@ -506,13 +511,19 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     }
 }
 
-// Exclusive scan of n per-tile values by one 1024-thread workgroup:
+// Exclusive scan of a job's per-tile values by one 1024-thread workgroup:
 // out_off[n] = total.  is_capacity: in = capacities, the total is the
 // capacity sum; otherwise in = exact counts and the total is the pair count.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t n, const uint32_t *__restrict__ in,
-                                               uint32_t *__restrict__ out_off, uint32_t *__restrict__ zero_cursor,
-                                               uint32_t limit, FrameCounters *__restrict__ ctr, int is_capacity)
+__global__ __launch_bounds__(1024) void k_scan(const JobDesc *__restrict__ jobs, int is_capacity)
 {
+    const JobDesc &J = jobs[blockIdx.x];  // one workgroup per job
+    if ((J.single != 0) != (is_capacity != 0)) return;
+    const uint32_t n = J.fp.tiles_x * J.fp.tiles_y;
+    const uint32_t *__restrict__ in = is_capacity ? J.tile_cap : J.tile_count;
+    uint32_t *__restrict__ out_off = J.tile_off;
+    uint32_t *__restrict__ zero_cursor = is_capacity ? nullptr : J.tile_cursor;
+    const uint32_t limit = J.queue_cap;
+    FrameCounters *__restrict__ ctr = J.counters;
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -563,14 +574,20 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t n, const uint32_t *__res
 
 // Exact second pass over the binned triangles (the order inside a queue is
 // irrelevant: depth is resolved order-independently).
-__global__ __launch_bounds__(256) void k_fill(FrameParams fp, const uint32_t *__restrict__ survivors,
-                                              const uint32_t *__restrict__ trirec,
-                                              uint32_t *__restrict__ cursor, const uint32_t *__restrict__ tile_off,
-                                              uint32_t *__restrict__ queue, uint32_t queue_cap,
-                                              const FrameCounters *__restrict__ ctr)
+__global__ __launch_bounds__(256) void k_fill(const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table)
 {
-    const uint32_t n = ctr->num_binned;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, ST_FILL, local)];
+    const FrameParams &fp = J.fp;
+    const uint32_t *__restrict__ survivors = J.survivors;
+    const uint32_t *__restrict__ trirec = J.trirec;
+    uint32_t *__restrict__ cursor = J.tile_cursor;
+    const uint32_t *__restrict__ tile_off = J.tile_off;
+    uint32_t *__restrict__ queue = J.queue;
+    const uint32_t queue_cap = J.queue_cap;
+    const uint32_t n = J.counters->num_binned;
+    const uint32_t stride = J.nblocks[ST_FILL] * 256u;
+    for (uint32_t i = local * 256u + threadIdx.x; i < n; i += stride) {
         const uint32_t gid = survivors[i];
         const uint2 bb = *reinterpret_cast<const uint2 *>(trirec + (size_t)gid * kTriRecWords + 10);
         const uint32_t xmin = bb.x & 0xFFFFu, ymin = bb.x >> 16, xmax = bb.y & 0xFFFFu, ymax = bb.y >> 16;
@@ -587,13 +604,19 @@ __global__ __launch_bounds__(256) void k_fill(FrameParams fp, const uint32_t *__
 
 // Single-pass fallback, step 1 (per tile): the exact count of a tile is its
 // final fill cursor; the entries beyond its capacity get a pool region.
-__global__ __launch_bounds__(256) void k_fallback_alloc(uint32_t n, const uint32_t *__restrict__ count,
-                                                        const uint32_t *__restrict__ cap,
-                                                        uint32_t *__restrict__ tile_pool,
-                                                        uint32_t *__restrict__ fb_cursor, uint32_t pool_cap,
-                                                        FrameCounters *__restrict__ ctr)
+__global__ __launch_bounds__(256) void k_fallback_alloc(const JobDesc *__restrict__ jobs,
+                                                        const uint16_t *__restrict__ table)
 {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, ST_FBALLOC, local)];
+    const uint32_t n = J.fp.tiles_x * J.fp.tiles_y;
+    const uint32_t *__restrict__ count = J.tile_count;
+    const uint32_t *__restrict__ cap = J.tile_cap;
+    uint32_t *__restrict__ tile_pool = J.tile_pool;
+    uint32_t *__restrict__ fb_cursor = J.tile_cursor;
+    const uint32_t pool_cap = J.pool_cap;
+    FrameCounters *__restrict__ ctr = J.counters;
+    const uint32_t t = local * 256u + threadIdx.x;
     uint32_t c = 0, excess = 0;
     if (t < n) {
         c = count[t];
@@ -621,14 +644,20 @@ __global__ __launch_bounds__(256) void k_fallback_alloc(uint32_t n, const uint32
 }
 
 // Step 2 (per overflow entry): exact placement of the entries that did not fit.
-__global__ __launch_bounds__(256) void k_fallback_fill(const uint2 *__restrict__ ovf, uint32_t ovf_cap,
-                                                       const uint32_t *__restrict__ tile_pool,
-                                                       uint32_t *__restrict__ fb_cursor, uint32_t *__restrict__ pool,
-                                                       uint32_t pool_cap, const FrameCounters *__restrict__ ctr)
+__global__ __launch_bounds__(256) void k_fallback_fill(const JobDesc *__restrict__ jobs,
+                                                       const uint16_t *__restrict__ table)
 {
-    uint32_t n = ctr->ovf_entries;
-    n = n < ovf_cap ? n : ovf_cap;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, ST_FBFILL, local)];
+    const uint2 *__restrict__ ovf = J.ovf;
+    const uint32_t *__restrict__ tile_pool = J.tile_pool;
+    uint32_t *__restrict__ fb_cursor = J.tile_cursor;
+    uint32_t *__restrict__ pool = J.pool;
+    const uint32_t pool_cap = J.pool_cap;
+    uint32_t n = J.counters->ovf_entries;
+    n = n < J.ovf_cap ? n : J.ovf_cap;
+    const uint32_t stride = J.nblocks[ST_FBFILL] * 256u;
+    for (uint32_t i = local * 256u + threadIdx.x; i < n; i += stride) {
         const uint2 e = ovf[i];
         const uint32_t po = tile_pool[e.x];
         if (po == 0xFFFFFFFFu) continue;
@@ -662,20 +691,30 @@ __device__ __forceinline__ float cull_depth(const TriLds &t)
 // list (prefix sum over their clipped pixel ranges, LDS atomic min).  The few
 // screen-list triangles are then tested by every lane on its own pixels, with
 // a conservative per-pixel depth reject before the exact test.
-template <int T, int NT>
-__global__ __launch_bounds__(NT) void k_resolve(FrameParams fp, const uint32_t *__restrict__ tile_count,
-                                                const uint32_t *__restrict__ tile_off,
-                                                const uint32_t *__restrict__ tile_cap,
-                                                const uint32_t *__restrict__ tile_pool,
-                                                const uint32_t *__restrict__ queue, uint32_t queue_cap,
-                                                const uint32_t *__restrict__ pool, uint32_t pool_cap,
-                                                const uint32_t *__restrict__ big, uint32_t big_cap,
-                                                const FrameCounters *__restrict__ ctr,
-                                                const uint32_t *__restrict__ trirec,
-                                                unsigned long long *__restrict__ keys, uint32_t *__restrict__ vis,
-                                                float *__restrict__ depth, float *__restrict__ tile_min_depth,
-                                                uint32_t small_limit, uint32_t grid_tiles)
+template <int T, int NT, int STAGE>
+__global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table)
 {
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, STAGE, local)];
+    const FrameParams &fp = J.fp;
+    const uint32_t *__restrict__ tile_count = J.tile_count;
+    const uint32_t *__restrict__ tile_off = J.tile_off;
+    const uint32_t *__restrict__ tile_cap = J.single ? J.tile_cap : nullptr;
+    const uint32_t *__restrict__ tile_pool = J.tile_pool;
+    const uint32_t *__restrict__ queue = J.queue;
+    const uint32_t queue_cap = J.queue_cap;
+    const uint32_t *__restrict__ pool = J.pool;
+    const uint32_t pool_cap = J.pool_cap;
+    const uint32_t *__restrict__ big = J.big;
+    const uint32_t big_cap = J.big_cap;
+    const FrameCounters *__restrict__ ctr = J.counters;
+    const uint32_t *__restrict__ trirec = J.trirec;
+    unsigned long long *__restrict__ keys = J.keys;
+    uint32_t *__restrict__ vis = J.vis;
+    float *__restrict__ depth = J.depth;
+    float *__restrict__ tile_min_depth = J.tile_min_depth;
+    const uint32_t small_limit = J.small_limit;
+    const uint32_t grid_tiles = J.nblocks[STAGE];
     constexpr int NPIX = T * T;
     constexpr int PPT = NPIX / NT;
     constexpr int NW = NT / 64;
@@ -690,7 +729,7 @@ __global__ __launch_bounds__(NT) void k_resolve(FrameParams fp, const uint32_t *
     // XCD-aware order: consecutive workgroups go round-robin over the 8 XCDs,
     // so give each XCD a contiguous band of tiles (neighbours share triangles).
     const uint32_t per_xcd = grid_tiles / 8u;
-    const uint32_t tile = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+    const uint32_t tile = (local % 8u) * per_xcd + local / 8u;
     if (tile >= ntiles) return;
     const uint32_t txi = tile % fp.tiles_x, tyi = tile / fp.tiles_x;
     const int X0 = (int)(txi * T), Y0 = (int)(tyi * T);
@@ -961,16 +1000,23 @@ __device__ __forceinline__ void sample(const DevTexture &t, float u, float v, in
 // in the reference (so the same texels are fetched); the lighting terms use
 // the hardware reciprocal / rsqrt / exp2 / log2, which keeps RGBA8 within
 // 1/255 of the reference formula.  64x4-pixel blocks.
-__global__ __launch_bounds__(256) void k_shade(FrameParams fp, const DevMesh *__restrict__ meshes,
-                                               const uint32_t *__restrict__ vis, const float *__restrict__ depth,
-                                               const uint32_t *__restrict__ trirec,
-                                               const float4 *__restrict__ attr, uint32_t *__restrict__ rgba)
+__global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
+                                               const uint16_t *__restrict__ table)
 {
     __shared__ float unit[256];
     unit[threadIdx.x] = (float)threadIdx.x / 255.f;
     __syncthreads();
-    const uint32_t x = blockIdx.x * 64u + (threadIdx.x & 63u);
-    const uint32_t y = blockIdx.y * 4u + (threadIdx.x >> 6);
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, ST_SHADE, local)];
+    const FrameParams &fp = J.fp;
+    const uint32_t *__restrict__ vis = J.vis;
+    const float *__restrict__ depth = J.depth;
+    const uint32_t *__restrict__ trirec = J.trirec;
+    const float4 *__restrict__ attr = J.attr;
+    uint32_t *__restrict__ rgba = J.rgba;
+    const uint32_t bw = (fp.width + 63u) / 64u;
+    const uint32_t x = (local % bw) * 64u + (threadIdx.x & 63u);
+    const uint32_t y = (local / bw) * 4u + (threadIdx.x >> 6);
     if (x >= fp.width || y >= fp.height) return;
     const size_t pid = (size_t)y * fp.width + x;
     const uint32_t id = vis[pid];
@@ -1008,7 +1054,8 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams fp, const DevMesh *__
         rgba[pid] = pack_rgba(nx, ny, nz);
         return;
     }
-    const DevMesh &M = meshes[q3.w < fp.num_meshes ? q3.w : 0];
+    const uint32_t mi = q3.w - J.mesh0 < fp.num_meshes ? q3.w : J.mesh0;  // entry in the launch's mesh table
+    const DevMesh &M = meshes[mi];
     if (!M.has_diffuse) {  // vertex colour (perspective-weighted, not x w)
         const float4 a03 = A0[3], a13 = A1[3], a23 = A2[3];
         rgba[pid] = pack_rgba(RAST_INTERP(a03.x, a13.x, a23.x), RAST_INTERP(a03.y, a13.y, a23.y),
@@ -1061,96 +1108,106 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams fp, const DevMesh *__
     rgba[pid] = pack_rgba(outr, outg, outb);
 }
 
+// Copies every job's counters to its renderer's pinned host block (one word
+// per thread), replacing a device-to-host copy per job.
+__global__ __launch_bounds__(256) void k_publish(const JobDesc *__restrict__ jobs, uint32_t njobs)
+{
+    constexpr uint32_t W = sizeof(FrameCounters) / 4;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= njobs * W) return;
+    const JobDesc &J = jobs[i / W];
+    reinterpret_cast<volatile uint32_t *>(J.host_counters)[i % W] =
+        reinterpret_cast<const uint32_t *>(J.counters)[i % W];
+}
+
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 }  // namespace
 
-hipError_t launch_frame_init(const FrameParams &fp, const FrameBuffers &b, bool single_pass, hipStream_t s)
+hipError_t launch_frame_init(const Launch &L, hipStream_t s)
 {
-    const uint32_t n = fp.tiles_x * fp.tiles_y;
-    const uint32_t nn = n > 64 ? n : 64;
-    hipLaunchKernelGGL(k_frame_init, dim3(ceil_div(nn, 256)), dim3(256), 0, s, b.tile_count,
-                       single_pass ? b.tile_pool : nullptr, n, b.counters, b.density, b.tile_min_depth);
+    hipLaunchKernelGGL(k_frame_init, dim3(L.grid[ST_INIT]), dim3(256), 0, s, L.jobs, L.table[ST_INIT]);
     return hipGetLastError();
 }
 
 hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_clear_keys, dim3(2048), dim3(256), 0, s, keys, n);
+    const size_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_clear_keys, dim3(blocks < 2048 ? (uint32_t)blocks : 2048u), dim3(256), 0, s, keys, n);
     return hipGetLastError();
 }
 
-hipError_t launch_vertex(const FrameParams &fp, const FrameBuffers &b, hipStream_t s)
+hipError_t launch_vertex(const Launch &L, hipStream_t s)
 {
-    if (b.vs_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_vertex, dim3(b.vs_blocks), dim3(256), 0, s, b.meshes, b.vs_block_mesh, fp, b.pos4, b.attr,
-                       b.density);
+    if (L.vs_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vertex, dim3(L.vs_blocks), dim3(256), 0, s, L.meshes, L.vs_block_mesh, L.jobs);
     return hipGetLastError();
 }
 
-hipError_t launch_setup(const FrameParams &fp, const FrameBuffers &b, bool single_pass, hipStream_t s)
+hipError_t launch_setup(const Launch &L, hipStream_t s)
 {
-    if (b.tri_blocks == 0) return hipSuccess;
-    if (single_pass)
-        hipLaunchKernelGGL(k_setup<true>, dim3(b.tri_blocks), dim3(256), 0, s, b.meshes, b.tri_block_mesh, fp, b.pos4,
-                           b.trirec, b.alive, b.tile_count, b.tile_off, b.tile_cap, b.queue, b.queue_cap,
-                           b.survivors, b.big, b.big_cap, b.ovf, b.ovf_cap, b.keys, b.counters);
-    else
-        hipLaunchKernelGGL(k_setup<false>, dim3(b.tri_blocks), dim3(256), 0, s, b.meshes, b.tri_block_mesh, fp,
-                           b.pos4, b.trirec, b.alive, b.tile_count, b.tile_off, b.tile_cap, b.queue, b.queue_cap,
-                           b.survivors, b.big, b.big_cap, b.ovf, b.ovf_cap, b.keys, b.counters);
+    if (L.tri_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_setup, dim3(L.tri_blocks), dim3(256), 0, s, L.meshes, L.tri_block_mesh, L.jobs);
     return hipGetLastError();
 }
 
-hipError_t launch_scan(uint32_t n, const uint32_t *in, uint32_t *out_off, uint32_t *zero_cursor, uint32_t limit,
-                       FrameCounters *counters, bool is_capacity, hipStream_t s)
+hipError_t launch_scan(const Launch &L, bool capacities, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n, in, out_off, zero_cursor, limit, counters,
-                       is_capacity ? 1 : 0);
+    if ((capacities ? L.cap_jobs : L.exact_jobs) == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scan, dim3(L.njobs), dim3(1024), 0, s, L.jobs, capacities ? 1 : 0);
     return hipGetLastError();
 }
 
-hipError_t launch_fill(const FrameParams &fp, const FrameBuffers &b, hipStream_t s)
+hipError_t launch_fill(const Launch &L, hipStream_t s)
 {
-    const uint32_t blocks = ceil_div(fp.num_tris > 0 ? fp.num_tris : 1, 256);
-    hipLaunchKernelGGL(k_fill, dim3(blocks < 1024 ? blocks : 1024), dim3(256), 0, s, fp, b.survivors, b.trirec,
-                       b.tile_cursor, b.tile_off, b.queue, b.queue_cap, b.counters);
+    if (!L.grid[ST_FILL]) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(L.grid[ST_FILL]), dim3(256), 0, s, L.jobs, L.table[ST_FILL]);
     return hipGetLastError();
 }
 
-hipError_t launch_fallback(const FrameParams &fp, const FrameBuffers &b, hipStream_t s)
+hipError_t launch_fallback(const Launch &L, hipStream_t s)
 {
-    const uint32_t n = fp.tiles_x * fp.tiles_y;
-    hipLaunchKernelGGL(k_fallback_alloc, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, b.tile_count, b.tile_cap,
-                       b.tile_pool, b.tile_cursor, b.pool_cap, b.counters);
+    if (!L.grid[ST_FBALLOC]) return hipSuccess;
+    hipLaunchKernelGGL(k_fallback_alloc, dim3(L.grid[ST_FBALLOC]), dim3(256), 0, s, L.jobs, L.table[ST_FBALLOC]);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fallback_fill, dim3(256), dim3(256), 0, s, b.ovf, b.ovf_cap, b.tile_pool, b.tile_cursor,
-                       b.pool, b.pool_cap, b.counters);
+    hipLaunchKernelGGL(k_fallback_fill, dim3(L.grid[ST_FBFILL]), dim3(256), 0, s, L.jobs, L.table[ST_FBFILL]);
     return hipGetLastError();
 }
 
-hipError_t launch_raster(const FrameParams &fp, const FrameBuffers &b, uint32_t small_limit, hipStream_t s)
+hipError_t launch_resolve(const Launch &L, hipStream_t s)
 {
-    const uint32_t n = fp.tiles_x * fp.tiles_y;
-    const uint32_t grid = ceil_div(n, 8) * 8;
-#define RAST_RESOLVE_ARGS                                                                                        \
-    fp, b.tile_count, b.tile_off, b.tile_cap, b.tile_pool, b.queue, b.queue_cap, b.pool, b.pool_cap, b.big,     \
-        b.big_cap, b.counters, b.trirec, b.keys, b.vis, b.depth, b.tile_min_depth, small_limit, grid
-    switch (fp.tile) {
-    case 8: hipLaunchKernelGGL((k_resolve<8, 64>), dim3(grid), dim3(64), 0, s, RAST_RESOLVE_ARGS); break;
-    case 16: hipLaunchKernelGGL((k_resolve<16, 256>), dim3(grid), dim3(256), 0, s, RAST_RESOLVE_ARGS); break;
-    case 32: hipLaunchKernelGGL((k_resolve<32, 256>), dim3(grid), dim3(256), 0, s, RAST_RESOLVE_ARGS); break;
-    default: return hipErrorInvalidValue;
+    if (L.grid[ST_RES8]) {
+        hipLaunchKernelGGL((k_resolve<8, 64, ST_RES8>), dim3(L.grid[ST_RES8]), dim3(64), 0, s, L.jobs, L.table[ST_RES8]);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-#undef RAST_RESOLVE_ARGS
+    if (L.grid[ST_RES16]) {
+        hipLaunchKernelGGL((k_resolve<16, 256, ST_RES16>), dim3(L.grid[ST_RES16]), dim3(256), 0, s, L.jobs,
+                           L.table[ST_RES16]);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (L.grid[ST_RES32]) {
+        hipLaunchKernelGGL((k_resolve<32, 256, ST_RES32>), dim3(L.grid[ST_RES32]), dim3(256), 0, s, L.jobs,
+                           L.table[ST_RES32]);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_shade(const Launch &L, hipStream_t s)
+{
+    if (!L.grid[ST_SHADE]) return hipSuccess;
+    hipLaunchKernelGGL(k_shade, dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs, L.table[ST_SHADE]);
     return hipGetLastError();
 }
 
-hipError_t launch_shade(const FrameParams &fp, const FrameBuffers &b, hipStream_t s)
+hipError_t launch_publish(const Launch &L, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shade, dim3(ceil_div(fp.width, 64), ceil_div(fp.height, 4)), dim3(256), 0, s, fp, b.meshes, b.vis,
-                       b.depth, b.trirec, b.attr, b.rgba);
+    const uint32_t n = L.njobs * (uint32_t)(sizeof(FrameCounters) / 4);
+    hipLaunchKernelGGL(k_publish, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.jobs, L.njobs);
     return hipGetLastError();
 }
 

@@ -78,25 +78,38 @@ __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t
     for (int q = 0; q < F; ++q) out[(size_t)t * F + q] = f[q];
 }
 
-// Frame-time inference: a block serves 64 tiles (one per lane); each of its 4
-// waves computes a quarter of the hidden units, weights come through scalar
-// loads (uniform across the wave), layer-1 activations are shared in LDS.
+// Frame-time capacities of the single-pass jobs of a launch.  Learned: a
+// block serves 64 tiles (one per lane); each of its 4 waves computes a
+// quarter of the hidden units, weights come through scalar loads (uniform
+// across the wave), layer-1 activations are shared in LDS.  Otherwise the
+// previous frame's count: ceil(prev * margin) + slack.
 template <int H>
-__global__ __launch_bounds__(256) void k_mlp_capacity(FrameParams fp, const uint32_t *__restrict__ prev,
-                                                      const float *__restrict__ prev_depth,
-                                                      const uint32_t *__restrict__ density,
-                                                      const float *__restrict__ p, float margin, uint32_t slack,
-                                                      uint32_t *__restrict__ cap)
+__global__ __launch_bounds__(256) void k_capacity(const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table)
 {
     constexpr int Q = H / 4;
     __shared__ float h1s[H][65];
     __shared__ float part[4][64];
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, ST_CAP, local)];
+    const FrameParams &fp = J.fp;
+    const uint32_t *__restrict__ prev = J.prev_count;
+    const float margin = J.margin;
+    const uint32_t slack = J.slack;
+    uint32_t *__restrict__ cap = J.tile_cap;
     const uint32_t n = fp.tiles_x * fp.tiles_y;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t t = blockIdx.x * 64u + lane;
+    const uint32_t t = local * 64u + lane;
+    if (!J.learned) {
+        if (w == 0 && t < n) {
+            const float c = ceilf((float)prev[t] * margin);
+            cap[t] = (c < 1.0e8f ? (uint32_t)c : 100000000u) + slack;
+        }
+        return;
+    }
+    const float *__restrict__ p = J.model;
     const float *W1 = p, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
     float x[F];
-    if (t < n) tile_features(fp, t, prev, prev_depth, density, x);
+    if (t < n) tile_features(fp, t, prev, J.prev_depth, J.density, x);
     else for (int f = 0; f < F; ++f) x[f] = 0.f;
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
@@ -132,16 +145,6 @@ __global__ __launch_bounds__(256) void k_targets(uint32_t n, const uint32_t *__r
 {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t < n) y[t] = log1pf((float)count[t]);
-}
-
-__global__ __launch_bounds__(256) void k_prevframe_capacity(uint32_t n, const uint32_t *__restrict__ prev,
-                                                            float margin, uint32_t slack, uint32_t *__restrict__ cap)
-{
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t < n) {
-        const float c = ceilf((float)prev[t] * margin);
-        cap[t] = (c < 1.0e8f ? (uint32_t)c : 100000000u) + slack;
-    }
 }
 
 // ----------------------------------------------------------------- forward
@@ -414,13 +417,6 @@ int loss_grad(rast_binmodel *m, uint32_t B, uint32_t N, const float *X, const fl
 
 using namespace rast_detail;
 
-hipError_t launch_prevframe_capacity(uint32_t n, const uint32_t *prev, float margin, uint32_t slack, uint32_t *cap,
-                                     hipStream_t s)
-{
-    hipLaunchKernelGGL(k_prevframe_capacity, dim3(cdiv(n, 256)), dim3(256), 0, s, n, prev, margin, slack, cap);
-    return hipGetLastError();
-}
-
 namespace rast {
 hipError_t launch_tile_features(const FrameParams &fp, const uint32_t *prev_count, const float *prev_depth,
                                 const uint32_t *density, float *features, hipStream_t s)
@@ -431,23 +427,19 @@ hipError_t launch_tile_features(const FrameParams &fp, const uint32_t *prev_coun
 }
 }  // namespace rast
 
-int rast_model_capacities(rast_binmodel *m, const FrameParams &fp, const uint32_t *prev_count, const float *prev_depth,
-                          const uint32_t *density, float *features, float margin, uint32_t slack, uint32_t *cap,
-                          hipStream_t s)
+namespace rast {
+hipError_t launch_capacity(const Launch &L, hipStream_t s)
 {
-    (void)features;
-    if (!m) return rast_set_error(RAST_E_STATE, "learned binning without a model");
-    const uint32_t n = fp.tiles_x * fp.tiles_y;
-    const dim3 g(cdiv(n, 64)), b(256);
-    const float *p = (const float *)m->params.p;
-    switch (m->H) {
-    case 16: hipLaunchKernelGGL(k_mlp_capacity<16>, g, b, 0, s, fp, prev_count, prev_depth, density, p, margin, slack, cap); break;
-    case 32: hipLaunchKernelGGL(k_mlp_capacity<32>, g, b, 0, s, fp, prev_count, prev_depth, density, p, margin, slack, cap); break;
-    default: hipLaunchKernelGGL(k_mlp_capacity<64>, g, b, 0, s, fp, prev_count, prev_depth, density, p, margin, slack, cap); break;
+    if (!L.grid[ST_CAP]) return hipSuccess;
+    const dim3 g(L.grid[ST_CAP]), b(256);
+    switch (L.model_hidden) {
+    case 32: hipLaunchKernelGGL(k_capacity<32>, g, b, 0, s, L.jobs, L.table[ST_CAP]); break;
+    case 64: hipLaunchKernelGGL(k_capacity<64>, g, b, 0, s, L.jobs, L.table[ST_CAP]); break;
+    default: hipLaunchKernelGGL(k_capacity<16>, g, b, 0, s, L.jobs, L.table[ST_CAP]); break;
     }
-    HIP_TRY(hipGetLastError());
-    return RAST_OK;
+    return hipGetLastError();
 }
+}  // namespace rast
 
 void rast_model_detach_renderer(rast_binmodel *m, rast_renderer *r)
 {
@@ -665,3 +657,6 @@ extern "C" int rast_binmodel_loss_grad(rast_binmodel *m, const float *x, const f
     if (loss) HIP_TRY(hipMemcpy(loss, m->loss.p, 4, hipMemcpyDeviceToHost));
     return RAST_OK;
 }
+
+int rast_model_hidden(const rast_binmodel *m) { return m ? m->H : 0; }
+const float *rast_model_params(const rast_binmodel *m) { return m ? (const float *)m->params.p : nullptr; }

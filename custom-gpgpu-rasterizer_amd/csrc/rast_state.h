@@ -28,11 +28,6 @@ int rast_set_error(int code, const char *fmt, ...);
         if (rc_ != RAST_OK) return rc_; \
     } while (0)
 
-using rast::FrameBuffers;
-
-// Drops cached batch graphs that refer to r (before r is destroyed).
-void rast_forget_renderer(rast_renderer *r);
-
 struct DBuf {
     void *p = nullptr;
     size_t n = 0;
@@ -44,6 +39,27 @@ inline void dfree(DBuf &b)
     b.p = nullptr;
     b.n = 0;
 }
+
+// Device tables, staging and (optionally) the captured graph of one launch
+// shape: a renderer's single frames, or a cached batch shape.  Built for a
+// structure key (frame sizes, tile sizes, binning modes, mesh sizes); the
+// per-launch contents (cameras, options, buffer pointers) are restaged into
+// the pinned block and uploaded by the launch itself.
+struct LaunchPlan {
+    uint64_t key = 0;
+    bool built = false;
+    rast::Launch L{};
+    uint32_t njobs = 0, nmeshes = 0;
+    std::vector<uint32_t> block0, nblocks;  // per job x stage (host copies for JobDesc)
+    std::vector<uint32_t> vs_block0, tri_block0;  // per mesh entry
+    DBuf d_block_mesh, d_tables, d_stage;
+    void *h_stage = nullptr;  // pinned: JobDesc[njobs] then DevMesh[nmeshes]
+    size_t stage_bytes = 0;
+    hipGraphExec_t exec = nullptr;  // captured launch (graph mode)
+    hipEvent_t done = nullptr;      // end of the plan's last launch (staging reusable)
+    bool done_pending = false;
+};
+void plan_free(LaunchPlan &p);
 
 struct rast_texture {
     rast_renderer *r = nullptr;
@@ -62,11 +78,6 @@ struct rast_mesh {
     uint64_t tri_base = 0;
 };
 
-struct FrameGraph {
-    hipGraphExec_t exec = nullptr;
-    uint64_t key = 0;
-};
-
 struct rast_renderer {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -78,30 +89,22 @@ struct rast_renderer {
     std::vector<rast_renderer *> views;    // views drawing this renderer's meshes
     uint64_t scene_version = 1;            // bumped when the mesh list changes
     uint64_t layout_version = 0;           // scene version the block tables were built for
-    uint64_t generation = 0;               // bumped whenever a device buffer moves
-    FrameGraph graphs[2];                  // captured frame per count-buffer parity
-    hipEvent_t join = nullptr;             // batch capture: end of this renderer's branch
-    rast::FrameParams fp{};
-    FrameBuffers fb{};
-    size_t staged_meshes = 0;
+    hipEvent_t join = nullptr;             // orders a batch after this renderer's earlier work
+    LaunchPlan plan;                       // single-frame launches
     // scene layout
-    uint32_t total_verts = 0, total_tris = 0, vs_blocks = 0, tri_blocks = 0;
-    std::vector<uint32_t> vs_block_mesh, tri_block_mesh;
-    DBuf d_vs_block_mesh, d_tri_block_mesh, d_meshes;
-    rast::DevMesh *h_meshes = nullptr;  // pinned staging
-    size_t h_meshes_cap = 0;
+    uint32_t total_verts = 0, total_tris = 0;
     // frame buffers
     DBuf pos4, attr, trirec, alive, survivors;
-    DBuf tile_count[2], tile_off, tile_cap, tile_pool, tile_cursor, density, tile_min_depth[2], features;
-    DBuf queue, pool, big, ovf, counters, defaults;
+    DBuf tile_count[2], tile_off, tile_cap, tile_pool, tile_cursor, density, tile_min_depth[2];
+    DBuf queue, pool, big, ovf, counters;
     uint32_t queue_cap = 0, pool_cap = 0, big_cap = 0, ovf_cap = 0;
     DBuf vis, depth, rgba, keys;
     bool keys_dirty = true;  // key buffer must be reset before the next frame
     uint32_t micro_limit = 16;
     int cur = 0;  // index of the current frame's tile_count / tile_min_depth
     bool have_prev = false;
-    rast::FrameCounters *h_counters = nullptr;  // pinned
-    uint32_t *h_defaults = nullptr;       // pinned
+    rast::FrameCounters *h_counters = nullptr;  // pinned, written by the launch
+    rast::FrameCounters *d_counters_host = nullptr;  // device address of h_counters
     // timing
     hipEvent_t ev[12] = {};
     bool timing = false;
