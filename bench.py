@@ -3,16 +3,21 @@
 
 Workload: a 1024x1024-quad fBm terrain (2,097,152 triangles) seen at a 10
 degree grazing angle plus 8 screen-covering backdrop triangles, 3840x2160,
-four 2048^2 RGBA8 maps, fp32 depth, tiles of 16x16.  A step is one frame over
-the next camera of a seeded 100-frame path (disjoint from the 200-frame path
-the bin-capacity model is trained on).  Scene data is uploaded once, so the
-inputs are resident in HBM when the timed region starts.
+four 2048^2 RGBA8 maps, fp32 depth.  A step is one frame over the next
+camera of a seeded 100-frame path (disjoint from the 200-frame path the
+bin-capacity model is trained on), replayed from a captured graph.  Scene
+data is uploaded once, so the inputs are resident in HBM when the timed
+region starts.  Binning parameters come from the autotuner's table
+(custom-gpgpu-rasterizer_amd/tuning.json) unless given on the command line.
 
 Prints one JSON line: whole-frame Mtris/s (value) for learned single-pass
-binning, the same for exact two-pass binning, shaded Gpix/s, the shading
-pass against the HBM roofline (algorithmic bytes from the committed fixture
-tests/golden/terrain_shading_bytes.json), and the CPU oracle timed on the
-host cores (cpu_baseline).
+binning, the same for exact two-pass binning and for the previous-frame
+heuristic, the plain configuration (every triangle binned, no setup-time
+raster) for the binner comparison, shaded Gpix/s, the shading pass against
+the HBM roofline (algorithmic bytes from the committed fixture
+tests/golden/terrain_shading_bytes.json, HBM traffic from the committed
+rocprofv3 counter summary), the 256-job batched workload, and the CPU oracle
+timed on the host cores (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -28,6 +33,7 @@ sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 SHADING_FIXTURE = os.path.join(ROOT, "tests", "golden", "terrain_shading_bytes.json")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_shade_terrain4k_traffic.txt")
 
 
 def parse():
@@ -35,9 +41,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--tile", type=int, default=None)
+    ap.add_argument("--big", type=int, default=None, help="tile span from which a triangle goes to the screen list")
     ap.add_argument("--micro", type=int, default=None, help="bbox pixel limit of the setup-time raster")
     ap.add_argument("--small", type=int, default=None, help="tile coverage from which a triangle is rasterized per pixel")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-binner comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-job batched workload")
+    ap.add_argument("--only-batch", action="store_true", help="run only the 256-job batched workload")
+    ap.add_argument("--no-graph", action="store_true", help="enqueue stage by stage instead of replaying graphs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--train-frames", type=int, default=200)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--margin", type=float, default=1.25)
@@ -48,32 +61,144 @@ def parse():
     return ap.parse_args()
 
 
-def time_policy(R, r, cams, opts, policy, warmup, steps):
-    """Warm up, then time `steps` frames; returns (seconds, per-stage ms sums, overflow stats)."""
+def time_policy(R, r, cams, opts, policy, warmup, steps, graph=True):
+    """Warm up, then time `steps` frames (graph replay); then the same frames
+    again with per-stage HIP events.  Returns (seconds, per-stage ms, stats)."""
+    flags = R.FLAG_GRAPH if graph else 0
     for i in range(warmup):
-        r.render(cams[i % len(cams)], opts, policy, R.FLAG_TIMING)
+        r.render(cams[i % len(cams)], opts, policy, flags)
     r.sync()
-    keys = ("ms_vertex", "ms_model", "ms_setup", "ms_scan", "ms_fill", "ms_fallback", "ms_raster", "ms_shade",
-            "ms_total")
-    acc = {k: 0.0 for k in keys}
     ovf_tiles = 0
     cap_total = 0
     pairs = 0
     rerenders0 = r.stats()["rerenders"]
     t0 = time.perf_counter()
     for i in range(steps):
-        r.render(cams[(warmup + i) % len(cams)], opts, policy, R.FLAG_TIMING)
-        st = r.stats()  # waits for the frame: per-stage HIP events are read here
-        for k in keys:
-            acc[k] += st[k]
+        r.render(cams[(warmup + i) % len(cams)], opts, policy, flags)
+        st = r.stats()  # waits for the frame
         ovf_tiles += st["overflow_tiles"]
         cap_total += st["capacity_total"]
         pairs += st["num_bin_pairs"]
     r.sync()
     secs = time.perf_counter() - t0
+    keys = ("ms_vertex", "ms_model", "ms_setup", "ms_scan", "ms_fill", "ms_fallback", "ms_raster", "ms_shade",
+            "ms_total")
+    acc = {k: 0.0 for k in keys}
+    for i in range(steps):
+        r.render(cams[(warmup + i) % len(cams)], opts, policy, R.FLAG_TIMING)
+        st = r.stats()
+        for k in keys:
+            acc[k] += st[k]
     extra = {"overflow_tiles_per_frame": ovf_tiles / steps, "capacity_entries_per_frame": cap_total / steps,
              "bin_pairs_per_frame": pairs / steps, "rerenders": r.stats()["rerenders"] - rerenders0}
     return secs, {k: v / steps for k, v in acc.items()}, extra
+
+
+def train_model(R, r, w, train_path, args):
+    model = R.BinModel(r, hidden=64, seed=7)
+    model.attach(r, args.margin, args.slack)
+    for i, cam in enumerate(train_path):
+        r.render(cam, w.opts, R.BIN_EXACT, R.FLAG_NO_SHADE)
+        if i > 0:
+            model.record(r)
+    model.train(epochs=args.epochs, batch=8192, lr=2e-3, tau=0.95)
+    return model
+
+
+def run_policies(R, r, w, train_path, test_path, policies, args):
+    """Trains the capacity model (if the learned policy is asked for) and
+    times every policy on the test path; returns (results, train seconds)."""
+    results = {}
+    model = None
+    train_s = 0.0
+    if "learned" in policies:
+        t0 = time.perf_counter()
+        model = train_model(R, r, w, train_path, args)
+        train_s = time.perf_counter() - t0
+    code = {"exact": R.BIN_EXACT, "learned": R.BIN_LEARNED, "prevframe": R.BIN_PREVFRAME}
+    for p in policies:
+        # the projected-vertex density feeding the model is computed only while a model is attached
+        if model is not None:
+            if p == "learned":
+                model.attach(r, args.margin, args.slack)
+            else:
+                model.detach(r)
+        secs, stages, extra = time_policy(R, r, test_path, w.opts, code[p], args.warmup, args.steps,
+                                          graph=not args.no_graph)
+        results[p] = {"ms_per_frame": secs * 1e3 / args.steps, "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+                      **extra}
+    if model is not None:
+        model.close()
+    return results, train_s
+
+
+def batch_workload(R, S, autotune, args):
+    """The 256-job batch of uneven frames: each job's parameters from the
+    tuning table (its size bucket), all jobs in one batched launch."""
+    jobs = S.config4()
+    table = autotune.load_table()
+    rs = []
+    spec = []
+    for j in jobs:
+        r = R.Renderer(j.width, j.height)
+        autotune.choose(j.width, j.height, j.scene.num_triangles, table).apply(r)
+        r.add_scene(j.scene)
+        rs.append(r)
+        spec.append((r, j.camera, j.opts, R.BIN_EXACT, 0))
+    for _ in range(max(args.warmup, 2)):
+        R.render_batch(spec)
+    for r in rs:
+        r.sync()
+    steps = max(args.steps, 10)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.render_batch(spec)
+    for r in rs:
+        r.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    for r in rs:
+        r.close()
+    tris = sum(j.scene.num_triangles for j in jobs)
+    px = sum(j.pixels for j in jobs)
+    return {"workload": "256 jobs, 64..1024 px per axis (log-uniform), icospheres and terrains of 1k..200k tris",
+            "jobs": len(jobs), "triangles": tris, "pixels": px, "ms_per_batch": round(ms, 4),
+            "mtris_per_s": round(tris / (ms * 1e-3) / 1e6, 2), "gpix_per_s": round(px / (ms * 1e-3) / 1e9, 3),
+            "tuning": "per job-size bucket from tuning.json" if table else "defaults"}
+
+
+def read_traffic():
+    """HBM bytes per k_shade launch from the committed counter summary:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the factor
+    that maps the same counter onto a plain copy of known size."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    vals = {}
+    with open(TRAFFIC_SUMMARY) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    need = ("fetch_kib_per_dispatch", "write_kib_per_dispatch", "copy_scale_fetch", "copy_scale_write")
+    if not all(k in vals for k in need):
+        return None
+    return int((vals["fetch_kib_per_dispatch"] * vals["copy_scale_fetch"] +
+                vals["write_kib_per_dispatch"] * vals["copy_scale_write"]) * 1024)
+
+
+def copy_calibration(mib):
+    """Plain device copies of a known size, for scaling the HBM counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 5}))
 
 
 def cpu_baseline(w, cams, frames):
@@ -113,43 +238,44 @@ def main():
         print(json.dumps({"error": "this build renders on one GPU only; run with --gpus 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    import autotune
     import rast as R
     import scenes as S
 
+    if args.only_batch:
+        print(json.dumps({"batched": batch_workload(R, S, autotune, args)}))
+        return
     w = S.config3()
     train_path, test_path = S.train_test_paths(args.train_frames, 100)
-    r = R.Renderer(w.width, w.height, tile=args.tile)
-    if args.micro is not None or args.small is not None:
-        r.set_tuning(args.micro if args.micro is not None else 16, args.small if args.small is not None else 16)
-    r.add_scene(w.scene)
     ntri = w.scene.num_triangles
     P = w.width * w.height
-
+    tuned = autotune.choose(w.width, w.height, ntri)
+    tuned = autotune.Params(tile=args.tile or tuned.tile, micro=tuned.micro if args.micro is None else args.micro,
+                            small=tuned.small if args.small is None else args.small, big=args.big or tuned.big)
+    r = R.Renderer(w.width, w.height)
+    tuned.apply(r)
+    r.add_scene(w.scene)
     policies = [p for p in args.policies.split(",") if p]
-    results = {}
-    model = None
-    train_s = 0.0
-    if "learned" in policies:
-        t0 = time.perf_counter()
-        model = R.BinModel(r, hidden=64, seed=7)
-        model.attach(r, args.margin, args.slack)
-        for i, cam in enumerate(train_path):
-            r.render(cam, w.opts, R.BIN_EXACT, R.FLAG_NO_SHADE)
-            if i > 0:
-                model.record(r)
-        model.train(epochs=args.epochs, batch=8192, lr=2e-3, tau=0.95)
-        train_s = time.perf_counter() - t0
-    code = {"exact": R.BIN_EXACT, "learned": R.BIN_LEARNED, "prevframe": R.BIN_PREVFRAME}
-    for p in policies:
-        # the projected-vertex density feeding the model is computed only while a model is attached
-        if model is not None:
-            if p == "learned":
-                model.attach(r, args.margin, args.slack)
-            else:
-                model.detach(r)
-        secs, stages, extra = time_policy(R, r, test_path, w.opts, code[p], args.warmup, args.steps)
-        results[p] = {"ms_per_frame": secs * 1e3 / args.steps, "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-                      **extra}
+    results, train_s = run_policies(R, r, w, train_path, test_path, policies, args)
+    r.close()
+
+    plain = None
+    if not args.no_plain and "exact" in policies and "learned" in policies:
+        # every triangle binned (no setup-time raster), default tile: the plain binner comparison
+        pp = autotune.Params(tile=16, micro=0, small=16, big=256)
+        r2 = R.Renderer(w.width, w.height)
+        pp.apply(r2)
+        r2.add_scene(w.scene)
+        res2, _ = run_policies(R, r2, w, train_path, test_path, ["exact", "learned"], args)
+        r2.close()
+        plain = {"params": vars(pp), "exact_ms_per_frame": round(res2["exact"]["ms_per_frame"], 4),
+                 "learned_ms_per_frame": round(res2["learned"]["ms_per_frame"], 4),
+                 "learned_over_exact": round(res2["exact"]["ms_per_frame"] / res2["learned"]["ms_per_frame"], 3),
+                 "stages_ms": {k: v["stages_ms"] for k, v in res2.items()},
+                 "overflow_tiles_per_frame": res2["learned"]["overflow_tiles_per_frame"]}
 
     head = "learned" if "learned" in results else policies[0]
     ms = results[head]["ms_per_frame"]
@@ -166,9 +292,11 @@ def main():
         if t_shade > 0:
             ach = b_shade / t_shade / 1e9
             roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_shade",
-                        "algorithmic_bytes_per_launch": int(b_shade)}
+                        "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic(), "kernel": "k_shade",
+                        "algorithmic_bytes_per_launch": int(b_shade),
+                        "avg_launch_ms": round(results[head]["stages_ms"]["ms_shade"], 4)}
 
+    batch = None if args.no_batch else batch_workload(R, S, autotune, args)
     cpu = None if args.no_cpu else cpu_baseline(w, test_path, args.cpu_frames)
     line = {
         "metric": "Mtris/s + shaded Gpix/s at 4K, 2M-tri scene; learned vs exact binning",
@@ -183,22 +311,25 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural terrain, textures and camera path)",
-        "config": {"workload": w.name, "triangles": ntri, "width": w.width, "height": w.height, "tile": args.tile,
-                   "binning": head, "camera_path_frames": 100, "train_frames": args.train_frames},
+        "config": {"workload": w.name, "triangles": ntri, "width": w.width, "height": w.height,
+                   "binning": head, "params": vars(tuned), "camera_path_frames": 100,
+                   "train_frames": args.train_frames, "graph_replay": not args.no_graph},
         "gpix_per_s": round(P / (ms * 1e-3) / 1e9, 3),
         "policies": {k: {"mtris_per_s": round(ntri / (v["ms_per_frame"] * 1e-3) / 1e6, 2), **v}
                      for k, v in results.items()},
+        "plain_binner": plain,
+        "batched": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "train_seconds": round(train_s, 2),
     }
     if "exact" in results and "learned" in results:
         line["learned_over_exact"] = round(results["exact"]["ms_per_frame"] / results["learned"]["ms_per_frame"], 3)
+    if "prevframe" in results and "learned" in results:
+        line["learned_over_prevframe"] = round(
+            results["prevframe"]["ms_per_frame"] / results["learned"]["ms_per_frame"], 3)
     if roofline:
         line["shading_gpix_per_s"] = round(P / (results[head]["stages_ms"]["ms_shade"] * 1e-3) / 1e9, 2)
-    if model is not None:
-        model.close()
-    r.close()
     print(json.dumps(line))
 
 

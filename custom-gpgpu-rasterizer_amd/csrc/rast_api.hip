@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -138,11 +139,48 @@ static void camera_frame(const rast_camera &cam, uint32_t W, uint32_t H, rast_fr
     memcpy(fr.cam_up, cu, sizeof cu);
 }
 
+// Batches run on one process-wide stream.  A renderer whose last frame came
+// from batch k waits on the batch event (re-recorded by every batch, so it
+// covers k) unless a batch at least as recent is already known complete.
+struct BatchSync {
+    std::atomic<uint64_t> recorded{ 0 };
+    std::atomic<uint64_t> waited{ 0 };
+    hipEvent_t done = nullptr;
+};
+static BatchSync g_bsync;
+
+static hipError_t renderer_wait(rast_renderer *r)
+{
+    if (r->batch_seq) {
+        if (r->batch_seq > g_bsync.waited.load()) {
+            const uint64_t target = g_bsync.recorded.load();
+            const hipError_t e = hipEventSynchronize(g_bsync.done);
+            if (e != hipSuccess) return e;
+            uint64_t w = g_bsync.waited.load();
+            while (w < target && !g_bsync.waited.compare_exchange_weak(w, target)) {
+            }
+        }
+        r->batch_seq = 0;
+    }
+    if (r->stream_busy) {
+        const hipError_t e = hipStreamSynchronize(r->stream);
+        if (e != hipSuccess) return e;
+        r->stream_busy = false;
+    }
+    return hipSuccess;
+}
+
+int rast_settle(rast_renderer *r)
+{
+    HIP_TRY(renderer_wait(r));
+    return RAST_OK;
+}
+
 static hipError_t grow(rast_renderer *r, DBuf &b, size_t bytes, bool *synced)
 {
     if (b.p && b.n >= bytes) return hipSuccess;
     if (!*synced) {
-        hipError_t e = hipStreamSynchronize(r->stream);
+        hipError_t e = renderer_wait(r);
         if (e != hipSuccess) return e;
         *synced = true;
     }
@@ -266,7 +304,7 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
         return rast_set_error(RAST_E_STATE, "rast_renderer_destroy: %zu views still draw this renderer's meshes",
                               r->views.size());
     (void)hipSetDevice(r->device);
-    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    if (r->stream) (void)renderer_wait(r);
     if (r->scene_src) {
         std::vector<rast_renderer *> &v = r->scene_src->views;
         v.erase(std::remove(v.begin(), v.end(), r), v.end());
@@ -381,9 +419,9 @@ extern "C" int rast_mesh_destroy(rast_mesh *m)
     if (!m) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_destroy: null mesh");
     rast_renderer *r = m->r;
     (void)hipSetDevice(r->device);
-    HIP_TRY(hipStreamSynchronize(r->stream));
+    HIP_TRY(renderer_wait(r));
     for (rast_renderer *v : r->views) {
-        HIP_TRY(hipStreamSynchronize(v->stream));
+        HIP_TRY(renderer_wait(v));
         v->have_prev = false;
     }
     r->meshes.erase(std::remove(r->meshes.begin(), r->meshes.end(), m), r->meshes.end());
@@ -465,8 +503,8 @@ extern "C" int rast_texture_destroy(rast_texture *t)
     if (!t) return rast_set_error(RAST_E_INVALID_ARG, "rast_texture_destroy: null texture");
     rast_renderer *r = t->r;
     (void)hipSetDevice(r->device);
-    HIP_TRY(hipStreamSynchronize(r->stream));
-    for (rast_renderer *v : r->views) HIP_TRY(hipStreamSynchronize(v->stream));
+    HIP_TRY(renderer_wait(r));
+    for (rast_renderer *v : r->views) HIP_TRY(renderer_wait(v));
     for (rast_mesh *m : r->meshes)
         for (auto &slot : m->tex)
             if (slot == t) slot = nullptr;
@@ -587,6 +625,14 @@ void plan_free(LaunchPlan &p)
 {
     if (p.exec) (void)hipGraphExecDestroy(p.exec);
     p.exec = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (p.side[i]) (void)hipStreamDestroy(p.side[i]);
+        if (p.joined[i]) (void)hipEventDestroy(p.joined[i]);
+        p.side[i] = nullptr;
+        p.joined[i] = nullptr;
+    }
+    if (p.fork) (void)hipEventDestroy(p.fork);
+    p.fork = nullptr;
     if (p.done) (void)hipEventDestroy(p.done);
     p.done = nullptr;
     p.done_pending = false;
@@ -626,6 +672,13 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     P.exec = nullptr;
     P.built = false;
     if (!P.done) HIP_TRY(hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
+    if (!P.fork) {
+        HIP_TRY(hipEventCreateWithFlags(&P.fork, hipEventDisableTiming));
+        for (int i = 0; i < 2; ++i) {
+            HIP_TRY(hipStreamCreateWithFlags(&P.side[i], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&P.joined[i], hipEventDisableTiming));
+        }
+    }
     if (n > 65535) return rast_set_error(RAST_E_INVALID_ARG, "at most 65535 jobs per launch");
     Launch &L = P.L;
     L = Launch{};
@@ -855,7 +908,21 @@ static int enqueue_launch(LaunchPlan &P, hipStream_t s, rast_renderer *timed)
     RAST_TRY(mark(timed, EV_FILL, s));
     HIP_TRY(launch_fallback(L, s));
     RAST_TRY(mark(timed, EV_FALLBACK, s));
-    HIP_TRY(launch_resolve(L, s));
+    {
+        // tile-size groups are independent: the second and third run on side streams
+        int groups[3], ng = 0;
+        for (int st : { ST_RES16, ST_RES32, ST_RES8 })
+            if (L.grid[st]) groups[ng++] = st;
+        if (ng > 1) {
+            HIP_TRY(hipEventRecord(P.fork, s));
+            for (int i = 1; i < ng; ++i) HIP_TRY(hipStreamWaitEvent(P.side[i - 1], P.fork, 0));
+        }
+        for (int i = 0; i < ng; ++i) HIP_TRY(launch_resolve(L, groups[i], i == 0 ? s : P.side[i - 1]));
+        for (int i = 1; i < ng; ++i) {
+            HIP_TRY(hipEventRecord(P.joined[i - 1], P.side[i - 1]));
+            HIP_TRY(hipStreamWaitEvent(s, P.joined[i - 1], 0));
+        }
+    }
     RAST_TRY(mark(timed, EV_RASTER, s));
     HIP_TRY(launch_shade(L, s));
     RAST_TRY(mark(timed, EV_SHADE, s));
@@ -905,10 +972,10 @@ static void finish_enqueue(rast_renderer *r)
     r->cur ^= 1;
 }
 
-static int clear_keys_if_dirty(rast_renderer *r)
+static int clear_keys_if_dirty(rast_renderer *r, hipStream_t s)
 {
     if (!r->keys_dirty) return RAST_OK;
-    HIP_TRY(launch_clear_keys((unsigned long long *)r->keys.p, (size_t)r->W * r->H, r->stream));
+    HIP_TRY(launch_clear_keys((unsigned long long *)r->keys.p, (size_t)r->W * r->H, s));
     r->keys_dirty = false;
     return RAST_OK;
 }
@@ -939,7 +1006,8 @@ static int enqueue_frame(rast_renderer *r)
     const uint64_t key = plan_key(&r, 1, hidden);
     if (!r->plan.built || r->plan.key != key) RAST_TRY(build_plan(r->plan, &r, 1, hidden, key));
     RAST_TRY(stage_launch(r->plan, &r, 1));
-    RAST_TRY(clear_keys_if_dirty(r));
+    r->stream_busy = true;
+    RAST_TRY(clear_keys_if_dirty(r, r->stream));
     r->keys_dirty = true;  // until the frame (whose resolve pass resets the keys) is enqueued
     RAST_TRY(run_plan(r->plan, r->stream, graph, timing ? r : nullptr));
     r->keys_dirty = false;
@@ -951,7 +1019,7 @@ static int enqueue_frame(rast_renderer *r)
 static int check_frame(rast_renderer *r)
 {
     for (int attempt = 0; attempt < 4; ++attempt) {
-        HIP_TRY(hipStreamSynchronize(r->stream));
+        HIP_TRY(renderer_wait(r));
         const FrameCounters c = *r->h_counters;
         if (!c.storage_overflow) {
             r->frame_pending = false;
@@ -988,7 +1056,7 @@ extern "C" int rast_sync(rast_renderer *r)
     if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_sync: null renderer");
     HIP_TRY(hipSetDevice(r->device));
     if (r->frame_pending) return check_frame(r);
-    HIP_TRY(hipStreamSynchronize(r->stream));
+    HIP_TRY(renderer_wait(r));
     return RAST_OK;
 }
 
@@ -1099,7 +1167,6 @@ struct BatchCache {
     uint64_t last_use[4] = {};
     uint64_t clock = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
     int device = -1;
 };
 static BatchCache g_batch;
@@ -1130,13 +1197,15 @@ extern "C" int rast_render_batch(const rast_job *jobs, uint32_t n)
     if (g_batch.device != device) {
         if (g_batch.stream) {
             (void)hipStreamSynchronize(g_batch.stream);
+            g_bsync.waited.store(g_bsync.recorded.load());
             for (LaunchPlan &p : g_batch.plans) plan_free(p);
             (void)hipStreamDestroy(g_batch.stream);
-            (void)hipEventDestroy(g_batch.done);
+            (void)hipEventDestroy(g_bsync.done);
             g_batch.stream = nullptr;
+            g_bsync.done = nullptr;
         }
         HIP_TRY(hipStreamCreateWithFlags(&g_batch.stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&g_batch.done, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&g_bsync.done, hipEventDisableTiming));
         g_batch.device = device;
     }
     // host part of every job; each renderer's previous frame is checked first
@@ -1166,14 +1235,18 @@ extern "C" int rast_render_batch(const rast_job *jobs, uint32_t n)
     g_batch.last_use[pick] = ++g_batch.clock;
     RAST_TRY(stage_launch(P, rs.data(), n));
     for (rast_renderer *t : rs) {
-        RAST_TRY(clear_keys_if_dirty(t));
-        HIP_TRY(hipEventRecord(t->join, t->stream));  // the batch follows earlier work on t
-        HIP_TRY(hipStreamWaitEvent(g_batch.stream, t->join, 0));
+        if (t->stream_busy) {  // the batch follows earlier work on t
+            HIP_TRY(hipEventRecord(t->join, t->stream));
+            HIP_TRY(hipStreamWaitEvent(g_batch.stream, t->join, 0));
+        }
+        RAST_TRY(clear_keys_if_dirty(t, g_batch.stream));
     }
     RAST_TRY(run_plan(P, g_batch.stream, true, nullptr));
-    HIP_TRY(hipEventRecord(g_batch.done, g_batch.stream));
+    const uint64_t seq = g_bsync.recorded.load() + 1;
+    HIP_TRY(hipEventRecord(g_bsync.done, g_batch.stream));
+    g_bsync.recorded.store(seq);
     for (rast_renderer *t : rs) {
-        HIP_TRY(hipStreamWaitEvent(t->stream, g_batch.done, 0));  // later work on t follows the batch
+        t->batch_seq = seq;  // later use of t waits for this batch (renderer_wait)
         finish_enqueue(t);
     }
     return RAST_OK;

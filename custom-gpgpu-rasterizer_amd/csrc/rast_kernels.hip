@@ -603,43 +603,63 @@ __global__ __launch_bounds__(256) void k_fill(const JobDesc *__restrict__ jobs, 
 }
 
 // Single-pass fallback, step 1 (per tile): the exact count of a tile is its
-// final fill cursor; the entries beyond its capacity get a pool region.
+// final fill cursor; the entries beyond its capacity get a pool region.  The
+// block allocates for its 256 tiles with one atomic and adds its totals with
+// one atomic each (no per-wave traffic on the shared counters).
 __global__ __launch_bounds__(256) void k_fallback_alloc(const JobDesc *__restrict__ jobs,
                                                         const uint16_t *__restrict__ table)
 {
+    __shared__ uint32_t wsum[3][4];
+    __shared__ uint32_t base;
     uint32_t local;
     const JobDesc &J = jobs[job_of(table, jobs, ST_FBALLOC, local)];
     const uint32_t n = J.fp.tiles_x * J.fp.tiles_y;
     const uint32_t *__restrict__ count = J.tile_count;
     const uint32_t *__restrict__ cap = J.tile_cap;
-    uint32_t *__restrict__ tile_pool = J.tile_pool;
-    uint32_t *__restrict__ fb_cursor = J.tile_cursor;
-    const uint32_t pool_cap = J.pool_cap;
     FrameCounters *__restrict__ ctr = J.counters;
     const uint32_t t = local * 256u + threadIdx.x;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     uint32_t c = 0, excess = 0;
     if (t < n) {
         c = count[t];
         excess = c > cap[t] ? c - cap[t] : 0;
     }
-    uint32_t off = 0;
-    if (excess) {
-        off = atomicAdd(&ctr->pool_used, excess);
-        if (off + excess > pool_cap) ctr->storage_overflow = 1;
+    // inclusive wave prefix of the excess; wave totals of count and overflowed tiles
+    uint32_t incl = excess;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= (uint32_t)o) incl += y;
     }
-    if (t < n) {
-        tile_pool[t] = excess ? off : 0xFFFFFFFFu;
-        fb_cursor[t] = 0;
-    }
-    // frame totals: exact pair count and number of overflowed tiles
     uint32_t tot = c, ovt = excess ? 1u : 0u;
     for (int o = 32; o > 0; o >>= 1) {
         tot += __shfl_xor(tot, o);
         ovt += __shfl_xor(ovt, o);
     }
-    if (lane_id() == 0) {
-        if (tot) atomicAdd(&ctr->total_pairs, tot);
-        if (ovt) atomicAdd(&ctr->overflow_tiles, ovt);
+    if (lane == 63) wsum[0][wave] = incl;
+    if (lane == 0) {
+        wsum[1][wave] = tot;
+        wsum[2][wave] = ovt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t ex = wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
+        const uint32_t pairs = wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
+        const uint32_t tiles = wsum[2][0] + wsum[2][1] + wsum[2][2] + wsum[2][3];
+        uint32_t b0 = 0;
+        if (ex) {
+            b0 = atomicAdd(&ctr->pool_used, ex);
+            if (b0 + ex > J.pool_cap || b0 + ex < b0) ctr->storage_overflow = 1;
+        }
+        base = b0;
+        if (pairs) atomicAdd(&ctr->total_pairs, pairs);
+        if (tiles) atomicAdd(&ctr->overflow_tiles, tiles);
+    }
+    __syncthreads();
+    if (t < n) {
+        uint32_t before = base + incl - excess;
+        for (uint32_t q = 0; q < wave; ++q) before += wsum[0][q];
+        J.tile_pool[t] = excess ? before : 0xFFFFFFFFu;
+        J.tile_cursor[t] = 0;
     }
 }
 
@@ -1175,26 +1195,21 @@ hipError_t launch_fallback(const Launch &L, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_resolve(const Launch &L, hipStream_t s)
+hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s)
 {
-    if (L.grid[ST_RES8]) {
-        hipLaunchKernelGGL((k_resolve<8, 64, ST_RES8>), dim3(L.grid[ST_RES8]), dim3(64), 0, s, L.jobs, L.table[ST_RES8]);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+    const uint32_t g = L.grid[stage];
+    if (!g) return hipSuccess;
+    switch (stage) {
+    case ST_RES8: hipLaunchKernelGGL((k_resolve<8, 64, ST_RES8>), dim3(g), dim3(64), 0, s, L.jobs, L.table[ST_RES8]); break;
+    case ST_RES16:
+        hipLaunchKernelGGL((k_resolve<16, 256, ST_RES16>), dim3(g), dim3(256), 0, s, L.jobs, L.table[ST_RES16]);
+        break;
+    case ST_RES32:
+        hipLaunchKernelGGL((k_resolve<32, 256, ST_RES32>), dim3(g), dim3(256), 0, s, L.jobs, L.table[ST_RES32]);
+        break;
+    default: return hipErrorInvalidValue;
     }
-    if (L.grid[ST_RES16]) {
-        hipLaunchKernelGGL((k_resolve<16, 256, ST_RES16>), dim3(L.grid[ST_RES16]), dim3(256), 0, s, L.jobs,
-                           L.table[ST_RES16]);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if (L.grid[ST_RES32]) {
-        hipLaunchKernelGGL((k_resolve<32, 256, ST_RES32>), dim3(L.grid[ST_RES32]), dim3(256), 0, s, L.jobs,
-                           L.table[ST_RES32]);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 hipError_t launch_shade(const Launch &L, hipStream_t s)

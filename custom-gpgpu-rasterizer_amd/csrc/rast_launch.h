@@ -35,7 +35,8 @@ hipError_t launch_scan(const Launch &L, bool capacities, hipStream_t s);
 hipError_t launch_setup(const Launch &L, hipStream_t s);
 hipError_t launch_fill(const Launch &L, hipStream_t s);
 hipError_t launch_fallback(const Launch &L, hipStream_t s);
-hipError_t launch_resolve(const Launch &L, hipStream_t s);
+// One tile-size group (ST_RES8 / ST_RES16 / ST_RES32); the groups are independent.
+hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s);
 hipError_t launch_shade(const Launch &L, hipStream_t s);
 hipError_t launch_publish(const Launch &L, hipStream_t s);
 

@@ -79,14 +79,18 @@ __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t
 }
 
 // Frame-time capacities of the single-pass jobs of a launch.  Learned: a
-// block serves 64 tiles (one per lane); each of its 4 waves computes a
-// quarter of the hidden units, weights come through scalar loads (uniform
-// across the wave), layer-1 activations are shared in LDS.  Otherwise the
-// previous frame's count: ceil(prev * margin) + slack.
+// block serves 64 tiles (one per lane).  The parameters are staged in LDS
+// once per block and read as wave-uniform broadcasts; wave 0 gathers the
+// features, each of the 4 waves computes a quarter of both hidden layers,
+// and the output sums the four partial dot products in a fixed order.
+// Otherwise the previous frame's count: ceil(prev * margin) + slack.
 template <int H>
 __global__ __launch_bounds__(256) void k_capacity(const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table)
 {
     constexpr int Q = H / 4;
+    constexpr int NP = H * F + H + H * H + H + H + 1;
+    __shared__ __align__(16) float wts[(NP + 3) & ~3];
+    __shared__ float xs[F][64];
     __shared__ float h1s[H][65];
     __shared__ float part[4][64];
     uint32_t local;
@@ -107,10 +111,19 @@ __global__ __launch_bounds__(256) void k_capacity(const JobDesc *__restrict__ jo
         return;
     }
     const float *__restrict__ p = J.model;
-    const float *W1 = p, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
+    for (int i = (int)threadIdx.x; i < NP; i += 256) wts[i] = p[i];
+    if (w == 0) {
+        float x[F];
+        if (t < n) tile_features(fp, t, prev, J.prev_depth, J.density, x);
+        else for (int f = 0; f < F; ++f) x[f] = 0.f;
+#pragma unroll
+        for (int f = 0; f < F; ++f) xs[f][lane] = x[f];
+    }
+    __syncthreads();
+    const float *W1 = wts, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
     float x[F];
-    if (t < n) tile_features(fp, t, prev, J.prev_depth, J.density, x);
-    else for (int f = 0; f < F; ++f) x[f] = 0.f;
+#pragma unroll
+    for (int f = 0; f < F; ++f) x[f] = xs[f][lane];
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
         const int k = (int)w * Q + u;
@@ -126,9 +139,16 @@ __global__ __launch_bounds__(256) void k_capacity(const JobDesc *__restrict__ jo
     float o = 0.f;
     for (int u = 0; u < Q; ++u) {
         const int j = (int)w * Q + u;
+        const float4 *row = reinterpret_cast<const float4 *>(W2 + j * H);  // H * 4 bytes per row: 16-byte aligned
         float a = b2[j];
 #pragma unroll
-        for (int k = 0; k < H; ++k) a += W2[j * H + k] * h1[k];
+        for (int k4 = 0; k4 < H / 4; ++k4) {
+            const float4 wv = row[k4];
+            a += wv.x * h1[4 * k4];
+            a += wv.y * h1[4 * k4 + 1];
+            a += wv.z * h1[4 * k4 + 2];
+            a += wv.w * h1[4 * k4 + 3];
+        }
         o += W3[j] * (a > 0.f ? a : 0.f);
     }
     part[w][lane] = o;
@@ -507,7 +527,7 @@ extern "C" int rast_binmodel_destroy(rast_binmodel *m)
     (void)hipSetDevice(m->device);
     for (rast_renderer *r : m->users)
         if (r) {
-            (void)hipStreamSynchronize(r->stream);
+            (void)rast_settle(r);
             if (r->model == m) r->model = nullptr;
         }
     DBuf *bufs[] = { &m->params, &m->grad, &m->adam_m, &m->adam_v, &m->X, &m->Y, &m->idx, &m->partial, &m->loss,

@@ -58,8 +58,14 @@ struct LaunchPlan {
     hipGraphExec_t exec = nullptr;  // captured launch (graph mode)
     hipEvent_t done = nullptr;      // end of the plan's last launch (staging reusable)
     bool done_pending = false;
+    // side streams running the resolve passes of different tile sizes concurrently
+    hipStream_t side[2] = { nullptr, nullptr };
+    hipEvent_t fork = nullptr, joined[2] = { nullptr, nullptr };
 };
 void plan_free(LaunchPlan &p);
+// Waits until no queued GPU work uses r's buffers (its stream and, when its
+// last frame came from a batch, that batch).
+int rast_settle(rast_renderer *r);
 
 struct rast_texture {
     rast_renderer *r = nullptr;
@@ -90,6 +96,8 @@ struct rast_renderer {
     uint64_t scene_version = 1;            // bumped when the mesh list changes
     uint64_t layout_version = 0;           // scene version the block tables were built for
     hipEvent_t join = nullptr;             // orders a batch after this renderer's earlier work
+    uint64_t batch_seq = 0;                // batch that rendered the last frame (0: none pending)
+    bool stream_busy = false;              // work queued on `stream` since its last synchronization
     LaunchPlan plan;                       // single-frame launches
     // scene layout
     uint32_t total_verts = 0, total_tris = 0;
