@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-binner comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-job batched workload")
     ap.add_argument("--only-batch", action="store_true", help="run only the 256-job batched workload")
+    ap.add_argument("--no-flythrough", action="store_true", help="skip the 7-view flythrough workload")
+    ap.add_argument("--only-flythrough", action="store_true", help="run only the 7-view flythrough workload")
+    ap.add_argument("--fly-frames", type=int, default=300)
+    ap.add_argument("--fly-train-frames", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true", help="enqueue stage by stage instead of replaying graphs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
@@ -166,6 +170,87 @@ def batch_workload(R, S, autotune, args):
             "tuning": "per job-size bucket from tuning.json" if table else "defaults"}
 
 
+def flythrough_workload(R, S, autotune, args):
+    """The combined workload: terrain + 64 instanced icospheres (4,001,920
+    tris), every frame one batched launch of the 3840x2160 main view and six
+    1024^2 cube-map faces (views of one scene) with learned single-pass
+    binning.  A capacity model per view is trained on a separate path first.
+    A second pass over the path compares every 10th frame with the exact
+    binner (vis and depth bit for bit)."""
+    import numpy as np
+
+    fly = S.config5(frames=args.fly_frames)
+    ntri = fly.scene.num_triangles
+    table = autotune.load_table()
+
+    def views_for():
+        owner = R.Renderer(*fly.main)
+        autotune.choose(fly.main[0], fly.main[1], ntri, table).apply(owner)
+        owner.add_scene(fly.scene)
+        vs = [owner]
+        for _ in range(6):
+            v = owner.view(fly.face, fly.face)
+            autotune.choose(fly.face, fly.face, ntri, table).apply(v)
+            vs.append(v)
+        return vs
+
+    learned, exact = views_for(), views_for()
+    models = [R.BinModel(v, hidden=64, seed=31 + k) for k, v in enumerate(learned)]
+    for m, v in zip(models, learned):
+        m.attach(v, args.margin, args.slack)
+    train = S.Flythrough(fly.scene, S.camera_path(S.SEED_BASE + 0x505, args.fly_train_frames), fly.main, fly.face)
+    t0 = time.perf_counter()
+    for f in range(len(train.path)):
+        R.render_batch([(v, cam, fly.opts, R.BIN_EXACT, R.FLAG_NO_SHADE) for v, (cam, _, _) in zip(learned, train.views(f))])
+        if f > 0:
+            for m, v in zip(models, learned):
+                m.record(v)
+    for m in models:
+        m.train(epochs=args.epochs, batch=8192, lr=2e-3, tau=0.95)
+    train_s = time.perf_counter() - t0
+
+    def jobs(vs, f, policy):
+        return [(v, cam, fly.opts, policy, 0) for v, (cam, _, _) in zip(vs, fly.views(f))]
+
+    for f in range(3):
+        R.render_batch(jobs(learned, f, R.BIN_LEARNED))
+    ovf = tiles = 0
+    t0 = time.perf_counter()
+    for f in range(len(fly.path)):
+        R.render_batch(jobs(learned, f, R.BIN_LEARNED))
+        for v in learned:
+            st = v.stats()
+            ovf += st["overflow_tiles"]
+            tiles += st["num_tiles"]
+    secs = time.perf_counter() - t0
+    checked = mismatched = 0
+    for f in range(len(fly.path)):
+        R.render_batch(jobs(learned, f, R.BIN_LEARNED))
+        if f % 10 == 0:
+            R.render_batch(jobs(exact, f, R.BIN_EXACT))
+            for a, b in zip(learned, exact):
+                va, da, _ = a.readback(rgba=False)
+                vb, db, _ = b.readback(rgba=False)
+                mismatched += int(not (np.array_equal(va, vb) and np.array_equal(da.view(np.uint32), db.view(np.uint32))))
+            checked += 1
+    for m in models:
+        m.close()
+    for vs in (learned, exact):
+        for v in vs[1:]:
+            v.close()
+        vs[0].close()
+    frames = len(fly.path)
+    px = fly.main[0] * fly.main[1] + 6 * fly.face * fly.face
+    ms = secs * 1e3 / frames
+    return {"workload": f"terrain {1160}^2 + 64 icospheres s5, main 3840x2160 + six {fly.face}^2 cube faces per frame",
+            "frames": frames, "views_per_frame": 7, "triangles": ntri, "pixels_per_frame": px,
+            "ms_per_frame": round(ms, 4), "mtris_per_s": round(ntri * 7 / (ms * 1e-3) / 1e6, 2),
+            "gpix_per_s": round(px / (ms * 1e-3) / 1e9, 3), "binning": "learned (one model per view)",
+            "fallback_tiles_per_frame": round(ovf / frames, 2), "fallback_rate": round(ovf / max(tiles, 1), 5),
+            "checked_frames": checked, "views_checked": checked * 7, "mismatched_views": mismatched,
+            "train_seconds": round(train_s, 2)}
+
+
 def read_traffic():
     """HBM bytes per k_shade launch from the committed counter summary:
     FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the factor
@@ -248,6 +333,9 @@ def main():
     if args.only_batch:
         print(json.dumps({"batched": batch_workload(R, S, autotune, args)}))
         return
+    if args.only_flythrough:
+        print(json.dumps({"flythrough": flythrough_workload(R, S, autotune, args)}))
+        return
     w = S.config3()
     train_path, test_path = S.train_test_paths(args.train_frames, 100)
     ntri = w.scene.num_triangles
@@ -297,6 +385,7 @@ def main():
                         "avg_launch_ms": round(results[head]["stages_ms"]["ms_shade"], 4)}
 
     batch = None if args.no_batch else batch_workload(R, S, autotune, args)
+    fly = None if args.no_flythrough else flythrough_workload(R, S, autotune, args)
     cpu = None if args.no_cpu else cpu_baseline(w, test_path, args.cpu_frames)
     line = {
         "metric": "Mtris/s + shaded Gpix/s at 4K, 2M-tri scene; learned vs exact binning",
@@ -319,6 +408,7 @@ def main():
                      for k, v in results.items()},
         "plain_binner": plain,
         "batched": batch,
+        "flythrough": fly,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "train_seconds": round(train_s, 2),

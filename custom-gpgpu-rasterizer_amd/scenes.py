@@ -156,7 +156,9 @@ def noise_texture(w: int, h: int, seed: int, kind: int) -> np.ndarray:
 
 
 def noise_maps(size: int, seed: int) -> list:
-    return [noise_texture(size, size, seed + k, k) for k in range(4)]
+    # the generator releases the GIL inside its ctypes call: the four maps in parallel
+    with ThreadPoolExecutor(4) as ex:
+        return list(ex.map(lambda k: noise_texture(size, size, seed + k, k), range(4)))
 
 
 def world_matrix(translate, euler, scale) -> np.ndarray:

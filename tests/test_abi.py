@@ -83,3 +83,18 @@ def test_renderer_create_without_a_device_fails_cleanly():
         pytest.skip("a HIP device is present")
     assert rc in (-5, -2) and not h.value
     assert R.last_error()
+
+
+def test_cpp_header_compiles_against_reference_shaped_types():
+    """include/rast.hpp: the reference class names compile as C++17 with
+    app-side types shaped like the reference's (syntax and template check)."""
+    import shutil
+    import subprocess
+
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = os.path.join(ROOT, "tests", "cpp", "test_rast_cpp.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "oracle"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
