@@ -4,8 +4,9 @@
 // HBM layout of one frame (all device buffers owned by the renderer):
 //   ivert    per mesh, 14 floats / vertex (the 56-byte IVertex, as uploaded)
 //   pos4     float4 / global vertex: NDC x, y, z and clip w        (16 B)
-//   attr     4 x float4 / global vertex: u v 1/w n'.x | n'.yz tan'.xy |
-//            tan'.z vd | c.rgb 0 (64 B; PBR shading reads the first 48 B)
+//   attr     4 x float4 / global vertex, each attribute times 1/w:
+//            u v 1/w n'.x | n'.yz tan'.xy | tan'.z vd | c.rgb 0
+//            (64 B; PBR shading reads the first 48 B)
 //   trirec   16 words / global triangle ID (written for survivors):
 //            [0..5] screen x0,y0,x1,y1,x2,y2  [6..8] z0,z1,z2  [9] |area|
 //            [10] xmin | ymin << 16  [11] xmax | ymax << 16 (pixel range

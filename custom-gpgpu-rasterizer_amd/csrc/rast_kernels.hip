@@ -311,12 +311,16 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
     const float w2 = m4(Wd, 2, 0) * px + m4(Wd, 2, 1) * py + m4(Wd, 2, 2) * pz + m4(Wd, 2, 3);
     float vx = w0 - M.cam_pos[0], vy = w1 - M.cam_pos[1], vz = w2 - M.cam_pos[2];
     normalize3(vx, vy, vz);
-    // PBR shading reads the first three float4s only; the colour is last
+    // Every attribute is stored multiplied by 1/w: the interpolation of
+    // CCR/CUDAROPs.cu:347-380 forms w_k * (c_k * (1/w_k)) per pixel, and the
+    // product c_k * (1/w_k) is the same binary32 value whether it is formed
+    // here once or per pixel.  PBR shading reads the first three float4s only.
+    const float iw = 1.f / o3;
     float4 *a = attr + (size_t)g * 4;
-    a[0] = make_float4(u, vv, 1.f / o3, n0);
-    a[1] = make_float4(n1, n2, t0, t1);
-    a[2] = make_float4(t2, vx, vy, vz);
-    a[3] = make_float4(cr, cg, cb, 0.f);
+    a[0] = make_float4(u * iw, vv * iw, iw, n0 * iw);
+    a[1] = make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw);
+    a[2] = make_float4(t2 * iw, vx * iw, vy * iw, vz * iw);
+    a[3] = make_float4(cr * iw, cg * iw, cb * iw, 0.f);
     if (density && (v & 7u) == 0) {
         // projected-vertex density feature of the capacity model (every 8th
         // vertex; no parity role)
@@ -333,12 +337,16 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
 // whole bbox with the order-free key).
 __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshes,
                                                const uint32_t *__restrict__ block_mesh,
-                                               const JobDesc *__restrict__ jobs)
+                                               const JobDesc *__restrict__ jobs, uint32_t nblocks)
 {
     __shared__ uint32_t lds_append[12];
     __shared__ TriLds mstage[256];
     __shared__ uint32_t moff[256];
-    const uint32_t mi = block_mesh[blockIdx.x];
+    // workgroups in launch order (consecutive blocks spread over the XCDs;
+    // XCD-contiguous ranges measured slower)
+    const uint32_t blk = blockIdx.x;
+    if (blk >= nblocks) return;
+    const uint32_t mi = block_mesh[blk];
     const DevMesh &M = meshes[mi];
     const JobDesc &J = jobs[M.job];
     const FrameParams &fp = J.fp;
@@ -348,7 +356,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     unsigned long long *__restrict__ keys = J.keys;
     FrameCounters *__restrict__ ctr = J.counters;
     const bool single_pass = J.single != 0;
-    const uint32_t k = (blockIdx.x - M.tri_block0) * 256u + threadIdx.x;
+    const uint32_t k = (blk - M.tri_block0) * 256u + threadIdx.x;
     const bool valid = k < M.num_tris;
     const uint32_t gid = M.tri_base + k;
     bool ok = false;
@@ -686,20 +694,38 @@ __global__ __launch_bounds__(256) void k_fallback_fill(const JobDesc *__restrict
     }
 }
 
-// Conservative depth bound of t: every computed fragment z of t exceeds the
-// returned value, so t loses at any pixel whose current depth d < bound.
-// Only for well-conditioned triangles (|area| >= 0.1 * longest edge^2, all
-// z > 0): each computed barycentric is then within ~62 ulp of the exact one
-// and every computed fragment z is >= min(z) * (1 - 3e-5); the 2e-3 margin
-// keeps the bound strictly below.  Returns -1 (no rejection) otherwise.
-__device__ __forceinline__ float cull_depth(const TriLds &t)
+// Conservative depth plane of t over the tile at (X0, Y0): every fragment
+// depth the exact test can compute for t at pixel p satisfies
+//   z(p) >= c.x + c.y * (px - X0) + c.z * (py - Y0),
+// so t loses at p (its key is larger) whenever that bound exceeds the
+// pixel's current depth.  The exact depth of any pixel is the plane through
+// the three screen-space vertices; the computed one differs from it by the
+// rounding of the edge functions, the area, the divisions and the weighted
+// sum.  For a well-conditioned triangle (area >= 0.1 * longest edge^2, longest
+// edge >= 16 px) that difference is below 275 ulp(1) * zmax; the margin takes
+// 512 ulp(1) * zmax plus the float evaluation of the plane over the tile.
+// The plane is formed in double.  Returns c.x = -1e30 (no rejection) when t
+// does not qualify.
+__device__ __forceinline__ float4 cull_plane(const TriLds &t, int X0, int Y0, int T)
 {
-    const float zlo = fminf(t.z0, fminf(t.z1, t.z2));
-    if (!(zlo > 0.f)) return -1.f;
+    const float4 none = make_float4(-1.0e30f, 0.f, 0.f, 0.f);
+    const float zlo = fminf(t.z0, fminf(t.z1, t.z2)), zhi = fmaxf(t.z0, fmaxf(t.z1, t.z2));
+    if (!(zlo > 0.f) || !(zhi < 2.f)) return none;
     const float ea = t.ax * t.ax + t.ay * t.ay, eb = t.bx * t.bx + t.by * t.by, ec = t.cx * t.cx + t.cy * t.cy;
-    const float emax = fmaxf(ea, fmaxf(eb, ec));
-    if (!(t.area >= 0.1f * emax)) return -1.f;
-    return zlo * (1.0f - 2.0e-3f);
+    const float emax2 = fmaxf(ea, fmaxf(eb, ec));
+    if (!(t.area >= 0.1f * emax2) || !(emax2 >= 256.f) || !(emax2 < 1.0e12f)) return none;
+    const double e1x = (double)t.sx1 - (double)t.sx0, e1y = (double)t.sy1 - (double)t.sy0;
+    const double e1z = (double)t.z1 - (double)t.z0;
+    const double e2x = (double)t.sx2 - (double)t.sx0, e2y = (double)t.sy2 - (double)t.sy0;
+    const double e2z = (double)t.z2 - (double)t.z0;
+    const double nz = e1x * e2y - e1y * e2x;
+    if (!(fabs(nz) > 0.0)) return none;
+    const double dzdx = -(e1y * e2z - e1z * e2y) / nz;
+    const double dzdy = -(e1z * e2x - e1x * e2z) / nz;
+    const double zt = (double)t.z0 + dzdx * ((double)X0 - (double)t.sx0) + dzdy * ((double)Y0 - (double)t.sy0);
+    const double u = 1.0 / 16777216.0;  // 2^-24
+    const double margin = 512.0 * u * (double)zhi + 8.0 * u * (fabs(zt) + (fabs(dzdx) + fabs(dzdy)) * (double)T);
+    return make_float4((float)(zt - margin), (float)dzdx, (float)dzdy, 0.f);
 }
 
 // RA_PerTileKernel (CCR/CUDARenderer.cu:1039-1129) redesigned: one workgroup
@@ -741,15 +767,18 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     static_assert(NPIX % NT == 0, "tile pixels must be a multiple of the workgroup size");
     __shared__ uint64_t key[NPIX];
     __shared__ TriLds stage[NT];
+    __shared__ float4 bplane[NT];  // screen-list triangles: conservative depth plane over the tile
     __shared__ uint32_t off[NT];
     __shared__ uint32_t wlist[NT];
     __shared__ uint32_t wred_u[NW], wwhole[NW];
 
     const uint32_t ntiles = fp.tiles_x * fp.tiles_y;
-    // XCD-aware order: consecutive workgroups go round-robin over the 8 XCDs,
-    // so give each XCD a contiguous band of tiles (neighbours share triangles).
-    const uint32_t per_xcd = grid_tiles / 8u;
-    const uint32_t tile = (local % 8u) * per_xcd + local / 8u;
+    // Tiles in launch order: the hardware spreads consecutive workgroups over
+    // the 8 XCDs, so neighbouring tiles run on different XCDs at once.  Giving
+    // each XCD a contiguous band instead measured 30% slower here (the bands'
+    // key, vis and depth traffic then concentrates on fewer memory channels).
+    (void)grid_tiles;
+    const uint32_t tile = local;
     if (tile >= ntiles) return;
     const uint32_t txi = tile % fp.tiles_x, tyi = tile / fp.tiles_x;
     const int X0 = (int)(txi * T), Y0 = (int)(tyi * T);
@@ -898,9 +927,10 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             const uint32_t id = big[i];
             if (id < num_tris) {
                 load_tri(trirec, id, t);
-                t.inv_w = cull_depth(t);
+                bplane[tid] = cull_plane(t, X0, Y0, T);
             } else {
                 t.xr = 0; t.yr = 0;  // empty range
+                bplane[tid] = make_float4(-1.0e30f, 0.f, 0.f, 0.f);
             }
             stage[tid] = t;
         }
@@ -911,10 +941,12 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             const int x0 = (int)(st.xr & 0xFFFFu), y0 = (int)(st.xr >> 16);
             const int x1 = (int)(st.yr & 0xFFFFu), y1 = (int)(st.yr >> 16);
             if (x1 <= X0 || x0 >= X0 + T || y1 <= Y0 || y0 >= Y0 + T) continue;
-            const float zc = st.inv_w;  // cull depth
+            const float4 pl = bplane[j];
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 const int px = pxs[q], py = pys[q];
+                // lower bound of the triangle's depth here; above the current depth -> it loses
+                const float zc = pl.x + pl.y * (float)(px - X0) + pl.z * (float)(py - Y0);
                 if (px >= x0 && px < x1 && py >= y0 && py < y1 &&
                     !(zc > __uint_as_float((uint32_t)(best[q] >> 32)))) {
                     float z;
@@ -1034,9 +1066,10 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     const uint32_t *__restrict__ trirec = J.trirec;
     const float4 *__restrict__ attr = J.attr;
     uint32_t *__restrict__ rgba = J.rgba;
+    const uint32_t strip = local;
     const uint32_t bw = (fp.width + 63u) / 64u;
-    const uint32_t x = (local % bw) * 64u + (threadIdx.x & 63u);
-    const uint32_t y = (local / bw) * 4u + (threadIdx.x >> 6);
+    const uint32_t x = (strip % bw) * 64u + (threadIdx.x & 63u);
+    const uint32_t y = (strip / bw) * 4u + (threadIdx.x >> 6);
     if (x >= fp.width || y >= fp.height) return;
     const size_t pid = (size_t)y * fp.width + x;
     const uint32_t id = vis[pid];
@@ -1060,7 +1093,8 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     const float4 a01 = A0[1], a11 = A1[1], a21 = A2[1];
     const float i0 = a00.z, i1 = a10.z, i2 = a20.z;
     const float wI = 1.f / (i0 * w0 + i1 * w1 + i2 * w2);
-#define RAST_INTERP(c0, c1, c2) (w0 * ((c0) * i0) + w1 * ((c1) * i1) + w2 * ((c2) * i2))
+    // attributes arrive multiplied by their vertex's 1/w
+#define RAST_INTERP(c0, c1, c2) (w0 * (c0) + w1 * (c1) + w2 * (c2))
     const float u = RAST_INTERP(a00.x, a10.x, a20.x) * wI;
     const float v = RAST_INTERP(a00.y, a10.y, a20.y) * wI;
     const float nx = RAST_INTERP(a00.w, a10.w, a20.w) * wI;
@@ -1167,7 +1201,7 @@ hipError_t launch_vertex(const Launch &L, hipStream_t s)
 hipError_t launch_setup(const Launch &L, hipStream_t s)
 {
     if (L.tri_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_setup, dim3(L.tri_blocks), dim3(256), 0, s, L.meshes, L.tri_block_mesh, L.jobs);
+    hipLaunchKernelGGL(k_setup, dim3(L.tri_blocks), dim3(256), 0, s, L.meshes, L.tri_block_mesh, L.jobs, L.tri_blocks);
     return hipGetLastError();
 }
 

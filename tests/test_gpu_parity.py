@@ -83,9 +83,12 @@ def test_vertex_outputs_bitwise():
         attr = r.debug_read(R.DEBUG_VERTEX_ATTR, np.float32).reshape(-1, 16)
     opos, oattr = O.vertex_transform(w.scene, w.camera, w.width, w.height)
     assert_bitwise("pos4", pos, opos)
-    # GPU record: u v 1/w n.x | n.y n.z t.x t.y | t.z vd.xyz | c.rgb 0; oracle: uv n tan vd c 1/w
+    # GPU record: every attribute times 1/w: u v 1/w n.x | n.y n.z t.x t.y | t.z vd.xyz | c.rgb 0;
+    # oracle: uv n tan vd c 1/w (the product is one binary32 multiply on both sides)
     gpu_in_oracle_order = attr[:, [0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 2]]
-    assert_bitwise("attr", np.ascontiguousarray(gpu_in_oracle_order), np.ascontiguousarray(oattr[:, :15]))
+    inv_w = oattr[:, 14:15].astype(np.float32)
+    want = np.concatenate([(oattr[:, :14].astype(np.float32) * inv_w).astype(np.float32), inv_w], axis=1)
+    assert_bitwise("attr", np.ascontiguousarray(gpu_in_oracle_order), np.ascontiguousarray(want))
 
 
 def test_setup_records_bitwise():
