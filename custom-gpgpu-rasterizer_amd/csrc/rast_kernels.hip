@@ -163,6 +163,7 @@ __device__ __forceinline__ int clamp_trunc(float v, float hi)
 }
 
 constexpr uint64_t kClearKey = ((uint64_t)0x3F800000u << 32) | 0xFFFFFFFFull;  // depth 1.0, no triangle
+constexpr int kMaskChunks = 64;  // 64-item chunks per start-mask window (one mask word per lane)
 
 // Depth-test key: min z first, then max ID (low word = ~(id + 1)).
 __device__ __forceinline__ uint64_t make_key(float z, uint32_t id)
@@ -342,6 +343,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     __shared__ uint32_t lds_append[12];
     __shared__ TriLds mstage[256];
     __shared__ uint32_t moff[256];
+    __shared__ unsigned long long mmask[4 * kMaskChunks];
     // workgroups in launch order (consecutive blocks spread over the XCDs;
     // XCD-contiguous ranges measured slower)
     const uint32_t blk = blockIdx.x;
@@ -449,39 +451,61 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     // Micro triangles: the wave rasterizes the union of their pixel ranges as
     // one item list (prefix sum over the ranges), so a lane's work does not
     // depend on its own triangle's size; fragments go to the key buffer with
-    // 64-bit atomic min.
+    // 64-bit atomic min.  The micro triangles are compacted in LDS; for every
+    // 64-item chunk a 64-bit mask marks the items where a triangle starts, so
+    // an item's triangle is the count of starts up to it (one popcount).
     {
         const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
-        const uint32_t area = micro ? npix : 0u;
-        uint32_t incl = area;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o);
-            if (lane >= (uint32_t)o) incl += y;
-        }
-        const uint32_t total = __shfl(incl, 63);
-        if (total) {
+        const uint64_t mm = __ballot(micro);
+        if (mm) {
+            const uint32_t rank = popc_below(mm);
+            const uint32_t area = micro ? npix : 0u;
+            uint32_t incl = area;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                if (lane >= (uint32_t)o) incl += y;
+            }
+            const uint32_t total = __shfl(incl, 63);
+            const uint32_t start = incl - area;
             TriLds *ws = mstage + wave * 64;
             uint32_t *wo = moff + wave * 64;
+            unsigned long long *wm = mmask + wave * kMaskChunks;
             if (micro) {
                 set_edges(t);
                 t.xr = (uint32_t)xmin | ((uint32_t)xmax << 16);
                 t.yr = (uint32_t)ymin | ((uint32_t)ymax << 16);
                 t.id = gid;
                 t.inv_w = 1.0f / (float)(xmax - xmin);
-                ws[lane] = t;
+                ws[rank] = t;
+                wo[rank] = start;
             }
-            wo[lane] = incl - area;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            for (uint32_t kk = lane; kk < total; kk += 64) {
-                const uint32_t j = find_item_owner(wo, 64, kk);
-                const TriLds &mt = ws[j];
-                int px, py;
-                item_pixel(mt, kk - wo[j], px, py);
-                float z;
-                if (fragment(mt, (float)px, (float)py, z))
-                    atomicMin(keys + (size_t)py * fp.width + px, (unsigned long long)make_key(z, mt.id));
+            uint32_t before = 0;  // triangles starting before the current chunk (wave-uniform)
+            // windows of kMaskChunks chunks (one pass unless micro_limit is large)
+            for (uint32_t wbase = 0; wbase < total; wbase += 64u * kMaskChunks) {
+                const uint32_t nch = min((total - wbase + 63u) >> 6, (uint32_t)kMaskChunks);
+                wm[lane] = 0ull;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (micro && start >= wbase && start - wbase < 64u * kMaskChunks)
+                    atomicOr(wm + ((start - wbase) >> 6), 1ull << (start & 63u));
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (uint32_t c = 0; c < nch; ++c) {
+                    const uint64_t m = wm[c];
+                    const uint32_t kk = wbase + c * 64u + lane;
+                    const uint32_t j = before + (uint32_t)__popcll(m & ((2ull << lane) - 1ull)) - 1u;
+                    if (kk < total) {
+                        const TriLds &mt = ws[j];
+                        int px, py;
+                        item_pixel(mt, kk - wo[j], px, py);
+                        float z;
+                        if (fragment(mt, (float)px, (float)py, z))
+                            atomicMin(keys + (size_t)py * fp.width + px, (unsigned long long)make_key(z, mt.id));
+                    }
+                    before += (uint32_t)__popcll(m);
+                }
             }
         }
     }
@@ -1051,7 +1075,8 @@ __device__ __forceinline__ void sample(const DevTexture &t, float u, float v, in
 // The barycentrics, 1/w and the texture coordinates are computed exactly as
 // in the reference (so the same texels are fetched); the lighting terms use
 // the hardware reciprocal / rsqrt / exp2 / log2, which keeps RGBA8 within
-// 1/255 of the reference formula.  64x4-pixel blocks.
+// 1/255 of the reference formula.  64x4-pixel blocks (rows of 64 per wave:
+// fully coalesced vis reads and RGBA8 writes; 8x8 squares measured the same).
 __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
                                                const uint16_t *__restrict__ table)
 {
