@@ -143,7 +143,7 @@ def _main():
     import scenes as S
 
     ap = argparse.ArgumentParser(description="tune binning parameters on the GPU and write the table")
-    ap.add_argument("--workloads", default="terrain4k,batch256")
+    ap.add_argument("--workloads", default="terrain4k,batch256,flythrough")
     ap.add_argument("--out", default=TABLE_PATH)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="a reduced candidate set")
@@ -168,11 +168,22 @@ def _main():
             keep.append(r)
             by_bucket.setdefault(bucket_of(job.pixels, job.scene.num_triangles), []).append(
                 (r, job.camera, job.opts, R.BIN_EXACT, 0))
+    if "flythrough" in names:
+        # the main view and the six cube faces of the combined workload (views of one scene)
+        fly = S.config5(frames=1)
+        owner = R.Renderer(*fly.main)
+        owner.add_scene(fly.scene)
+        keep.append(owner)
+        views = [owner] + [owner.view(fly.face, fly.face) for _ in range(6)]
+        keep[len(keep):len(keep)] = views[1:]
+        for v, (cam, W, H) in zip(views, fly.views(0)):
+            by_bucket.setdefault(bucket_of(W * H, fly.scene.num_triangles), []).append(
+                (v, cam, fly.opts, R.BIN_EXACT, 0))
     t0 = time.time()
     table = tune(R, by_bucket, cands, args.reps, log)
     table["candidates"] = len(cands)
     table["seconds"] = round(time.time() - t0, 1)
-    for r in keep:
+    for r in reversed(keep):  # views before the renderer whose meshes they draw
         r.close()
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1)

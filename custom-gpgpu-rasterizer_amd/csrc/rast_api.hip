@@ -471,22 +471,26 @@ extern "C" int rast_mesh_triangle_base(const rast_mesh *m, uint64_t *out)
 
 // ---------------------------------------------------------------- textures
 
-extern "C" int rast_texture_create(rast_renderer *r, const uint32_t *rgba8, uint32_t w, uint32_t h,
-                                   rast_texture **out)
+extern "C" int rast_texture_create_pitched(rast_renderer *r, const void *rgba8, uint32_t w, uint32_t h,
+                                           size_t pitch_bytes, rast_texture **out)
 {
     if (!out) return rast_set_error(RAST_E_INVALID_ARG, "rast_texture_create: null output");
     *out = nullptr;
     if (!r || !rgba8 || !w || !h || w > 32768 || h > 32768)
         return rast_set_error(RAST_E_INVALID_ARG, "rast_texture_create: bad texture %ux%u", w, h);
+    if (pitch_bytes < (size_t)w * 4 || pitch_bytes % 4 != 0)
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_texture_create: pitch %zu below 4 x width or not a multiple of 4",
+                              pitch_bytes);
     HIP_TRY(hipSetDevice(r->device));
     rast_texture *t = new (std::nothrow) rast_texture();
     if (!t) return rast_set_error(RAST_E_OUT_OF_MEMORY, "host allocation failed");
     t->r = r;
     t->w = w;
     t->h = h;
-    const size_t bytes = (size_t)w * h * 4;
-    hipError_t e = hipMalloc((void **)&t->texels, bytes);
-    if (e == hipSuccess) e = hipMemcpy(t->texels, rgba8, bytes, hipMemcpyHostToDevice);
+    const size_t row = (size_t)w * 4;
+    // stored packed (pitch = width): the sampler addresses texel x + y * w
+    hipError_t e = hipMalloc((void **)&t->texels, row * h);
+    if (e == hipSuccess) e = hipMemcpy2D(t->texels, row, rgba8, pitch_bytes, row, h, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (t->texels) (void)hipFree(t->texels);
         delete t;
@@ -496,6 +500,12 @@ extern "C" int rast_texture_create(rast_renderer *r, const uint32_t *rgba8, uint
     r->textures.push_back(t);
     *out = t;
     return RAST_OK;
+}
+
+extern "C" int rast_texture_create(rast_renderer *r, const uint32_t *rgba8, uint32_t w, uint32_t h,
+                                   rast_texture **out)
+{
+    return rast_texture_create_pitched(r, rgba8, w, h, (size_t)w * 4, out);
 }
 
 extern "C" int rast_texture_destroy(rast_texture *t)

@@ -718,38 +718,70 @@ __global__ __launch_bounds__(256) void k_fallback_fill(const JobDesc *__restrict
     }
 }
 
-// Conservative depth plane of t over the tile at (X0, Y0): every fragment
-// depth the exact test can compute for t at pixel p satisfies
-//   z(p) >= c.x + c.y * (px - X0) + c.z * (py - Y0),
-// so t loses at p (its key is larger) whenever that bound exceeds the
-// pixel's current depth.  The exact depth of any pixel is the plane through
-// the three screen-space vertices; the computed one differs from it by the
-// rounding of the edge functions, the area, the divisions and the weighted
-// sum.  For a well-conditioned triangle (area >= 0.1 * longest edge^2, longest
-// edge >= 16 px) that difference is below 275 ulp(1) * zmax; the margin takes
-// 512 ulp(1) * zmax plus the float evaluation of the plane over the tile.
-// The plane is formed in double.  Returns c.x = -1e30 (no rejection) when t
-// does not qualify.
+// Conservative depth bounds of t over the tile at (X0, Y0).  The exact depth
+// at pixel p is the plane P through the three screen-space vertices; the
+// depth the exact fragment test computes differs from P(p) by the rounding of
+// the edge functions, the area, the three divisions and the weighted sum:
+//   |z(p) - P(p)| <= u * Zmax * (18.06 * R * E / A + 6.08 * E^2 / A + 6.06)
+// (u = 2^-24; R = largest coordinate distance between a vertex and a tile
+// pixel, E = largest edge-vector component, A = |area|, Zmax = largest |z_i|;
+// first order, valid while the bracket times u stays below 1e-3).  M is
+// twice that plus the float evaluation of the plane over the tile.  Returns
+//   x = P(X0, Y0) - M, y = dP/dx, z = dP/dy, w = 2 M,
+// so that lo(p) = x + y (px - X0) + z (py - Y0) <= z(p) <= lo(p) + w:
+// the fragment loses when lo(p) > the pixel's depth and is out of range when
+// lo(p) + w < 0.  x = +3e38 when no pixel of the tile can produce a
+// fragment (outside an edge at every tile pixel, or depth out of [0, 1]);
+// x = -1e30, w = 1e30 (no rejection) when the bound does not apply.
 __device__ __forceinline__ float4 cull_plane(const TriLds &t, int X0, int Y0, int T)
 {
-    const float4 none = make_float4(-1.0e30f, 0.f, 0.f, 0.f);
-    const float zlo = fminf(t.z0, fminf(t.z1, t.z2)), zhi = fmaxf(t.z0, fmaxf(t.z1, t.z2));
-    if (!(zlo > 0.f) || !(zhi < 2.f)) return none;
-    const float ea = t.ax * t.ax + t.ay * t.ay, eb = t.bx * t.bx + t.by * t.by, ec = t.cx * t.cx + t.cy * t.cy;
-    const float emax2 = fmaxf(ea, fmaxf(eb, ec));
-    if (!(t.area >= 0.1f * emax2) || !(emax2 >= 256.f) || !(emax2 < 1.0e12f)) return none;
-    const double e1x = (double)t.sx1 - (double)t.sx0, e1y = (double)t.sy1 - (double)t.sy0;
-    const double e1z = (double)t.z1 - (double)t.z0;
-    const double e2x = (double)t.sx2 - (double)t.sx0, e2y = (double)t.sy2 - (double)t.sy0;
-    const double e2z = (double)t.z2 - (double)t.z0;
+    const float4 none = make_float4(-1.0e30f, 0.f, 0.f, 1.0e30f);
+    const double u = 1.0 / 16777216.0;
+    const double sx[3] = { t.sx0, t.sx1, t.sx2 }, sy[3] = { t.sy0, t.sy1, t.sy2 }, sz[3] = { t.z0, t.z1, t.z2 };
+    double R = 0.0, zmax = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        R = fmax(R, fmax(fmax(fabs(sx[i] - X0), fabs(sx[i] - (X0 + T))), fmax(fabs(sy[i] - Y0), fabs(sy[i] - (Y0 + T)))));
+        zmax = fmax(zmax, fabs(sz[i]));
+    }
+    const double E = fmax(fmax(fmax(fabs((double)t.ax), fabs((double)t.ay)), fmax(fabs((double)t.bx), fabs((double)t.by))),
+                          fmax(fabs((double)t.cx), fabs((double)t.cy)));
+    const double A = (double)t.area;
+    if (!(R < 1.0e9)) return none;
+    // Coverage: the computed edge function at pixel p differs from the exact
+    // one, W(p) = (px - xa) (yb - ya) - (py - ya) (xb - xa), by at most
+    // 6.02 u R E + u |w|; W is affine, so when it is below -(7 u R E) at all
+    // four tile corners for one edge, no pixel of the tile passes that edge.
+    {
+        const double lim = -7.0 * u * R * E;
+        const double cxs[2] = { (double)X0, (double)(X0 + T - 1) }, cys[2] = { (double)Y0, (double)(Y0 + T - 1) };
+        for (int e = 0; e < 3; ++e) {
+            const int a = e, b = e == 2 ? 0 : e + 1;
+            const double ex = sx[b] - sx[a], ey = sy[b] - sy[a];
+            bool outside = true;
+            for (int c = 0; c < 4 && outside; ++c) {
+                const double w = (cxs[c & 1] - sx[a]) * ey - (cys[c >> 1] - sy[a]) * ex;
+                outside = w < lim * (1.0 + 1.0e-12) - 1.0e-300;
+            }
+            if (outside) return make_float4(3.0e38f, 0.f, 0.f, 0.f);
+        }
+    }
+    if (!(A > 0.0) || !(zmax < 1.0e6)) return none;
+    const double k = 18.06 * R * E / A + 6.08 * E * E / A + 6.06;
+    if (!(k * u < 1.0e-3)) return none;
+    const double e1x = sx[1] - sx[0], e1y = sy[1] - sy[0], e1z = sz[1] - sz[0];
+    const double e2x = sx[2] - sx[0], e2y = sy[2] - sy[0], e2z = sz[2] - sz[0];
     const double nz = e1x * e2y - e1y * e2x;
     if (!(fabs(nz) > 0.0)) return none;
     const double dzdx = -(e1y * e2z - e1z * e2y) / nz;
     const double dzdy = -(e1z * e2x - e1x * e2z) / nz;
-    const double zt = (double)t.z0 + dzdx * ((double)X0 - (double)t.sx0) + dzdy * ((double)Y0 - (double)t.sy0);
-    const double u = 1.0 / 16777216.0;  // 2^-24
-    const double margin = 512.0 * u * (double)zhi + 8.0 * u * (fabs(zt) + (fabs(dzdx) + fabs(dzdy)) * (double)T);
-    return make_float4((float)(zt - margin), (float)dzdx, (float)dzdy, 0.f);
+    const double zt = sz[0] + dzdx * ((double)X0 - sx[0]) + dzdy * ((double)Y0 - sy[0]);
+    const double span = (fabs(dzdx) + fabs(dzdy)) * (double)T;
+    const double M = 2.0 * u * zmax * k + 8.0 * u * (fabs(zt) + span) + 1.0e-12 * (fabs(zt) + (fabs(dzdx) + fabs(dzdy)) * R);
+    if (!(M < 0.25)) return none;
+    // whole-tile range test at the tile's corners (P is affine)
+    const double zlo_tile = zt - span - M, zhi_tile = zt + span + M;
+    if (zlo_tile > 1.0 || zhi_tile < 0.0) return make_float4(3.0e38f, 0.f, 0.f, 0.f);
+    return make_float4((float)(zt - M), (float)dzdx, (float)dzdy, (float)(2.0 * M + 8.0 * u * (fabs(zt) + span)));
 }
 
 // RA_PerTileKernel (CCR/CUDARenderer.cu:1039-1129) redesigned: one workgroup
@@ -954,7 +986,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
                 bplane[tid] = cull_plane(t, X0, Y0, T);
             } else {
                 t.xr = 0; t.yr = 0;  // empty range
-                bplane[tid] = make_float4(-1.0e30f, 0.f, 0.f, 0.f);
+                bplane[tid] = make_float4(-1.0e30f, 0.f, 0.f, 1.0e30f);
             }
             stage[tid] = t;
         }
@@ -966,13 +998,15 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             const int x1 = (int)(st.yr & 0xFFFFu), y1 = (int)(st.yr >> 16);
             if (x1 <= X0 || x0 >= X0 + T || y1 <= Y0 || y0 >= Y0 + T) continue;
             const float4 pl = bplane[j];
+            if (pl.x > 2.0f) continue;  // no pixel of this tile can produce a fragment
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 const int px = pxs[q], py = pys[q];
-                // lower bound of the triangle's depth here; above the current depth -> it loses
+                // lower bound of the triangle's depth here: above the current
+                // depth it loses, and with the band width below 0 it is out of range
                 const float zc = pl.x + pl.y * (float)(px - X0) + pl.z * (float)(py - Y0);
                 if (px >= x0 && px < x1 && py >= y0 && py < y1 &&
-                    !(zc > __uint_as_float((uint32_t)(best[q] >> 32)))) {
+                    !(zc > __uint_as_float((uint32_t)(best[q] >> 32))) && !(zc + pl.w < 0.f)) {
                     float z;
                     if (fragment(st, (float)px, (float)py, z)) {
                         const uint64_t kk = make_key(z, st.id);

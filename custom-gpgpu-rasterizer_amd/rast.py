@@ -34,7 +34,7 @@ EXPORTS = [
     "rast_debug_read", "rast_binmodel_create", "rast_binmodel_destroy", "rast_binmodel_num_features",
     "rast_renderer_set_binmodel", "rast_binmodel_record", "rast_binmodel_clear_data", "rast_binmodel_train",
     "rast_binmodel_get_params", "rast_binmodel_set_params", "rast_binmodel_num_params", "rast_binmodel_forward",
-    "rast_binmodel_loss_grad", "rast_render_batch", "rast_renderer_create_view",
+    "rast_binmodel_loss_grad", "rast_render_batch", "rast_renderer_create_view", "rast_texture_create_pitched",
 ]
 
 
@@ -105,6 +105,7 @@ def lib():
         L.rast_mesh_triangle_base.argtypes = [vp, C.POINTER(C.c_uint64)]
         L.rast_texture_create.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]
         L.rast_texture_destroy.argtypes = [vp]
+        L.rast_texture_create_pitched.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_size_t, C.POINTER(vp)]
         L.rast_camera_frame.argtypes = [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.POINTER(FrameConsts)]
         L.rast_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(FrameOpts)]
         L.rast_sync.argtypes = [vp]

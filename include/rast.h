@@ -201,6 +201,11 @@ int rast_mesh_triangle_base(const rast_mesh *m, uint64_t *out);
  * memory (row-major, pitch = width) instead of an image path. */
 int rast_texture_create(rast_renderer *r, const uint32_t *rgba8, uint32_t width, uint32_t height,
                         rast_texture **out);
+/* The same from rows pitch_bytes apart (an image surface's pitch, as
+ * CCR/CUDATexture.cpp:36-45 copies with cudaMemcpy2D); RGBA8 only.  The
+ * texels are stored packed. */
+int rast_texture_create_pitched(rast_renderer *r, const void *rgba8, uint32_t width, uint32_t height,
+                                size_t pitch_bytes, rast_texture **out);
 /* CUDATexture::Destroy (CCR/CUDATexture.cpp:74-82).  Meshes using it fall
  * back to "absent" for that slot. */
 int rast_texture_destroy(rast_texture *t);

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"^\s*(?:int|double|float|void)\s+\**(r(?:ast|sc)_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|double|float|void)\s+\**(r(?:ast|sc|io)_\w+)\s*\(", text, re.M)))
 
 
 def exported(lib_path):
@@ -38,6 +38,14 @@ def test_librast_exports_every_declared_entry_point():
 def test_scene_library_exports_every_declared_entry_point():
     names = declared("rast_scene.h")
     assert not sorted(set(names) - exported(S.SCENES_LIB))
+
+
+def test_io_library_exports_every_declared_entry_point():
+    import assets as A
+
+    names = declared("rast_io.h")
+    assert len(names) >= 5
+    assert not sorted(set(names) - exported(A.IO_LIB))
 
 
 def test_library_loads_and_binds_without_a_gpu():
