@@ -180,7 +180,8 @@ struct TriLds {
     float z0, z1, z2, area;
     uint32_t xr, yr;  // x0 | x1 << 16, y0 | y1 << 16 (absolute pixels)
     uint32_t id;
-    float inv_w;      // 1 / (x1 - x0), for item -> pixel decoding
+    float inv;        // item triangles: 1 / (x1 - x0), for item -> pixel decoding;
+                      // per-pixel (whole-tile and screen-list) triangles: 1 / area
 };
 
 // Item q of a triangle's pixel range [x0,x1) x [y0,y1), row-major.
@@ -188,7 +189,7 @@ __device__ __forceinline__ void item_pixel(const TriLds &t, uint32_t q, int &px,
 {
     const int x0 = (int)(t.xr & 0xFFFFu), x1 = (int)(t.xr >> 16), y0 = (int)(t.yr & 0xFFFFu);
     const int w = x1 - x0;
-    int r = (int)((float)q * t.inv_w);  // may be off by one; corrected below
+    int r = (int)((float)q * t.inv);  // may be off by one; corrected below
     int c = (int)q - r * w;
     if (c < 0) { --r; c += w; }
     if (c >= w) { ++r; c -= w; }
@@ -208,20 +209,54 @@ __device__ __forceinline__ uint32_t find_item_owner(const uint32_t *off, uint32_
     return lo;
 }
 
-// Fragment test of RasterizePixelAtomic (CCR/CUDAROPs.cu:329-345) with the D2
-// window: edge functions (CCR/CUDAROPs.cu:120-145), weights /= |area|,
-// z = w0 z0 + w1 z1 + w2 z2, accept 0 <= z <= 1.
-__device__ __forceinline__ bool fragment(const TriLds &t, float px, float py, float &z)
+// The three edge functions of CCR/CUDAROPs.cu:120-145 at pixel (px, py).
+__device__ __forceinline__ void edges(const TriLds &t, float px, float py, float &w0, float &w1, float &w2)
 {
-    const float w2 = (px - t.sx0) * t.ay - (py - t.sy0) * t.ax;
-    const float w0 = (px - t.sx1) * t.by - (py - t.sy1) * t.bx;
-    const float w1 = (px - t.sx2) * t.cy - (py - t.sy2) * t.cx;
-    if (!(w0 >= 0.f && w1 >= 0.f && w2 >= 0.f)) return false;
+    w2 = (px - t.sx0) * t.ay - (py - t.sy0) * t.ax;
+    w0 = (px - t.sx1) * t.by - (py - t.sy1) * t.bx;
+    w1 = (px - t.sx2) * t.cy - (py - t.sy2) * t.cx;
+}
+
+// Depth of RasterizePixelAtomic (CCR/CUDAROPs.cu:329-345) with the D2 window:
+// weights /= |area|, z = w0 z0 + w1 z1 + w2 z2, accept 0 <= z <= 1.
+__device__ __forceinline__ bool depth_of(const TriLds &t, float w0, float w1, float w2, float &z)
+{
     const float a0 = w0 / t.area, a1 = w1 / t.area, a2 = w2 / t.area;
     const float zi = (a0 * t.z0) + (a1 * t.z1) + (a2 * t.z2);
     if (!(zi >= 0.f && zi <= 1.f)) return false;
     z = zi;
     return true;
+}
+
+// Fragment test: all three edge functions >= 0, then the depth.
+__device__ __forceinline__ bool fragment(const TriLds &t, float px, float py, float &z)
+{
+    float w0, w1, w2;
+    edges(t, px, py, w0, w1, w2);
+    if (!(w0 >= 0.f && w1 >= 0.f && w2 >= 0.f)) return false;
+    return depth_of(t, w0, w1, w2, z);
+}
+
+// fragment() at a pixel whose current depth is dbest (<= 1), with a reject
+// that needs no division first.  With r = fl(1 / area) (t.inv) and
+// u = 2^-24, b_i = fl(w_i r) is within 3.0001 u of a_i = fl(w_i / area), and
+// both three-term sums are within 3.0001 u S of their exact values
+// (S = sum |a_i z_i|), so z' = fl(fl(b0 z0 + b1 z1) + b2 z2) is within
+// 9.0003 u S of the depth that depth_of() computes; S <= S' (1 + 6.01 u) for
+// the evaluated S' = sum |fl(b_i z_i)|.  m = 10 u S' + 1e-37 therefore covers
+// the difference (the constant covers gradual underflow): where
+// z' - m > dbest the fragment loses (or exceeds 1), where z' + m < 0 it is out
+// of range.  A NaN or infinite z' or m rejects nothing.
+__device__ __forceinline__ bool fragment_closer(const TriLds &t, float px, float py, float dbest, float &z)
+{
+    float w0, w1, w2;
+    edges(t, px, py, w0, w1, w2);
+    if (!(w0 >= 0.f && w1 >= 0.f && w2 >= 0.f)) return false;
+    const float p0 = (w0 * t.inv) * t.z0, p1 = (w1 * t.inv) * t.z1, p2 = (w2 * t.inv) * t.z2;
+    const float zq = (p0 + p1) + p2;
+    const float m = ((fabsf(p0) + fabsf(p1)) + fabsf(p2)) * (10.0f / 16777216.0f) + 1.0e-37f;
+    if (zq - m > dbest || zq + m < 0.f) return false;
+    return depth_of(t, w0, w1, w2, z);
 }
 
 __device__ __forceinline__ void set_edges(TriLds &t)
@@ -244,7 +279,7 @@ __device__ __forceinline__ void load_tri(const uint32_t *__restrict__ rec, uint3
     t.yr = q2.w;  // bbox hi (xmax | ymax << 16)
     set_edges(t);
     t.id = id;
-    t.inv_w = 0.f;
+    t.inv = 0.f;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -475,7 +510,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                 t.xr = (uint32_t)xmin | ((uint32_t)xmax << 16);
                 t.yr = (uint32_t)ymin | ((uint32_t)ymax << 16);
                 t.id = gid;
-                t.inv_w = 1.0f / (float)(xmax - xmin);
+                t.inv = 1.0f / (float)(xmax - xmin);
                 ws[rank] = t;
                 wo[rank] = start;
             }
@@ -718,70 +753,149 @@ __global__ __launch_bounds__(256) void k_fallback_fill(const JobDesc *__restrict
     }
 }
 
-// Conservative depth bounds of t over the tile at (X0, Y0).  The exact depth
-// at pixel p is the plane P through the three screen-space vertices; the
-// depth the exact fragment test computes differs from P(p) by the rounding of
-// the edge functions, the area, the three divisions and the weighted sum:
-//   |z(p) - P(p)| <= u * Zmax * (18.06 * R * E / A + 6.08 * E^2 / A + 6.06)
-// (u = 2^-24; R = largest coordinate distance between a vertex and a tile
-// pixel, E = largest edge-vector component, A = |area|, Zmax = largest |z_i|;
-// first order, valid while the bracket times u stays below 1e-3).  M is
-// twice that plus the float evaluation of the plane over the tile.  Returns
-//   x = P(X0, Y0) - M, y = dP/dx, z = dP/dy, w = 2 M,
-// so that lo(p) = x + y (px - X0) + z (py - Y0) <= z(p) <= lo(p) + w:
-// the fragment loses when lo(p) > the pixel's depth and is out of range when
-// lo(p) + w < 0.  x = +3e38 when no pixel of the tile can produce a
-// fragment (outside an edge at every tile pixel, or depth out of [0, 1]);
-// x = -1e30, w = 1e30 (no rejection) when the bound does not apply.
-__device__ __forceinline__ float4 cull_plane(const TriLds &t, int X0, int Y0, int T)
+// Bounds of a screen-list triangle t over the tile at (X0, Y0), used to skip
+// the tile or single pixels before the exact fragment test.  tmax is the
+// largest depth the tile's pixels hold; depths only decrease from there.
+//
+// Tile tests in float, exact.  Every operation of edges() and depth_of() is
+// a correctly rounded IEEE operation and rounding is monotone, so a computed
+// edge value is monotone in px (py fixed) and in py (px fixed): its extremes
+// over the tile's pixels are among the four corner pixels.  An edge negative
+// at all four corners fails at every pixel.  The computed depth is monotone
+// in each a_i = fl(w_i / area), which lies in [fl(max(lo_i, 0) / area),
+// fl(hi_i / area)] at pixels passing all edges; depth_of's sum evaluated
+// with each a_i at the matching end bounds the depth of every fragment of the
+// tile.  These need finite values below 1e18 in magnitude (no overflow, so
+// no NaN); a NaN bound rejects nothing.
+//
+// Per-pixel plane in double.  For edge j (vertex (ox, oy), float edge vector
+// (ex, ey)) the computed edge value at pixel p is within
+//   eps_j = 3.0001 u (Dx |ey| + Dy |ex|)
+// of W_j(p) = (px - ox) ey - (py - oy) ex (u = 2^-24; Dx, Dy the largest
+// |px - ox|, |py - oy| over the tile's pixels): two roundings in each
+// product term, one in the difference.  With a_j = fl(w_j / area) and the
+// rounded three-term sum, the computed depth is within
+//   B = sum_j |z_j| (eps_j + 4.0001 u (|W_j|max + eps_j)) / area
+// of the affine L(p) = sum_j z_j W_j(p) / area, which is evaluated in double
+// at the tile corner.  M adds the float evaluation of the returned plane and
+// the double rounding of L; the plane is used while M < 0.25.
+//
+// Returns (x, y, z, w) with lo(p) = x + y (px - X0) + z (py - Y0) <= z(p) <=
+// lo(p) + w for every fragment of the tile: the plane (x = L(X0, Y0) - M,
+// y = dL/dx, z = dL/dy, w = 2 M) where it applies, else the constant float
+// lower bound (w = 1e30).  A fragment loses where lo(p) > the pixel's depth
+// and is out of range where lo(p) + w < 0.  x = +3e38: no pixel of the tile
+// can take a fragment of t (outside an edge, depth out of [0, 1], or behind
+// every pixel).
+__device__ __forceinline__ float4 tile_bound(const TriLds &t, int X0, int Y0, int T, float tmax)
 {
+    const float4 skip = make_float4(3.0e38f, 0.f, 0.f, 0.f);
     const float4 none = make_float4(-1.0e30f, 0.f, 0.f, 1.0e30f);
-    const double u = 1.0 / 16777216.0;
-    const double sx[3] = { t.sx0, t.sx1, t.sx2 }, sy[3] = { t.sy0, t.sy1, t.sy2 }, sz[3] = { t.z0, t.z1, t.z2 };
-    double R = 0.0, zmax = 0.0;
-    for (int i = 0; i < 3; ++i) {
-        R = fmax(R, fmax(fmax(fabs(sx[i] - X0), fabs(sx[i] - (X0 + T))), fmax(fabs(sy[i] - Y0), fabs(sy[i] - (Y0 + T)))));
-        zmax = fmax(zmax, fabs(sz[i]));
+    const float vmax = fmaxf(fmaxf(fmaxf(fmaxf(fabsf(t.sx0), fabsf(t.sy0)), fmaxf(fabsf(t.sx1), fabsf(t.sy1))),
+                                   fmaxf(fmaxf(fabsf(t.sx2), fabsf(t.sy2)), fmaxf(fabsf(t.ax), fabsf(t.ay)))),
+                             fmaxf(fmaxf(fabsf(t.bx), fabsf(t.by)), fmaxf(fabsf(t.cx), fabsf(t.cy))));
+    if (!(vmax < 1.0e18f)) return none;
+    // exact edge extremes over the tile (float, corners)
+    float lo0 = 3.0e38f, lo1 = 3.0e38f, lo2 = 3.0e38f, hi0 = -3.0e38f, hi1 = -3.0e38f, hi2 = -3.0e38f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float w0, w1, w2;
+        edges(t, (float)(X0 + (c & 1) * (T - 1)), (float)(Y0 + (c >> 1) * (T - 1)), w0, w1, w2);
+        lo0 = fminf(lo0, w0); hi0 = fmaxf(hi0, w0);
+        lo1 = fminf(lo1, w1); hi1 = fmaxf(hi1, w1);
+        lo2 = fminf(lo2, w2); hi2 = fmaxf(hi2, w2);
     }
-    const double E = fmax(fmax(fmax(fabs((double)t.ax), fabs((double)t.ay)), fmax(fabs((double)t.bx), fabs((double)t.by))),
-                          fmax(fabs((double)t.cx), fabs((double)t.cy)));
+    if (hi0 < 0.f || hi1 < 0.f || hi2 < 0.f) return skip;
+    // the plane (double)
     const double A = (double)t.area;
-    if (!(R < 1.0e9)) return none;
-    // Coverage: the computed edge function at pixel p differs from the exact
-    // one, W(p) = (px - xa) (yb - ya) - (py - ya) (xb - xa), by at most
-    // 6.02 u R E + u |w|; W is affine, so when it is below -(7 u R E) at all
-    // four tile corners for one edge, no pixel of the tile passes that edge.
-    {
-        const double lim = -7.0 * u * R * E;
-        const double cxs[2] = { (double)X0, (double)(X0 + T - 1) }, cys[2] = { (double)Y0, (double)(Y0 + T - 1) };
-        for (int e = 0; e < 3; ++e) {
-            const int a = e, b = e == 2 ? 0 : e + 1;
-            const double ex = sx[b] - sx[a], ey = sy[b] - sy[a];
-            bool outside = true;
-            for (int c = 0; c < 4 && outside; ++c) {
-                const double w = (cxs[c & 1] - sx[a]) * ey - (cys[c >> 1] - sy[a]) * ex;
-                outside = w < lim * (1.0 + 1.0e-12) - 1.0e-300;
-            }
-            if (outside) return make_float4(3.0e38f, 0.f, 0.f, 0.f);
+    const double zmax = fmax(fmax(fabs((double)t.z0), fabs((double)t.z1)), fabs((double)t.z2));
+    if (A > 0.0 && zmax < 1.0e6) {
+        // edge j runs from vertex j along (ex, ey); its function weights depth zz[j]
+        const double u = 1.0 / 16777216.0;
+        const double ox[3] = { t.sx0, t.sx1, t.sx2 }, oy[3] = { t.sy0, t.sy1, t.sy2 };
+        const double ex[3] = { t.ax, t.bx, t.cx }, ey[3] = { t.ay, t.by, t.cy };
+        const double zz[3] = { t.z2, t.z0, t.z1 };
+        const double xa = (double)X0, xb = (double)(X0 + T - 1), ya = (double)Y0, yb = (double)(Y0 + T - 1);
+        double L0 = 0.0, Lx = 0.0, Ly = 0.0, B = 0.0, Lmag = 0.0;
+        for (int j = 0; j < 3; ++j) {
+            const double dxa = xa - ox[j], dxb = xb - ox[j], dya = ya - oy[j], dyb = yb - oy[j];
+            const double Dx = fmax(fabs(dxa), fabs(dxb)), Dy = fmax(fabs(dya), fabs(dyb));
+            const double w00 = dxa * ey[j] - dya * ex[j], w10 = dxb * ey[j] - dya * ex[j];
+            const double w01 = dxa * ey[j] - dyb * ex[j], w11 = dxb * ey[j] - dyb * ex[j];
+            const double Wabs = fmax(fmax(fabs(w00), fabs(w10)), fmax(fabs(w01), fabs(w11)));
+            const double eps = 3.0001 * u * (Dx * fabs(ey[j]) + Dy * fabs(ex[j]));
+            B += fabs(zz[j]) * (eps + 4.0001 * u * (Wabs + eps));
+            Lmag += fabs(zz[j]) * Wabs;
+            L0 += zz[j] * w00;
+            Lx += zz[j] * ey[j];
+            Ly -= zz[j] * ex[j];
+        }
+        const double rA = 1.0 / A;
+        B *= rA; Lmag *= rA; L0 *= rA; Lx *= rA; Ly *= rA;
+        const double span = (fabs(Lx) + fabs(Ly)) * (double)(T - 1);
+        const double M = 1.01 * B + 8.0 * u * (fabs(L0) + span + 1.0) + 1.0e-12 * Lmag + 1.0e-30;
+        if (M < 0.25) {
+            // whole-tile tests at the tile's corners (L is affine)
+            const double Lmin = L0 + fmin(Lx, 0.0) * (double)(T - 1) + fmin(Ly, 0.0) * (double)(T - 1);
+            const double Lmax = L0 + fmax(Lx, 0.0) * (double)(T - 1) + fmax(Ly, 0.0) * (double)(T - 1);
+            if (Lmin - M > 1.0 || Lmax + M < 0.0 || Lmin - M > (double)tmax) return skip;
+            return make_float4((float)(L0 - M), (float)Lx, (float)Ly, (float)(2.0 * M + 8.0 * u * (fabs(L0) + span + 1.0)));
         }
     }
-    if (!(A > 0.0) || !(zmax < 1.0e6)) return none;
-    const double k = 18.06 * R * E / A + 6.08 * E * E / A + 6.06;
-    if (!(k * u < 1.0e-3)) return none;
-    const double e1x = sx[1] - sx[0], e1y = sy[1] - sy[0], e1z = sz[1] - sz[0];
-    const double e2x = sx[2] - sx[0], e2y = sy[2] - sy[0], e2z = sz[2] - sz[0];
-    const double nz = e1x * e2y - e1y * e2x;
-    if (!(fabs(nz) > 0.0)) return none;
-    const double dzdx = -(e1y * e2z - e1z * e2y) / nz;
-    const double dzdy = -(e1z * e2x - e1x * e2z) / nz;
-    const double zt = sz[0] + dzdx * ((double)X0 - sx[0]) + dzdy * ((double)Y0 - sy[0]);
-    const double span = (fabs(dzdx) + fabs(dzdy)) * (double)T;
-    const double M = 2.0 * u * zmax * k + 8.0 * u * (fabs(zt) + span) + 1.0e-12 * (fabs(zt) + (fabs(dzdx) + fabs(dzdy)) * R);
-    if (!(M < 0.25)) return none;
-    // whole-tile range test at the tile's corners (P is affine)
-    const double zlo_tile = zt - span - M, zhi_tile = zt + span + M;
-    if (zlo_tile > 1.0 || zhi_tile < 0.0) return make_float4(3.0e38f, 0.f, 0.f, 0.f);
-    return make_float4((float)(zt - M), (float)dzdx, (float)dzdy, (float)(2.0 * M + 8.0 * u * (fabs(zt) + span)));
+    // no plane: the exact float depth range over the tile
+    const float l0 = fmaxf(lo0, 0.f) / t.area, l1 = fmaxf(lo1, 0.f) / t.area, l2 = fmaxf(lo2, 0.f) / t.area;
+    const float h0 = hi0 / t.area, h1 = hi1 / t.area, h2 = hi2 / t.area;
+    const float zlo = (((t.z0 >= 0.f ? l0 : h0) * t.z0) + ((t.z1 >= 0.f ? l1 : h1) * t.z1)) +
+                      ((t.z2 >= 0.f ? l2 : h2) * t.z2);
+    const float zhi = (((t.z0 >= 0.f ? h0 : l0) * t.z0) + ((t.z1 >= 0.f ? h1 : l1) * t.z1)) +
+                      ((t.z2 >= 0.f ? h2 : l2) * t.z2);
+    if (zlo > 1.f || zhi < 0.f || zlo > tmax) return skip;
+    return zlo == zlo ? make_float4(fmaxf(zlo, -1.0e30f), 0.f, 0.f, 1.0e30f) : none;
+}
+
+// tile_bound's x for "no pixel of the tile": a plane's x stays below 5e5
+// (its margin M < 0.25 bounds |x| by 0.25 / (8 u)), a constant bound below 1.
+constexpr float kSkipBound = 1.0e37f;
+
+// Float bits that order like the values (negative values reversed).
+__device__ __forceinline__ uint32_t ordered_bits(float v)
+{
+    const uint32_t b = __float_as_uint(v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// The largest depth a tile's pixels hold (block-wide; one barrier).
+template <int PPT, int NW>
+__device__ __forceinline__ float tile_max_depth(const uint64_t *best, uint32_t lane, uint32_t wave, float *wdmax)
+{
+    float d = 0.f;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) d = fmaxf(d, __uint_as_float((uint32_t)(best[q] >> 32)));
+    for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o));
+    __syncthreads();  // the previous values are read
+    if (lane == 0) wdmax[wave] = d;
+    __syncthreads();
+    float m = wdmax[0];
+    for (int w = 1; w < NW; ++w) m = fmaxf(m, wdmax[w]);
+    return m;
+}
+
+// Ascending bitonic sort of v[0, P) in LDS, P a power of two; ends with a barrier.
+template <int NT>
+__device__ __forceinline__ void bitonic_sort(uint64_t *v, uint32_t P, uint32_t tid)
+{
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < P; i += NT) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint64_t a = v[i], b = v[l];
+                    if ((a > b) == ((i & k) == 0)) { v[i] = b; v[l] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
 }
 
 // RA_PerTileKernel (CCR/CUDARenderer.cu:1039-1129) redesigned: one workgroup
@@ -824,9 +938,11 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     __shared__ uint64_t key[NPIX];
     __shared__ TriLds stage[NT];
     __shared__ float4 bplane[NT];  // screen-list triangles: conservative depth plane over the tile
+    __shared__ uint2 bbox[NT];     // and their pixel ranges
     __shared__ uint32_t off[NT];
     __shared__ uint32_t wlist[NT];
     __shared__ uint32_t wred_u[NW], wwhole[NW];
+    __shared__ float wdmax[NW];
 
     const uint32_t ntiles = fp.tiles_x * fp.tiles_y;
     // Tiles in launch order: the hardware spreads consecutive workgroups over
@@ -890,7 +1006,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             uint32_t area = 0;
             bool whole = false;
             TriLds t;
-            t.xr = 0; t.yr = 0; t.inv_w = 0.f;
+            t.xr = 0; t.yr = 0; t.inv = 0.f;
             if (i < n_tile) {
                 const uint32_t id = i < n_a ? list_a[i] : list_b[i - n_a];
                 if (id < num_tris) {
@@ -902,9 +1018,9 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
                     if (x0 < x1 && y0 < y1) {
                         t.xr = (uint32_t)x0 | ((uint32_t)x1 << 16);
                         t.yr = (uint32_t)y0 | ((uint32_t)y1 << 16);
-                        t.inv_w = 1.0f / (float)(x1 - x0);
                         area = (uint32_t)((x1 - x0) * (y1 - y0));
                         if (area >= small_limit) { whole = true; area = 0; }
+                        t.inv = whole ? 1.0f / t.area : 1.0f / (float)(x1 - x0);
                     }
                 }
             }
@@ -956,7 +1072,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
                         const int px = pxs[q], py = pys[q];
                         if (px >= x0 && px < x1 && py >= y0 && py < y1) {
                             float z;
-                            if (fragment(st, (float)px, (float)py, z)) {
+                            if (fragment_closer(st, (float)px, (float)py, __uint_as_float((uint32_t)(best[q] >> 32)), z)) {
                                 const uint64_t kk = make_key(z, st.id);
                                 best[q] = kk < best[q] ? kk : best[q];
                             }
@@ -972,50 +1088,136 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
         for (int q = 0; q < PPT; ++q) best[q] = key[(int)tid + q * NT];
     }
 
-    // screen-list triangles: staged in LDS with one parallel load per chunk,
-    // then every lane tests its own pixels
+    // Screen-list triangles.  Pass 1 bounds every one over the tile and keeps
+    // those that can still take a pixel, keyed by their depth lower bound
+    // (in the key array, free once the queue is resolved).  Sorted front to
+    // back, pass 2 then stages them chunk by chunk and stops where the next
+    // lower bound lies behind every pixel of the tile.  When more survive than
+    // fit, pass 2 takes the whole list in its stored order instead.  Each
+    // chunk is loaded and re-bounded in parallel, compacted into LDS, and
+    // every lane tests its own pixels against it.
     uint32_t n_big = ctr->num_big;
     n_big = n_big < big_cap ? n_big : big_cap;
-    for (uint32_t base = 0; base < n_big; base += NT) {
-        const uint32_t i = base + tid;
-        if (i < n_big) {
-            TriLds t;
-            const uint32_t id = big[i];
-            if (id < num_tris) {
-                load_tri(trirec, id, t);
-                bplane[tid] = cull_plane(t, X0, Y0, T);
-            } else {
-                t.xr = 0; t.yr = 0;  // empty range
-                bplane[tid] = make_float4(-1.0e30f, 0.f, 0.f, 1.0e30f);
+    if (n_big) {
+        float tmax = tile_max_depth<PPT, NW>(best, lane, wave, wdmax);
+        uint64_t *cand = key;  // NPIX entries: (ordered lower bound << 32) | list index
+        uint32_t n_cand = 0;
+        for (uint32_t base = 0; base < n_big; base += NT) {
+            const uint32_t i = base + tid;
+            bool keep = false;
+            float lb = 0.f;
+            if (i < n_big) {
+                const uint32_t id = big[i];
+                if (id < num_tris) {
+                    TriLds t;
+                    load_tri(trirec, id, t);
+                    const int x0 = (int)(t.xr & 0xFFFFu), y0 = (int)(t.xr >> 16);
+                    const int x1 = (int)(t.yr & 0xFFFFu), y1 = (int)(t.yr >> 16);
+                    if (x1 > X0 && x0 < X0 + T && y1 > Y0 && y0 < Y0 + T) {
+                        const float4 pl = tile_bound(t, X0, Y0, T, tmax);
+                        keep = !(pl.x > kSkipBound);
+                        lb = pl.x + fminf(pl.y, 0.f) * (float)(T - 1) + fminf(pl.z, 0.f) * (float)(T - 1);
+                    }
+                }
             }
-            stage[tid] = t;
+            const uint64_t km = __ballot(keep);
+            if (lane == 0) wred_u[wave] = (uint32_t)__popcll(km);
+            __syncthreads();
+            uint32_t before = 0, nk = 0;
+            for (int w = 0; w < NW; ++w) {
+                if (w < (int)wave) before += wred_u[w];
+                nk += wred_u[w];
+            }
+            const uint32_t slot = n_cand + before + popc_below(km);
+            if (keep && slot < (uint32_t)NPIX) cand[slot] = ((uint64_t)ordered_bits(lb) << 32) | i;
+            n_cand += nk;
+            __syncthreads();
         }
-        __syncthreads();
-        const uint32_t nb = n_big - base < (uint32_t)NT ? n_big - base : (uint32_t)NT;
-        for (uint32_t j = 0; j < nb; ++j) {
-            const TriLds &st = stage[j];
-            const int x0 = (int)(st.xr & 0xFFFFu), y0 = (int)(st.xr >> 16);
-            const int x1 = (int)(st.yr & 0xFFFFu), y1 = (int)(st.yr >> 16);
-            if (x1 <= X0 || x0 >= X0 + T || y1 <= Y0 || y0 >= Y0 + T) continue;
-            const float4 pl = bplane[j];
-            if (pl.x > 2.0f) continue;  // no pixel of this tile can produce a fragment
+        const bool sorted = n_cand <= (uint32_t)NPIX;
+        if (sorted) {
+            uint32_t P = 1;
+            while (P < n_cand) P <<= 1;
+            for (uint32_t q = n_cand + tid; q < P; q += NT) cand[q] = ~0ull;
+            __syncthreads();
+            bitonic_sort<NT>(cand, P, tid);
+        }
+        const uint32_t n_list = sorted ? n_cand : n_big;
+        for (uint32_t base = 0; base < n_list; base += NT) {
+            if (base) tmax = tile_max_depth<PPT, NW>(best, lane, wave, wdmax);
+            // sorted: every further triangle is bounded below by this one's bound
+            if (sorted && (uint32_t)(cand[base] >> 32) > ordered_bits(tmax)) break;
+            const uint32_t i = base + tid;
+            bool keep = false;
+            TriLds t;
+            float4 pl;
+            if (i < n_list) {
+                const uint32_t li = sorted ? (uint32_t)cand[i] : i;
+                const uint32_t id = big[li];
+                if (id < num_tris) {
+                    load_tri(trirec, id, t);
+                    const int x0 = (int)(t.xr & 0xFFFFu), y0 = (int)(t.xr >> 16);
+                    const int x1 = (int)(t.yr & 0xFFFFu), y1 = (int)(t.yr >> 16);
+                    if (x1 > X0 && x0 < X0 + T && y1 > Y0 && y0 < Y0 + T) {
+                        pl = tile_bound(t, X0, Y0, T, tmax);
+                        keep = !(pl.x > kSkipBound);
+                    }
+                }
+            }
+            const uint64_t km = __ballot(keep);
+            if (lane == 0) wred_u[wave] = (uint32_t)__popcll(km);
+            __syncthreads();
+            uint32_t before = 0, nb = 0;
+            for (int w = 0; w < NW; ++w) {
+                if (w < (int)wave) before += wred_u[w];
+                nb += wred_u[w];
+            }
+            if (keep) {
+                const uint32_t slot = before + popc_below(km);
+                t.inv = 1.0f / t.area;
+                stage[slot] = t;
+                bplane[slot] = pl;
+                bbox[slot] = make_uint2(t.xr, t.yr);
+            }
+            __syncthreads();
+            // the plane and pixel range of the next triangle are read ahead;
+            // the whole triangle only where some lane passes them
+            float4 pl_next = bplane[0];
+            uint2 bb_next = bbox[0];
+            for (uint32_t j = 0; j < nb; ++j) {
+                const float4 pl = pl_next;
+                const uint2 bb = bb_next;
+                if (j + 1 < nb) {
+                    pl_next = bplane[j + 1];
+                    bb_next = bbox[j + 1];
+                }
+                const int x0 = (int)(bb.x & 0xFFFFu), y0 = (int)(bb.x >> 16);
+                const int x1 = (int)(bb.y & 0xFFFFu), y1 = (int)(bb.y >> 16);
+                bool pass[PPT];
+                bool any = false;
 #pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int px = pxs[q], py = pys[q];
-                // lower bound of the triangle's depth here: above the current
-                // depth it loses, and with the band width below 0 it is out of range
-                const float zc = pl.x + pl.y * (float)(px - X0) + pl.z * (float)(py - Y0);
-                if (px >= x0 && px < x1 && py >= y0 && py < y1 &&
-                    !(zc > __uint_as_float((uint32_t)(best[q] >> 32))) && !(zc + pl.w < 0.f)) {
+                for (int q = 0; q < PPT; ++q) {
+                    const int px = pxs[q], py = pys[q];
+                    // lower bound of the triangle's depth here: above the current
+                    // depth it loses, and with the band width below 0 it is out of range
+                    const float zc = pl.x + pl.y * (float)(px - X0) + pl.z * (float)(py - Y0);
+                    const float db = __uint_as_float((uint32_t)(best[q] >> 32));
+                    pass[q] = px >= x0 && px < x1 && py >= y0 && py < y1 && !(zc > db) && !(zc + pl.w < 0.f);
+                    any = any || pass[q];
+                }
+                if (__ballot(any) == 0) continue;
+                const TriLds &st = stage[j];
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) {
                     float z;
-                    if (fragment(st, (float)px, (float)py, z)) {
+                    if (pass[q] && fragment_closer(st, (float)pxs[q], (float)pys[q],
+                                                   __uint_as_float((uint32_t)(best[q] >> 32)), z)) {
                         const uint64_t kk = make_key(z, st.id);
                         best[q] = kk < best[q] ? kk : best[q];
                     }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 
     // write out and the tile's min depth (a feature of the capacity model)
