@@ -709,7 +709,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         const bool single = r->last_single;
         uint32_t nb[ST_COUNT] = {};
         nb[ST_INIT] = ceil_div(ntiles, 256);
-        nb[ST_CAP] = single ? ceil_div(ntiles, 64) : 0;
+        nb[ST_CAP] = single ? ceil_div(ntiles, kCapacityTilesPerBlock) : 0;
         nb[ST_FILL] = single ? 0 : std::min<uint32_t>(ceil_div(ntris, 256), 1024);
         nb[ST_FBALLOC] = single ? ceil_div(ntiles, 256) : 0;
         nb[ST_FBFILL] = single ? std::min<uint32_t>(ceil_div(ntris, 256), 256) : 0;
@@ -1141,6 +1141,7 @@ extern "C" int rast_debug_read(rast_renderer *r, int which, void *dst, size_t by
     case RAST_DEBUG_TRI_RECORDS: size = (size_t)r->total_tris * kTriRecWords * 4; src = r->trirec.p; break;
     case RAST_DEBUG_TRI_ALIVE: size = r->total_tris; src = r->alive.p; break;
     case RAST_DEBUG_TILE_COUNTS: size = (size_t)ntiles * 4; src = r->tile_count[last].p; break;
+    case RAST_DEBUG_TILE_CAPS: size = (size_t)ntiles * 4; src = r->tile_cap.p; break;
     case RAST_DEBUG_BIG_LIST: {
         const uint32_t nb = std::min(r->h_counters->num_big, r->big_cap);
         tmp.resize(nb + 1);

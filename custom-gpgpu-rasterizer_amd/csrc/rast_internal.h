@@ -30,6 +30,8 @@ constexpr int kIVertexFloats = 14;
 constexpr int kTriRecWords = 16;
 constexpr int kMaxMeshes = 1 << 16;
 constexpr int kNumFeatures = 8;
+// Tiles per workgroup of the capacity-model kernel (16 per wave).
+constexpr int kCapacityTilesPerBlock = 32;
 
 struct DevTexture {
     const uint32_t *texels;

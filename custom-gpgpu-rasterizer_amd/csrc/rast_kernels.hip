@@ -180,8 +180,8 @@ struct TriLds {
     float z0, z1, z2, area;
     uint32_t xr, yr;  // x0 | x1 << 16, y0 | y1 << 16 (absolute pixels)
     uint32_t id;
-    float inv;        // item triangles: 1 / (x1 - x0), for item -> pixel decoding;
-                      // per-pixel (whole-tile and screen-list) triangles: 1 / area
+    float inv;        // queued triangles: 1 / (x1 - x0), for item -> pixel decoding;
+                      // screen-list triangles: 1 / area
 };
 
 // Item q of a triangle's pixel range [x0,x1) x [y0,y1), row-major.
@@ -1020,7 +1020,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
                         t.yr = (uint32_t)y0 | ((uint32_t)y1 << 16);
                         area = (uint32_t)((x1 - x0) * (y1 - y0));
                         if (area >= small_limit) { whole = true; area = 0; }
-                        t.inv = whole ? 1.0f / t.area : 1.0f / (float)(x1 - x0);
+                        t.inv = 1.0f / (float)(x1 - x0);
                     }
                 }
             }
@@ -1072,7 +1072,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
                         const int px = pxs[q], py = pys[q];
                         if (px >= x0 && px < x1 && py >= y0 && py < y1) {
                             float z;
-                            if (fragment_closer(st, (float)px, (float)py, __uint_as_float((uint32_t)(best[q] >> 32)), z)) {
+                            if (fragment(st, (float)px, (float)py, z)) {
                                 const uint64_t kk = make_key(z, st.id);
                                 best[q] = kk < best[q] ? kk : best[q];
                             }

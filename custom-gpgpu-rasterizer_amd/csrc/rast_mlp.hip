@@ -43,19 +43,29 @@ __device__ __forceinline__ void tile_features(const FrameParams &fp, uint32_t t,
                                               const uint32_t *__restrict__ density, float *o)
 {
     const int tx = (int)(t % fp.tiles_x), ty = (int)(t / fp.tiles_x);
+    // all loads first (clamped to the tile itself outside the grid), so they
+    // are in flight together; then the sums in neighbourhood order
+    uint32_t pc[9], dc[9];
+    bool ok[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int x = tx + k % 3 - 1, y = ty + k / 3 - 1;
+        ok[k] = x >= 0 && y >= 0 && x < (int)fp.tiles_x && y < (int)fp.tiles_y;
+        const uint32_t q = ok[k] ? (uint32_t)y * fp.tiles_x + (uint32_t)x : t;
+        pc[k] = prev[q];
+        dc[k] = density ? density[q] : 0u;
+    }
     float sum = 0.f, mx = 0.f, dsum = 0.f;
     int cnt = 0;
-    for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int x = tx + dx, y = ty + dy;
-            if (x < 0 || y < 0 || x >= (int)fp.tiles_x || y >= (int)fp.tiles_y) continue;
-            const uint32_t q = (uint32_t)y * fp.tiles_x + (uint32_t)x;
-            const float c = (float)prev[q];
-            sum += c;
-            mx = c > mx ? c : mx;
-            dsum += density ? (float)density[q] : 0.f;
-            ++cnt;
-        }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        if (!ok[k]) continue;
+        const float c = (float)pc[k];
+        sum += c;
+        mx = c > mx ? c : mx;
+        dsum += density ? (float)dc[k] : 0.f;
+        ++cnt;
+    }
     o[0] = log1pf((float)prev[t]);
     o[1] = log1pf(sum / (float)cnt);
     o[2] = log1pf(mx);
@@ -78,21 +88,28 @@ __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t
     for (int q = 0; q < F; ++q) out[(size_t)t * F + q] = f[q];
 }
 
-// Frame-time capacities of the single-pass jobs of a launch.  Learned: a
-// block serves 64 tiles (one per lane).  The parameters are staged in LDS
-// once per block and read as wave-uniform broadcasts; wave 0 gathers the
-// features, each of the 4 waves computes a quarter of both hidden layers,
-// and the output sums the four partial dot products in a fixed order.
-// Otherwise the previous frame's count: ceil(prev * margin) + slack.
+// Frame-time capacities of the single-pass jobs of a launch.  Learned: the
+// two hidden layers run on the matrix cores (v_mfma_f32_16x16x4f32, fp32 in
+// and out), each wave serving 16 tiles: A holds 16 tiles x 4 inputs (lane l:
+// tile l % 16, input l / 16), B 4 inputs x 16 units (lane l: input l / 16,
+// unit l % 16), and the 16 x 16 result sits in lane u + 16 (t / 4),
+// register t % 4 (tile t, unit u).  The parameters are staged in LDS once per
+// block; the first layer's activations pass through LDS as the second
+// layer's A operand; the output layer sums over units within each 16-lane
+// group.  Otherwise the previous frame's count: ceil(prev * margin) + slack.
+constexpr int kCapTiles = kCapacityTilesPerBlock;
+constexpr int kCapWaves = kCapTiles / 16;  // waves per block
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 template <int H>
-__global__ __launch_bounds__(256) void k_capacity(const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table)
+__global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__restrict__ jobs,
+                                                             const uint16_t *__restrict__ table)
 {
-    constexpr int Q = H / 4;
+    static_assert(H % 16 == 0 && F % 4 == 0, "unit and input counts must suit 16 x 16 x 4 tiles");
     constexpr int NP = H * F + H + H * H + H + H + 1;
     __shared__ __align__(16) float wts[(NP + 3) & ~3];
-    __shared__ float xs[F][64];
-    __shared__ float h1s[H][65];
-    __shared__ float part[4][64];
+    __shared__ float xs[kCapWaves][F][16];
+    __shared__ float h1s[kCapWaves][H][16];
     uint32_t local;
     const JobDesc &J = jobs[job_of(table, jobs, ST_CAP, local)];
     const FrameParams &fp = J.fp;
@@ -102,61 +119,86 @@ __global__ __launch_bounds__(256) void k_capacity(const JobDesc *__restrict__ jo
     uint32_t *__restrict__ cap = J.tile_cap;
     const uint32_t n = fp.tiles_x * fp.tiles_y;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t t = local * 64u + lane;
     if (!J.learned) {
-        if (w == 0 && t < n) {
+        const uint32_t t = local * kCapTiles + threadIdx.x;
+        if (threadIdx.x < (uint32_t)kCapTiles && t < n) {
             const float c = ceilf((float)prev[t] * margin);
             cap[t] = (c < 1.0e8f ? (uint32_t)c : 100000000u) + slack;
         }
         return;
     }
     const float *__restrict__ p = J.model;
-    for (int i = (int)threadIdx.x; i < NP; i += 256) wts[i] = p[i];
-    if (w == 0) {
+    {
+        // 16-byte loads for the bulk of the parameters (the buffer is 256-byte aligned)
+        constexpr int NP4 = NP / 4;
+        const float4 *p4 = reinterpret_cast<const float4 *>(p);
+        float4 *w4 = reinterpret_cast<float4 *>(wts);
+        for (int i = (int)threadIdx.x; i < NP4; i += 64 * kCapWaves) w4[i] = p4[i];
+        for (int i = NP4 * 4 + (int)threadIdx.x; i < NP; i += 64 * kCapWaves) wts[i] = p[i];
+    }
+    const uint32_t t0 = local * kCapTiles + w * 16u;  // this wave's first tile
+    if (lane < 16) {
         float x[F];
-        if (t < n) tile_features(fp, t, prev, J.prev_depth, J.density, x);
+        if (t0 + lane < n) tile_features(fp, t0 + lane, prev, J.prev_depth, J.density, x);
         else for (int f = 0; f < F; ++f) x[f] = 0.f;
 #pragma unroll
-        for (int f = 0; f < F; ++f) xs[f][lane] = x[f];
+        for (int f = 0; f < F; ++f) xs[w][f][lane] = x[f];
     }
     __syncthreads();
     const float *W1 = wts, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
-    float x[F];
+    const uint32_t r16 = lane & 15u, g = lane >> 4;  // operand row / column, 4-lane-group index
+    // layer 1: h1 = relu(x W1^T + b1), K = F
 #pragma unroll
-    for (int f = 0; f < F; ++f) x[f] = xs[f][lane];
+    for (int jc = 0; jc < H / 16; ++jc) {
+        v4f acc = { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
-    for (int u = 0; u < Q; ++u) {
-        const int k = (int)w * Q + u;
-        float a = b1[k];
-#pragma unroll
-        for (int f = 0; f < F; ++f) a += W1[k * F + f] * x[f];
-        h1s[k][lane] = a > 0.f ? a : 0.f;
-    }
-    __syncthreads();
-    float h1[H];
-#pragma unroll
-    for (int k = 0; k < H; ++k) h1[k] = h1s[k][lane];
-    float o = 0.f;
-    for (int u = 0; u < Q; ++u) {
-        const int j = (int)w * Q + u;
-        const float4 *row = reinterpret_cast<const float4 *>(W2 + j * H);  // H * 4 bytes per row: 16-byte aligned
-        float a = b2[j];
-#pragma unroll
-        for (int k4 = 0; k4 < H / 4; ++k4) {
-            const float4 wv = row[k4];
-            a += wv.x * h1[4 * k4];
-            a += wv.y * h1[4 * k4 + 1];
-            a += wv.z * h1[4 * k4 + 2];
-            a += wv.w * h1[4 * k4 + 3];
+        for (int ks = 0; ks < F / 4; ++ks) {
+            const float a = xs[w][ks * 4 + g][r16];
+            const float b = W1[(jc * 16 + r16) * F + ks * 4 + g];
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
         }
-        o += W3[j] * (a > 0.f ? a : 0.f);
+        const int u = jc * 16 + (int)r16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float hv = acc[r] + b1[u];
+            h1s[w][u][4 * g + r] = hv > 0.f ? hv : 0.f;
+        }
     }
-    part[w][lane] = o;
     __syncthreads();
-    if (w == 0 && t < n) {
-        const float out = ((b3[0] + part[0][lane]) + part[1][lane] + part[2][lane]) + part[3][lane];
-        const float c = ceilf(expm1f(out < 20.f ? out : 20.f) * margin);
-        cap[t] = (c > 0.f ? (uint32_t)c : 0u) + slack;
+    // layer 2: h2 = relu(h1 W2^T + b2), K = H; layer 3 folded into the epilogue
+    v4f acc[H / 16];
+#pragma unroll
+    for (int jc = 0; jc < H / 16; ++jc) acc[jc] = v4f{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+    for (int ks = 0; ks < H / 4; ++ks) {
+        const float a = h1s[w][ks * 4 + g][r16];
+#pragma unroll
+        for (int jc = 0; jc < H / 16; ++jc) {
+            const float b = W2[(jc * 16 + r16) * H + ks * 4 + g];
+            acc[jc] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[jc], 0, 0, 0);
+        }
+    }
+    float o[4] = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+    for (int jc = 0; jc < H / 16; ++jc) {
+        const int u = jc * 16 + (int)r16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float hv = acc[jc][r] + b2[u];
+            o[r] += W3[u] * (hv > 0.f ? hv : 0.f);
+        }
+    }
+    // sum over the 16 units held by each 16-lane group; tile 4 g + r
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        for (int m = 8; m > 0; m >>= 1) o[r] += __shfl_xor(o[r], m);
+    if (r16 < 4) {
+        const uint32_t t = t0 + 4u * g + r16;
+        const float out = b3[0] + (r16 == 0 ? o[0] : r16 == 1 ? o[1] : r16 == 2 ? o[2] : o[3]);
+        if (t < n) {
+            const float c = ceilf(expm1f(out < 20.f ? out : 20.f) * margin);
+            cap[t] = (c > 0.f ? (uint32_t)c : 0u) + slack;
+        }
     }
 }
 
@@ -451,7 +493,7 @@ namespace rast {
 hipError_t launch_capacity(const Launch &L, hipStream_t s)
 {
     if (!L.grid[ST_CAP]) return hipSuccess;
-    const dim3 g(L.grid[ST_CAP]), b(256);
+    const dim3 g(L.grid[ST_CAP]), b(64 * kCapWaves);
     switch (L.model_hidden) {
     case 32: hipLaunchKernelGGL(k_capacity<32>, g, b, 0, s, L.jobs, L.table[ST_CAP]); break;
     case 64: hipLaunchKernelGGL(k_capacity<64>, g, b, 0, s, L.jobs, L.table[ST_CAP]); break;

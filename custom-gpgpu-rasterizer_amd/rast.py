@@ -24,7 +24,7 @@ RAST_OK = 0
 VIS_NONE = 0xFFFFFFFF
 BIN_EXACT, BIN_LEARNED, BIN_PREVFRAME = 0, 1, 2
 FLAG_TIMING, FLAG_NO_SHADE, FLAG_GRAPH = 1, 2, 4
-DEBUG_VERTEX_POS, DEBUG_VERTEX_ATTR, DEBUG_TRI_RECORDS, DEBUG_TRI_ALIVE, DEBUG_TILE_COUNTS, DEBUG_BIG_LIST = range(6)
+DEBUG_VERTEX_POS, DEBUG_VERTEX_ATTR, DEBUG_TRI_RECORDS, DEBUG_TRI_ALIVE, DEBUG_TILE_COUNTS, DEBUG_BIG_LIST, DEBUG_TILE_CAPS = range(7)
 
 EXPORTS = [
     "rast_last_error", "rast_device_query", "rast_renderer_create", "rast_renderer_destroy", "rast_setup_bins",

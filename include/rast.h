@@ -237,7 +237,8 @@ enum {
     RAST_DEBUG_TRI_RECORDS = 2, /* 16 words per triangle (valid for survivors) */
     RAST_DEBUG_TRI_ALIVE = 3,   /* u8 per triangle */
     RAST_DEBUG_TILE_COUNTS = 4, /* u32 per tile: queue length */
-    RAST_DEBUG_BIG_LIST = 5     /* u32 count followed by the IDs */
+    RAST_DEBUG_BIG_LIST = 5,    /* u32 count followed by the IDs */
+    RAST_DEBUG_TILE_CAPS = 6    /* u32 per tile: single-pass queue capacity of the last frame */
 };
 int rast_debug_read(rast_renderer *r, int which, void *dst, size_t bytes, size_t *size_out);
 

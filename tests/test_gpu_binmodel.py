@@ -115,3 +115,47 @@ def test_learned_frames_are_bit_identical_to_exact(renderer):
             for x, y in zip(a, b):
                 np.testing.assert_array_equal(x.view(np.uint32), y.view(np.uint32))
         m.close()
+
+
+@pytest.mark.parametrize("hidden", [16, 32, 64])
+def test_frame_capacities_match_the_oracle_forward(hidden):
+    """The frame-time capacity kernel (matrix-core layers, 16 tiles per wave)
+    against the oracle's forward pass.  The first layer's weights are zero
+    except on the two tile-position features, so each tile's prediction is a
+    function of its position alone; capacity = ceil(expm1(out) * margin) +
+    slack, exact except where the value sits within 1e-4 of an integer."""
+    w = small_terrain()
+    tile, margin, slack = 16, 1.25, 4
+    rng = np.random.default_rng(100 + hidden)
+    with R.Renderer(w.width, w.height, tile=tile) as r:
+        r.add_scene(w.scene)
+        m = R.BinModel(r, hidden=hidden, seed=9)
+        p = rng.normal(0, 0.4, m.num_params).astype(np.float32)
+        W1 = p[:hidden * F].reshape(hidden, F)
+        W1[:, :6] = 0.0
+        W1[:, 6:] *= 6.0  # position only, steep enough to vary over the screen
+        o3 = hidden * F + hidden + hidden * hidden + hidden
+        p[o3:o3 + hidden] *= 2.0
+        p[-1] = 2.5
+        m.set_params(p)
+        m.attach(r, margin, slack)
+        cam = S.train_test_paths(2, 1)[1][0]
+        r.render(cam, w.opts, R.BIN_EXACT)
+        r.render(cam, w.opts, R.BIN_LEARNED)
+        caps = r.debug_read(R.DEBUG_TILE_CAPS, np.uint32)
+        m.close()
+    tx_n, ty_n = -(-w.width // tile), -(-w.height // tile)
+    assert caps.size >= tx_n * ty_n
+    caps = caps[:tx_n * ty_n]
+    t = np.arange(tx_n * ty_n)
+    x = np.zeros((t.size, F), np.float32)
+    x[:, 6] = ((t // tx_n).astype(np.float32) + np.float32(0.5)) / np.float32(ty_n)
+    x[:, 7] = ((t % tx_n).astype(np.float32) + np.float32(0.5)) / np.float32(tx_n)
+    out = O.mlp_forward(p, F, hidden, x).astype(np.float64)
+    v = np.expm1(np.minimum(out, 20.0)) * margin
+    want = np.maximum(np.ceil(v), 0) + slack
+    near = np.abs(v - np.round(v)) < 1e-4 * np.maximum(1.0, np.abs(v))
+    diff = caps.astype(np.int64) - want.astype(np.int64)
+    assert np.all(diff[~near] == 0), f"{np.count_nonzero(diff[~near])} tiles differ"
+    assert np.all(np.abs(diff[near]) <= 1)
+    assert len(np.unique(caps)) > 20  # the predictions vary over the screen
