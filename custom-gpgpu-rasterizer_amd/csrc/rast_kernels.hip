@@ -1259,6 +1259,44 @@ __device__ __forceinline__ uint32_t pack_rgba(float r, float g, float b)
     return pack_channel(r) | (pack_channel(g) << 8) | (pack_channel(b) << 16) | 0xFF000000u;
 }
 
+// Load through a global-address-space pointer.  Pointers read from the job
+// and mesh tables are generic to the compiler, which would emit flat loads;
+// those also count against the LDS / scalar wait counter, so every LDS read
+// behind them would wait for the gathers in flight.
+template <typename T>
+__device__ __forceinline__ T ldg(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+// uint4 / float4 are classes whose copy takes a generic reference (a flat
+// load again): load them as the compiler's own 4-vectors.
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 ldg(const uint4 *p)
+{
+    const u32x4_t v = *(const __attribute__((address_space(1))) u32x4_t *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ float4 ldg(const float4 *p)
+{
+    const f32x4_t v = *(const __attribute__((address_space(1))) f32x4_t *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ DevTexture ldg(const DevTexture *p)
+{
+    static_assert(sizeof(DevTexture) == 16, "texture descriptor: pointer, width, height");
+    const u32x4_t v = *(const __attribute__((address_space(1))) u32x4_t *)p;
+    DevTexture t;
+    t.texels = reinterpret_cast<const uint32_t *>(((uint64_t)v.y << 32) | v.x);
+    t.w = v.z;
+    t.h = v.w;
+    return t;
+}
+
 __device__ __forceinline__ bool texel_coord(float uv, uint32_t n, int &out)
 {
     const float f = uv * (float)n + 0.5f;
@@ -1267,43 +1305,58 @@ __device__ __forceinline__ bool texel_coord(float uv, uint32_t n, int &out)
     return true;
 }
 
-// CUDATextureSampler::SamplePoint / SampleLinear (CCR/CUDATextureSampler.cuh:86-190).
-// unit[c] = c / 255.f (the same correctly rounded quotient as the reference).
-__device__ __forceinline__ void sample(const DevTexture &t, float u, float v, int mode, const float *unit, float &r,
-                                       float &g, float &b)
+// Texel indices of CUDATextureSampler::SamplePoint / SampleLinear
+// (CCR/CUDATextureSampler.cuh:86-190): the centre texel and, for the linear
+// mode, its left, right, lower and upper neighbours (the centre where one
+// falls outside the texture).  ok = false outside [0, 1] (magenta); the
+// indices are then 0, so the loads stay in bounds and go out with the others.
+struct Taps {
+    int c, n[4];
+    bool ok;
+};
+
+__device__ __forceinline__ Taps texel_taps(const DevTexture &t, float u, float v)
 {
-    int x, y;
-    if (!texel_coord(u, t.w, x) || !texel_coord(v, t.h, y)) { r = 1.f; g = 0.f; b = 1.f; return; }
-    const int w = (int)t.w;
-    const int orig = x + y * w;
-    if (mode == 0) {
-        const uint32_t s = t.texels[orig];
-        r = unit[s & 0xFFu];
-        g = unit[(s >> 8) & 0xFFu];
-        b = unit[(s >> 16) & 0xFFu];
-        return;
-    }
-    const int texSize = w * (int)t.h;
-    int nb[4];
-    nb[0] = orig - 1; if (nb[0] < 0) nb[0] = orig;
-    nb[1] = orig + 1; if (nb[1] >= texSize) nb[1] = orig;
-    nb[2] = orig + w; if (nb[2] >= texSize) nb[2] = orig;
-    nb[3] = orig - w; if (nb[3] < 0) nb[3] = orig;
+    Taps k;
+    int x = 0, y = 0;
+    const bool okx = texel_coord(u, t.w, x), oky = texel_coord(v, t.h, y);
+    k.ok = okx && oky;
+    const int w = (int)t.w, size = w * (int)t.h;
+    k.c = k.ok ? x + y * w : 0;
+    k.n[0] = k.c - 1 < 0 ? k.c : k.c - 1;
+    k.n[1] = k.c + 1 >= size ? k.c : k.c + 1;
+    k.n[2] = k.c + w >= size ? k.c : k.c + w;
+    k.n[3] = k.c - w < 0 ? k.c : k.c - w;
+    return k;
+}
+
+// Colour of point-sampled texel s; unit[c] = c / 255.f (the same correctly
+// rounded quotient as the reference).
+__device__ __forceinline__ void decode_point(uint32_t s, bool ok, const float *unit, float &r, float &g, float &b)
+{
+    r = ok ? unit[s & 0xFFu] : 1.f;
+    g = ok ? unit[(s >> 8) & 0xFFu] : 0.f;
+    b = ok ? unit[(s >> 16) & 0xFFu] : 1.f;
+}
+
+// SampleLinear's weighting: neighbours x 1/8 each in order, then the centre / 2, then / 255.
+__device__ __forceinline__ void decode_linear(const uint32_t *s, uint32_t sc, bool ok, float &r, float &g, float &b)
+{
     const float weight = 1.f / (4 * 2);
     float ar = 0.f, ag = 0.f, ab = 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const uint32_t s = t.texels[nb[i]];
-        ar += (float)(s & 0xFFu) * weight;
-        ag += (float)((s >> 8) & 0xFFu) * weight;
-        ab += (float)((s >> 16) & 0xFFu) * weight;
+        ar += (float)(s[i] & 0xFFu) * weight;
+        ag += (float)((s[i] >> 8) & 0xFFu) * weight;
+        ab += (float)((s[i] >> 16) & 0xFFu) * weight;
     }
-    const uint32_t s = t.texels[orig];
-    ar += (float)(s & 0xFFu) / 2.f;
-    ag += (float)((s >> 8) & 0xFFu) / 2.f;
-    ab += (float)((s >> 16) & 0xFFu) / 2.f;
+    ar += (float)(sc & 0xFFu) / 2.f;
+    ag += (float)((sc >> 8) & 0xFFu) / 2.f;
+    ab += (float)((sc >> 16) & 0xFFu) / 2.f;
     const float rev = 1.0f / 255.f;
-    r = ar * rev; g = ag * rev; b = ab * rev;
+    r = ok ? ar * rev : 1.f;
+    g = ok ? ag * rev : 0.f;
+    b = ok ? ab * rev : 1.f;
 }
 
 // PS_Kernel + ShadePixelUnsafe (CCR/CUDARenderer.cu:833-897, CCR/CUDAROPs.cu:472-514)
@@ -1313,6 +1366,11 @@ __device__ __forceinline__ void sample(const DevTexture &t, float u, float v, in
 // the hardware reciprocal / rsqrt / exp2 / log2, which keeps RGBA8 within
 // 1/255 of the reference formula.  64x4-pixel blocks (rows of 64 per wave:
 // fully coalesced vis reads and RGBA8 writes; 8x8 squares measured the same).
+// A pixel's data is a chain of dependent gathers (vis, triangle record,
+// vertex attributes and texture table, texels); every level's loads are
+// issued together, with no branch between them, and range checks are applied
+// to the loaded values.  The texture table is read with scalar loads when all
+// of a wave's pixels belong to one mesh.
 __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
                                                const uint16_t *__restrict__ table)
 {
@@ -1333,25 +1391,42 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     const uint32_t y = (strip / bw) * 4u + (threadIdx.x >> 6);
     if (x >= fp.width || y >= fp.height) return;
     const size_t pid = (size_t)y * fp.width + x;
-    const uint32_t id = vis[pid];
+    const uint32_t id = ldg(vis + pid);
     if (id >= fp.num_tris) {
         rgba[pid] = fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u;
         return;
     }
     const uint4 *rp = reinterpret_cast<const uint4 *>(trirec + (size_t)id * kTriRecWords);
-    const uint4 q0 = rp[0], q1 = rp[1], q2 = rp[2], q3 = rp[3];
+    const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q2 = ldg(rp + 2), q3 = ldg(rp + 3);
     const float sx0 = __uint_as_float(q0.x), sy0 = __uint_as_float(q0.y);
     const float sx1 = __uint_as_float(q0.z), sy1 = __uint_as_float(q0.w);
     const float sx2 = __uint_as_float(q1.x), sy2 = __uint_as_float(q1.y);
     const float area = __uint_as_float(q2.y);
+    // level 3: vertex attributes and the mesh's texture table
+    const float4 *A0 = attr + (size_t)q3.x * 4, *A1 = attr + (size_t)q3.y * 4, *A2 = attr + (size_t)q3.z * 4;
+    const float4 a00 = ldg(A0), a10 = ldg(A1), a20 = ldg(A2);
+    const float4 a01 = ldg(A0 + 1), a11 = ldg(A1 + 1), a21 = ldg(A2 + 1);
+    const float4 a02 = ldg(A0 + 2), a12 = ldg(A1 + 2), a22 = ldg(A2 + 2);
+    const uint32_t mi = q3.w - J.mesh0 < fp.num_meshes ? q3.w : J.mesh0;  // entry in the launch's mesh table
+    const uint32_t mi0 = __builtin_amdgcn_readfirstlane(mi);
+    DevTexture tex[4];
+    int has_diffuse;
+    if (__ballot(mi != mi0) == 0) {
+        const DevMesh &M = meshes[mi0];  // wave-uniform: scalar loads
+#pragma unroll
+        for (int k = 0; k < 4; ++k) tex[k] = M.tex[k];
+        has_diffuse = M.has_diffuse;
+    } else {
+        const DevMesh *M = meshes + mi;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) tex[k] = ldg(M->tex + k);
+        has_diffuse = ldg(&M->has_diffuse);
+    }
     const float px = (float)x, py = (float)y;
     float w2 = (px - sx0) * (sy1 - sy0) - (py - sy0) * (sx1 - sx0);
     float w0 = (px - sx1) * (sy2 - sy1) - (py - sy1) * (sx2 - sx1);
     float w1 = (px - sx2) * (sy0 - sy2) - (py - sy2) * (sx0 - sx2);
     w0 = w0 / area; w1 = w1 / area; w2 = w2 / area;
-    const float4 *A0 = attr + (size_t)q3.x * 4, *A1 = attr + (size_t)q3.y * 4, *A2 = attr + (size_t)q3.z * 4;
-    const float4 a00 = A0[0], a10 = A1[0], a20 = A2[0];
-    const float4 a01 = A0[1], a11 = A1[1], a21 = A2[1];
     const float i0 = a00.z, i1 = a10.z, i2 = a20.z;
     const float wI = 1.f / (i0 * w0 + i1 * w1 + i2 * w2);
     // attributes arrive multiplied by their vertex's 1/w
@@ -1362,22 +1437,46 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     const float ny = RAST_INTERP(a01.x, a11.x, a21.x) * wI;
     const float nz = RAST_INTERP(a01.y, a11.y, a21.y) * wI;
     if (fp.visualisation == 1) {  // depth colour, Remap(z, 0.985, 1)
-        rgba[pid] = pack_rgba((depth[pid] - 0.985f) / (1.f - 0.985f), 0.f, 0.f);
+        rgba[pid] = pack_rgba((ldg(depth + pid) - 0.985f) / (1.f - 0.985f), 0.f, 0.f);
         return;
     }
     if (fp.visualisation == 2) {  // normal colour
         rgba[pid] = pack_rgba(nx, ny, nz);
         return;
     }
-    const uint32_t mi = q3.w - J.mesh0 < fp.num_meshes ? q3.w : J.mesh0;  // entry in the launch's mesh table
-    const DevMesh &M = meshes[mi];
-    if (!M.has_diffuse) {  // vertex colour (perspective-weighted, not x w)
-        const float4 a03 = A0[3], a13 = A1[3], a23 = A2[3];
+    if (!has_diffuse) {  // vertex colour (perspective-weighted, not x w)
+        const float4 a03 = ldg(A0 + 3), a13 = ldg(A1 + 3), a23 = ldg(A2 + 3);
         rgba[pid] = pack_rgba(RAST_INTERP(a03.x, a13.x, a23.x), RAST_INTERP(a03.y, a13.y, a23.y),
                               RAST_INTERP(a03.z, a13.z, a23.z));
         return;
     }
-    const float4 a02 = A0[2], a12 = A1[2], a22 = A2[2];
+    // level 4: the texels of all four maps
+    Taps tp[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) tp[k] = texel_taps(tex[k], u, v);
+    float dr, dg, db, nr, ng, nb, sr, sg, sb, gr, gg, gb;
+    if (fp.sample_state == 0) {
+        uint32_t sc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sc[k] = ldg(tex[k].texels + tp[k].c);
+        decode_point(sc[0], tp[0].ok, unit, dr, dg, db);
+        decode_point(sc[1], tp[1].ok, unit, nr, ng, nb);
+        decode_point(sc[2], tp[2].ok, unit, sr, sg, sb);
+        decode_point(sc[3], tp[3].ok, unit, gr, gg, gb);
+    } else {
+        uint32_t sc[4], sn[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            sc[k] = ldg(tex[k].texels + tp[k].c);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sn[k][i] = ldg(tex[k].texels + tp[k].n[i]);
+        }
+        decode_linear(sn[0], sc[0], tp[0].ok, dr, dg, db);
+        decode_linear(sn[1], sc[1], tp[1].ok, nr, ng, nb);
+        decode_linear(sn[2], sc[2], tp[2].ok, sr, sg, sb);
+        decode_linear(sn[3], sc[3], tp[3].ok, gr, gg, gb);
+    }
+    (void)gg; (void)gb;
     const float tx = RAST_INTERP(a01.z, a11.z, a21.z);
     const float ty = RAST_INTERP(a01.w, a11.w, a21.w);
     const float tz = RAST_INTERP(a02.x, a12.x, a22.x);
@@ -1388,12 +1487,6 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     const float vl = vx * vx + vy * vy + vz * vz;
     const float vinv = vl > 0.f ? __builtin_amdgcn_rsqf(vl) : 0.f;
     vx *= vinv; vy *= vinv; vz *= vinv;
-    float dr, dg, db, nr, ng, nb, sr, sg, sb, gr, gg, gb;
-    sample(M.tex[0], u, v, fp.sample_state, unit, dr, dg, db);
-    sample(M.tex[1], u, v, fp.sample_state, unit, nr, ng, nb);
-    sample(M.tex[2], u, v, fp.sample_state, unit, sr, sg, sb);
-    sample(M.tex[3], u, v, fp.sample_state, unit, gr, gg, gb);
-    (void)gg; (void)gb;
     // binormal = -Cross(tan, n); N = {tan, binormal, n} * (2 * sample - 1)
     const float cx = ty * nz - tz * ny, cy = tz * nx - tx * nz, cz = tx * ny - ty * nx;
     const float fx = 2.f * nr - 1.f, fy = 2.f * ng - 1.f, fz = 2.f * nb - 1.f;

@@ -40,14 +40,17 @@ hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s);
 hipError_t launch_shade(const Launch &L, hipStream_t s);
 hipError_t launch_publish(const Launch &L, hipStream_t s);
 
-// Block -> job lookup of a shared stage grid.
+// Block -> job lookup of a shared stage grid.  The 16-bit table entry is
+// read with a vector load (there is no 16-bit scalar load); it is the same
+// for the whole block, and readfirstlane says so, which keeps the job index
+// and every job field read through it in scalar registers.
 __device__ __forceinline__ uint32_t job_of(const uint16_t *table, const JobDesc *jobs, int stage, uint32_t &local)
 {
     if (!table) {
         local = blockIdx.x;
         return 0;
     }
-    const uint32_t j = table[blockIdx.x];
+    const uint32_t j = __builtin_amdgcn_readfirstlane((uint32_t)table[blockIdx.x]);
     local = blockIdx.x - jobs[j].block0[stage];
     return j;
 }
