@@ -46,6 +46,89 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Global-memory access through global-address-space pointers.  Pointers
+// read from the job and mesh tables are generic to the compiler, which would
+// emit flat loads, stores and atomics; those also count against the LDS /
+// scalar wait counter, so every LDS access behind them would wait for the
+// global operations in flight (the rasterizer loops mix both).
+template <typename T>
+__device__ __forceinline__ T ldg(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+// uint4 / float4 are classes whose copy takes a generic reference (a flat
+// load again): load them as the compiler's own 4-vectors.
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 ldg(const uint4 *p)
+{
+    const u32x4_t v = *(const __attribute__((address_space(1))) u32x4_t *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ float4 ldg(const float4 *p)
+{
+    const f32x4_t v = *(const __attribute__((address_space(1))) f32x4_t *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ DevTexture ldg(const DevTexture *p)
+{
+    static_assert(sizeof(DevTexture) == 16, "texture descriptor: pointer, width, height");
+    const u32x4_t v = *(const __attribute__((address_space(1))) u32x4_t *)p;
+    DevTexture t;
+    t.texels = reinterpret_cast<const uint32_t *>(((uint64_t)v.y << 32) | v.x);
+    t.w = v.z;
+    t.h = v.w;
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+template <typename T>
+__device__ __forceinline__ void stg(T *p, T v)
+{
+    *gptr(p) = v;
+}
+
+__device__ __forceinline__ void stg(uint4 *p, uint4 v)
+{
+    *(__attribute__((address_space(1))) u32x4_t *)p = u32x4_t{ v.x, v.y, v.z, v.w };
+}
+
+__device__ __forceinline__ void stg(float4 *p, float4 v)
+{
+    *(__attribute__((address_space(1))) f32x4_t *)p = f32x4_t{ v.x, v.y, v.z, v.w };
+}
+
+__device__ __forceinline__ void stg(uint2 *p, uint2 v)
+{
+    typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+    *(__attribute__((address_space(1))) u32x2_t *)p = u32x2_t{ v.x, v.y };
+}
+
+// atomicAdd / atomicMin with relaxed order at device scope, on global memory.
+__device__ __forceinline__ uint32_t gatomic_add(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void gatomic_min(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_fetch_min(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void gatomic_min(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_fetch_min(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Order-free append of one element per thread with pred set, one atomic per
 // 256-thread block.  Every thread of the block must call it.
 __device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool pred, uint32_t *lds)
@@ -57,7 +140,7 @@ __device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool pred, u
     if (threadIdx.x == 0) {
         const uint32_t a = lds[0], b = lds[1], c = lds[2], d = lds[3];
         const uint32_t tot = a + b + c + d;
-        const uint32_t base = tot ? atomicAdd(counter, tot) : 0u;
+        const uint32_t base = tot ? gatomic_add(counter, tot) : 0u;
         lds[0] = base;
         lds[1] = base + a;
         lds[2] = base + a + b;
@@ -103,7 +186,7 @@ __device__ __forceinline__ void block_append3(uint32_t *c0, bool p0, uint32_t *c
         uint32_t *l = lds + 4 * threadIdx.x;
         const uint32_t a = l[0], b = l[1], c = l[2], d = l[3];
         const uint32_t tot = a + b + c + d;
-        const uint32_t base = tot ? atomicAdd(cnt, tot) : 0u;
+        const uint32_t base = tot ? gatomic_add(cnt, tot) : 0u;
         l[0] = base;
         l[1] = base + a;
         l[2] = base + a + b;
@@ -120,7 +203,7 @@ __device__ __forceinline__ void block_append3(uint32_t *c0, bool p0, uint32_t *c
 __device__ __forceinline__ void count_tile(uint32_t *counters, uint32_t t)
 {
     const uint64_t g = match_tile(t);
-    if (popc_below(g) == 0) atomicAdd(counters + t, (uint32_t)__popcll(g));
+    if (popc_below(g) == 0) gatomic_add(counters + t, (uint32_t)__popcll(g));
 }
 
 // Returns a unique slot in tile t's queue (previous cursor value): the group
@@ -131,7 +214,7 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t *cursor, uint32_t t)
     const uint32_t rank = popc_below(g);
     const int leader = __builtin_ctzll(g);
     uint32_t base = 0;
-    if (rank == 0) base = atomicAdd(cursor + t, (uint32_t)__popcll(g));
+    if (rank == 0) base = gatomic_add(cursor + t, (uint32_t)__popcll(g));
     base = __shfl(base, leader);
     return base + rank;
 }
@@ -269,7 +352,7 @@ __device__ __forceinline__ void set_edges(TriLds &t)
 __device__ __forceinline__ void load_tri(const uint32_t *__restrict__ rec, uint32_t id, TriLds &t)
 {
     const uint4 *r = reinterpret_cast<const uint4 *>(rec + (size_t)id * kTriRecWords);
-    const uint4 q0 = r[0], q1 = r[1], q2 = r[2];
+    const uint4 q0 = ldg(r), q1 = ldg(r + 1), q2 = ldg(r + 2);
     t.sx0 = __uint_as_float(q0.x); t.sy0 = __uint_as_float(q0.y);
     t.sx1 = __uint_as_float(q0.z); t.sy1 = __uint_as_float(q0.w);
     t.sx2 = __uint_as_float(q1.x); t.sy2 = __uint_as_float(q1.y);
@@ -320,7 +403,7 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
     const uint32_t v = (blockIdx.x - M.vs_block0) * 256u + threadIdx.x;
     if (v >= M.num_verts) return;
     const float4 *iv = reinterpret_cast<const float4 *>(M.ivert + (size_t)v * 16);
-    const float4 q0 = iv[0], q1 = iv[1], q2 = iv[2], q3 = iv[3];
+    const float4 q0 = ldg(iv), q1 = ldg(iv + 1), q2 = ldg(iv + 2), q3 = ldg(iv + 3);
     const float px = q0.x, py = q0.y, pz = q0.z;
     const float u = q0.w, vv = q1.x;
     const float nx = q1.y, ny = q1.z, nz = q1.w;
@@ -335,7 +418,7 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
     o1 /= o3;
     o2 /= o3;
     const uint32_t g = M.vert_base + v;
-    pos4[g] = make_float4(o0, o1, o2, o3);
+    stg(pos4 + g, make_float4(o0, o1, o2, o3));
     const float n0 = m4(Wd, 0, 0) * nx + m4(Wd, 0, 1) * ny + m4(Wd, 0, 2) * nz;
     const float n1 = m4(Wd, 1, 0) * nx + m4(Wd, 1, 1) * ny + m4(Wd, 1, 2) * nz;
     const float n2 = m4(Wd, 2, 0) * nx + m4(Wd, 2, 1) * ny + m4(Wd, 2, 2) * nz;
@@ -353,10 +436,10 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
     // here once or per pixel.  PBR shading reads the first three float4s only.
     const float iw = 1.f / o3;
     float4 *a = attr + (size_t)g * 4;
-    a[0] = make_float4(u * iw, vv * iw, iw, n0 * iw);
-    a[1] = make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw);
-    a[2] = make_float4(t2 * iw, vx * iw, vy * iw, vz * iw);
-    a[3] = make_float4(cr * iw, cg * iw, cb * iw, 0.f);
+    stg(a, make_float4(u * iw, vv * iw, iw, n0 * iw));
+    stg(a + 1, make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw));
+    stg(a + 2, make_float4(t2 * iw, vx * iw, vy * iw, vz * iw));
+    stg(a + 3, make_float4(cr * iw, cg * iw, cb * iw, 0.f));
     if (density && (v & 7u) == 0) {
         // projected-vertex density feature of the capacity model (every 8th
         // vertex; no parity role)
@@ -402,15 +485,15 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     if (valid) {
         uint32_t i0, i1, i2;
         if (M.topology == 0) {
-            i0 = M.idx[3 * (size_t)k]; i1 = M.idx[3 * (size_t)k + 1]; i2 = M.idx[3 * (size_t)k + 2];
+            i0 = ldg(M.idx + 3 * (size_t)k); i1 = ldg(M.idx + 3 * (size_t)k + 1); i2 = ldg(M.idx + 3 * (size_t)k + 2);
         } else {  // strip: odd triangles swap their last two indices
             const bool odd = (k & 1u) != 0;
-            i0 = M.idx[k];
-            i1 = odd ? M.idx[k + 2] : M.idx[k + 1];
-            i2 = odd ? M.idx[k + 1] : M.idx[k + 2];
+            i0 = ldg(M.idx + k);
+            i1 = ldg(M.idx + k + (odd ? 2 : 1));
+            i2 = ldg(M.idx + k + (odd ? 1 : 2));
         }
         i0 += M.vert_base; i1 += M.vert_base; i2 += M.vert_base;
-        const float4 P0 = pos4[i0], P1 = pos4[i1], P2 = pos4[i2];
+        const float4 P0 = ldg(pos4 + i0), P1 = ldg(pos4 + i1), P2 = ldg(pos4 + i2);
         ok = true;
         if (fp.cull_mode != 2) {  // back / front face (CCR/ERenderer.cpp:82-101)
             const float e1x = P1.x - P0.x, e1y = P1.y - P0.y, e1z = P1.z - P0.z;
@@ -458,16 +541,16 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                 xmax = xmax > (int)fp.width ? (int)fp.width : xmax;
                 ymax = ymax > (int)fp.height ? (int)fp.height : ymax;
                 uint4 *r = reinterpret_cast<uint4 *>(trirec + (size_t)gid * kTriRecWords);
-                r[0] = make_uint4(__float_as_uint(t.sx0), __float_as_uint(t.sy0), __float_as_uint(t.sx1),
-                                  __float_as_uint(t.sy1));
-                r[1] = make_uint4(__float_as_uint(t.sx2), __float_as_uint(t.sy2), __float_as_uint(t.z0),
-                                  __float_as_uint(t.z1));
-                r[2] = make_uint4(__float_as_uint(t.z2), __float_as_uint(t.area),
-                                  (uint32_t)xmin | ((uint32_t)ymin << 16), (uint32_t)xmax | ((uint32_t)ymax << 16));
-                r[3] = make_uint4(i0, i1, i2, mi);
+                stg(r, make_uint4(__float_as_uint(t.sx0), __float_as_uint(t.sy0), __float_as_uint(t.sx1),
+                                  __float_as_uint(t.sy1)));
+                stg(r + 1, make_uint4(__float_as_uint(t.sx2), __float_as_uint(t.sy2), __float_as_uint(t.z0),
+                                      __float_as_uint(t.z1)));
+                stg(r + 2, make_uint4(__float_as_uint(t.z2), __float_as_uint(t.area),
+                                      (uint32_t)xmin | ((uint32_t)ymin << 16), (uint32_t)xmax | ((uint32_t)ymax << 16)));
+                stg(r + 3, make_uint4(i0, i1, i2, mi));
             }
         }
-        J.alive[gid] = ok ? 1 : 0;
+        stg(J.alive + gid, (uint8_t)(ok ? 1 : 0));
     }
 
     // classification by the pixel range [xmin,xmax) x [ymin,ymax)
@@ -537,7 +620,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                         item_pixel(mt, kk - wo[j], px, py);
                         float z;
                         if (fragment(mt, (float)px, (float)py, z))
-                            atomicMin(keys + (size_t)py * fp.width + px, (unsigned long long)make_key(z, mt.id));
+                            gatomic_min(keys + (size_t)py * fp.width + px, (unsigned long long)make_key(z, mt.id));
                     }
                     before += (uint32_t)__popcll(m);
                 }
@@ -550,24 +633,24 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                   sslot);
     (void)n_ok;
     if (is_big) {
-        if (bslot < J.big_cap) J.big[bslot] = gid;
+        if (bslot < J.big_cap) stg(J.big + bslot, gid);
         else ctr->storage_overflow = 1;
     }
     if (binned) {
-        J.survivors[sslot] = gid;
+        stg(J.survivors + sslot, gid);
         const uint32_t w = tx1 - tx0 + 1;
         uint32_t tx = tx0, ty = ty0;
         for (uint32_t s = 0; s < span; ++s) {
             const uint32_t tl = ty * fp.tiles_x + tx;
             if (single_pass) {
                 const uint32_t slot = claim_slot(tile_count, tl);
-                if (slot < J.tile_cap[tl]) {
-                    const uint32_t pos = J.tile_off[tl] + slot;
-                    if (pos < J.queue_cap) J.queue[pos] = gid;
+                if (slot < ldg(J.tile_cap + tl)) {
+                    const uint32_t pos = ldg(J.tile_off + tl) + slot;
+                    if (pos < J.queue_cap) stg(J.queue + pos, gid);
                 } else {
                     // capacity exceeded: keep the entry for the exact fallback
-                    const uint32_t o = atomicAdd(&ctr->ovf_entries, 1u);
-                    if (o < J.ovf_cap) J.ovf[o] = make_uint2(tl, gid);
+                    const uint32_t o = gatomic_add(&ctr->ovf_entries, 1u);
+                    if (o < J.ovf_cap) stg(J.ovf + o, make_uint2(tl, gid));
                     else ctr->storage_overflow = 1;
                 }
             } else {
@@ -969,22 +1052,22 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
         best[q] = kClearKey;
         if (inside[q]) {
             const size_t o = (size_t)pys[q] * fp.width + pxs[q];
-            best[q] = keys[o];
-            keys[o] = kClearKey;
+            best[q] = ldg(keys + o);
+            stg(keys + o, (unsigned long long)kClearKey);
         }
     }
 
     // list A: the tile's queue region; list B: its fallback pool region
-    const uint32_t cnt = tile_count[tile];
+    const uint32_t cnt = ldg(tile_count + tile);
     uint32_t n_a = cnt, n_b = 0;
-    const uint32_t qoff = tile_off[tile];
+    const uint32_t qoff = ldg(tile_off + tile);
     const uint32_t *list_a = queue + qoff;
     const uint32_t *list_b = pool;
     if (tile_cap) {
-        const uint32_t cp = tile_cap[tile];
+        const uint32_t cp = ldg(tile_cap + tile);
         if (cnt > cp) {
             n_a = cp;
-            const uint32_t po = tile_pool[tile];
+            const uint32_t po = ldg(tile_pool + tile);
             if (po != 0xFFFFFFFFu) {
                 list_b = pool + po;
                 const uint32_t room_b = po < pool_cap ? pool_cap - po : 0;
@@ -1008,7 +1091,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             TriLds t;
             t.xr = 0; t.yr = 0; t.inv = 0.f;
             if (i < n_tile) {
-                const uint32_t id = i < n_a ? list_a[i] : list_b[i - n_a];
+                const uint32_t id = i < n_a ? ldg(list_a + i) : ldg(list_b + (i - n_a));
                 if (id < num_tris) {
                     load_tri(trirec, id, t);
                     const int bx0 = (int)(t.xr & 0xFFFFu), by0 = (int)(t.xr >> 16);
@@ -1096,7 +1179,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     // fit, pass 2 takes the whole list in its stored order instead.  Each
     // chunk is loaded and re-bounded in parallel, compacted into LDS, and
     // every lane tests its own pixels against it.
-    uint32_t n_big = ctr->num_big;
+    uint32_t n_big = ldg(&ctr->num_big);
     n_big = n_big < big_cap ? n_big : big_cap;
     if (n_big) {
         float tmax = tile_max_depth<PPT, NW>(best, lane, wave, wdmax);
@@ -1107,7 +1190,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             bool keep = false;
             float lb = 0.f;
             if (i < n_big) {
-                const uint32_t id = big[i];
+                const uint32_t id = ldg(big + i);
                 if (id < num_tris) {
                     TriLds t;
                     load_tri(trirec, id, t);
@@ -1152,7 +1235,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             float4 pl;
             if (i < n_list) {
                 const uint32_t li = sorted ? (uint32_t)cand[i] : i;
-                const uint32_t id = big[li];
+                const uint32_t id = ldg(big + li);
                 if (id < num_tris) {
                     load_tri(trirec, id, t);
                     const int x0 = (int)(t.xr & 0xFFFFu), y0 = (int)(t.xr >> 16);
@@ -1228,8 +1311,8 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
         dmin = d < dmin ? d : dmin;
         if (inside[q]) {
             const size_t o = (size_t)pys[q] * fp.width + pxs[q];
-            vis[o] = ~(uint32_t)best[q] - 1u;
-            depth[o] = d;
+            stg(vis + o, ~(uint32_t)best[q] - 1u);
+            stg(depth + o, d);
         }
     }
     if (tile_min_depth) {
@@ -1238,7 +1321,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             dmin = y < dmin ? y : dmin;
         }
         // depths are >= 0, so their bit patterns order like the values
-        if (lane == 0) atomicMin(reinterpret_cast<uint32_t *>(tile_min_depth) + tile, __float_as_uint(dmin));
+        if (lane == 0) gatomic_min(reinterpret_cast<uint32_t *>(tile_min_depth) + tile, __float_as_uint(dmin));
     }
 }
 
@@ -1257,44 +1340,6 @@ __device__ __forceinline__ uint32_t pack_channel(float c)
 __device__ __forceinline__ uint32_t pack_rgba(float r, float g, float b)
 {
     return pack_channel(r) | (pack_channel(g) << 8) | (pack_channel(b) << 16) | 0xFF000000u;
-}
-
-// Load through a global-address-space pointer.  Pointers read from the job
-// and mesh tables are generic to the compiler, which would emit flat loads;
-// those also count against the LDS / scalar wait counter, so every LDS read
-// behind them would wait for the gathers in flight.
-template <typename T>
-__device__ __forceinline__ T ldg(const T *p)
-{
-    return *(const __attribute__((address_space(1))) T *)p;
-}
-
-// uint4 / float4 are classes whose copy takes a generic reference (a flat
-// load again): load them as the compiler's own 4-vectors.
-typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint4 ldg(const uint4 *p)
-{
-    const u32x4_t v = *(const __attribute__((address_space(1))) u32x4_t *)p;
-    return make_uint4(v.x, v.y, v.z, v.w);
-}
-
-__device__ __forceinline__ float4 ldg(const float4 *p)
-{
-    const f32x4_t v = *(const __attribute__((address_space(1))) f32x4_t *)p;
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-
-__device__ __forceinline__ DevTexture ldg(const DevTexture *p)
-{
-    static_assert(sizeof(DevTexture) == 16, "texture descriptor: pointer, width, height");
-    const u32x4_t v = *(const __attribute__((address_space(1))) u32x4_t *)p;
-    DevTexture t;
-    t.texels = reinterpret_cast<const uint32_t *>(((uint64_t)v.y << 32) | v.x);
-    t.w = v.z;
-    t.h = v.w;
-    return t;
 }
 
 __device__ __forceinline__ bool texel_coord(float uv, uint32_t n, int &out)
@@ -1393,7 +1438,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     const size_t pid = (size_t)y * fp.width + x;
     const uint32_t id = ldg(vis + pid);
     if (id >= fp.num_tris) {
-        rgba[pid] = fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u;
+        stg(rgba + pid, (uint32_t)(fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u));
         return;
     }
     const uint4 *rp = reinterpret_cast<const uint4 *>(trirec + (size_t)id * kTriRecWords);
@@ -1437,17 +1482,17 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     const float ny = RAST_INTERP(a01.x, a11.x, a21.x) * wI;
     const float nz = RAST_INTERP(a01.y, a11.y, a21.y) * wI;
     if (fp.visualisation == 1) {  // depth colour, Remap(z, 0.985, 1)
-        rgba[pid] = pack_rgba((ldg(depth + pid) - 0.985f) / (1.f - 0.985f), 0.f, 0.f);
+        stg(rgba + pid, (uint32_t)(pack_rgba((ldg(depth + pid) - 0.985f) / (1.f - 0.985f), 0.f, 0.f)));
         return;
     }
     if (fp.visualisation == 2) {  // normal colour
-        rgba[pid] = pack_rgba(nx, ny, nz);
+        stg(rgba + pid, (uint32_t)(pack_rgba(nx, ny, nz)));
         return;
     }
     if (!has_diffuse) {  // vertex colour (perspective-weighted, not x w)
         const float4 a03 = ldg(A0 + 3), a13 = ldg(A1 + 3), a23 = ldg(A2 + 3);
-        rgba[pid] = pack_rgba(RAST_INTERP(a03.x, a13.x, a23.x), RAST_INTERP(a03.y, a13.y, a23.y),
-                              RAST_INTERP(a03.z, a13.z, a23.z));
+        stg(rgba + pid, (uint32_t)(pack_rgba(RAST_INTERP(a03.x, a13.x, a23.x), RAST_INTERP(a03.y, a13.y, a23.y),
+                              RAST_INTERP(a03.z, a13.z, a23.z))));
         return;
     }
     // level 4: the texels of all four maps
@@ -1513,7 +1558,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
         const float rev = __builtin_amdgcn_rcpf(mx);
         outr *= rev; outg *= rev; outb *= rev;
     }
-    rgba[pid] = pack_rgba(outr, outg, outb);
+    stg(rgba + pid, (uint32_t)(pack_rgba(outr, outg, outb)));
 }
 
 // Copies every job's counters to its renderer's pinned host block (one word
