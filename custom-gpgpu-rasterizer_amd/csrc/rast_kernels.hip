@@ -1171,21 +1171,23 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
         for (int q = 0; q < PPT; ++q) best[q] = key[(int)tid + q * NT];
     }
 
-    // Screen-list triangles.  Pass 1 bounds every one over the tile and keeps
-    // those that can still take a pixel, keyed by their depth lower bound
-    // (in the key array, free once the queue is resolved).  Sorted front to
-    // back, pass 2 then stages them chunk by chunk and stops where the next
-    // lower bound lies behind every pixel of the tile.  When more survive than
-    // fit, pass 2 takes the whole list in its stored order instead.  Each
-    // chunk is loaded and re-bounded in parallel, compacted into LDS, and
-    // every lane tests its own pixels against it.
+    // Screen-list triangles.  A list longer than one chunk is first bounded
+    // over the tile (pass 1), keeping the triangles that can still take a
+    // pixel, keyed by their depth lower bound (in the key array, free once the
+    // queue is resolved).  Sorted front to back, pass 2 then stages them chunk
+    // by chunk and stops where the next lower bound lies behind every pixel of
+    // the tile.  A list of one chunk, or more survivors than fit, goes to
+    // pass 2 in its stored order.  Each chunk is loaded and bounded in
+    // parallel, compacted into LDS, and every lane tests its own pixels
+    // against it.
     uint32_t n_big = ldg(&ctr->num_big);
     n_big = n_big < big_cap ? n_big : big_cap;
     if (n_big) {
         float tmax = tile_max_depth<PPT, NW>(best, lane, wave, wdmax);
         uint64_t *cand = key;  // NPIX entries: (ordered lower bound << 32) | list index
         uint32_t n_cand = 0;
-        for (uint32_t base = 0; base < n_big; base += NT) {
+        const bool presort = n_big > (uint32_t)NT;
+        for (uint32_t base = 0; presort && base < n_big; base += NT) {
             const uint32_t i = base + tid;
             bool keep = false;
             float lb = 0.f;
@@ -1216,7 +1218,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             n_cand += nk;
             __syncthreads();
         }
-        const bool sorted = n_cand <= (uint32_t)NPIX;
+        const bool sorted = presort && n_cand <= (uint32_t)NPIX;
         if (sorted) {
             uint32_t P = 1;
             while (P < n_cand) P <<= 1;
