@@ -94,9 +94,10 @@ __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t
 // tile l % 16, input l / 16), B 4 inputs x 16 units (lane l: input l / 16,
 // unit l % 16), and the 16 x 16 result sits in lane u + 16 (t / 4),
 // register t % 4 (tile t, unit u).  The parameters are staged in LDS once per
-// block; the first layer's activations pass through LDS as the second
-// layer's A operand; the output layer sums over units within each 16-lane
-// group.  Otherwise the previous frame's count: ceil(prev * margin) + slack.
+// block (rows padded against bank conflicts); the first layer's activations
+// pass through LDS as the second layer's A operand; the output layer sums
+// over units within each 16-lane group.  Otherwise the previous frame's
+// count: ceil(prev * margin) + slack.
 constexpr int kCapTiles = kCapacityTilesPerBlock;
 constexpr int kCapWaves = kCapTiles / 16;  // waves per block
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -106,10 +107,12 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
                                                              const uint16_t *__restrict__ table)
 {
     static_assert(H % 16 == 0 && F % 4 == 0, "unit and input counts must suit 16 x 16 x 4 tiles");
-    constexpr int NP = H * F + H + H * H + H + H + 1;
-    __shared__ __align__(16) float wts[(NP + 3) & ~3];
+    // weight rows padded by one float (H + 1, F + 1) and activations stored
+    // 17 wide, so the lanes of an operand read land in different LDS banks
+    constexpr int F1 = F + 1, H1 = H + 1;
+    __shared__ float w1s[H * F1], w2s[H * H1], b1s[H], b2s[H], w3s[H];
     __shared__ float xs[kCapWaves][F][16];
-    __shared__ float h1s[kCapWaves][H][16];
+    __shared__ float h1s[kCapWaves][H][17];
     uint32_t local;
     const JobDesc &J = jobs[job_of(table, jobs, ST_CAP, local)];
     const FrameParams &fp = J.fp;
@@ -127,14 +130,28 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
         }
         return;
     }
+    // parameter layout: W1 [H][F], b1 [H], W2 [H][H], b2 [H], W3 [H], b3
     const float *__restrict__ p = J.model;
-    {
-        // 16-byte loads for the bulk of the parameters (the buffer is 256-byte aligned)
-        constexpr int NP4 = NP / 4;
-        const float4 *p4 = reinterpret_cast<const float4 *>(p);
-        float4 *w4 = reinterpret_cast<float4 *>(wts);
-        for (int i = (int)threadIdx.x; i < NP4; i += 64 * kCapWaves) w4[i] = p4[i];
-        for (int i = NP4 * 4 + (int)threadIdx.x; i < NP; i += 64 * kCapWaves) wts[i] = p[i];
+    const float *gW1 = p, *gb1 = gW1 + H * F, *gW2 = gb1 + H, *gb2 = gW2 + H * H, *gW3 = gb2 + H;
+    const float b3 = p[H * F + H + H * H + H + H];
+    constexpr int NT = 64 * kCapWaves;
+    // 16-byte loads (W1 and W2 start at multiples of 16 bytes: offsets 0 and
+    // 9 H floats of a 256-byte aligned buffer), four padded LDS stores each
+    for (int i4 = (int)threadIdx.x; i4 < H * F / 4; i4 += NT) {
+        const float4 v = reinterpret_cast<const float4 *>(gW1)[i4];
+        float *d = w1s + (4 * i4 / F) * F1 + 4 * i4 % F;
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+#pragma unroll 4
+    for (int i4 = (int)threadIdx.x; i4 < H * H / 4; i4 += NT) {
+        const float4 v = reinterpret_cast<const float4 *>(gW2)[i4];
+        float *d = w2s + (4 * i4 / H) * H1 + 4 * i4 % H;
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+    for (int i = (int)threadIdx.x; i < H; i += NT) {
+        b1s[i] = gb1[i];
+        b2s[i] = gb2[i];
+        w3s[i] = gW3[i];
     }
     const uint32_t t0 = local * kCapTiles + w * 16u;  // this wave's first tile
     if (lane < 16) {
@@ -145,7 +162,6 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
         for (int f = 0; f < F; ++f) xs[w][f][lane] = x[f];
     }
     __syncthreads();
-    const float *W1 = wts, *b1 = W1 + H * F, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + H;
     const uint32_t r16 = lane & 15u, g = lane >> 4;  // operand row / column, 4-lane-group index
     // layer 1: h1 = relu(x W1^T + b1), K = F
 #pragma unroll
@@ -154,13 +170,13 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
 #pragma unroll
         for (int ks = 0; ks < F / 4; ++ks) {
             const float a = xs[w][ks * 4 + g][r16];
-            const float b = W1[(jc * 16 + r16) * F + ks * 4 + g];
+            const float b = w1s[(jc * 16 + r16) * F1 + ks * 4 + g];
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
         }
         const int u = jc * 16 + (int)r16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float hv = acc[r] + b1[u];
+            const float hv = acc[r] + b1s[u];
             h1s[w][u][4 * g + r] = hv > 0.f ? hv : 0.f;
         }
     }
@@ -174,7 +190,7 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
         const float a = h1s[w][ks * 4 + g][r16];
 #pragma unroll
         for (int jc = 0; jc < H / 16; ++jc) {
-            const float b = W2[(jc * 16 + r16) * H + ks * 4 + g];
+            const float b = w2s[(jc * 16 + r16) * H1 + ks * 4 + g];
             acc[jc] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[jc], 0, 0, 0);
         }
     }
@@ -184,8 +200,8 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
         const int u = jc * 16 + (int)r16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float hv = acc[jc][r] + b2[u];
-            o[r] += W3[u] * (hv > 0.f ? hv : 0.f);
+            const float hv = acc[jc][r] + b2s[u];
+            o[r] += w3s[u] * (hv > 0.f ? hv : 0.f);
         }
     }
     // sum over the 16 units held by each 16-lane group; tile 4 g + r
@@ -194,7 +210,7 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
         for (int m = 8; m > 0; m >>= 1) o[r] += __shfl_xor(o[r], m);
     if (r16 < 4) {
         const uint32_t t = t0 + 4u * g + r16;
-        const float out = b3[0] + (r16 == 0 ? o[0] : r16 == 1 ? o[1] : r16 == 2 ? o[2] : o[3]);
+        const float out = b3 + (r16 == 0 ? o[0] : r16 == 1 ? o[1] : r16 == 2 ? o[2] : o[3]);
         if (t < n) {
             const float c = ceilf(expm1f(out < 20.f ? out : 20.f) * margin);
             cap[t] = (c > 0.f ? (uint32_t)c : 0u) + slack;
