@@ -15,6 +15,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "rast.h"
@@ -620,13 +621,22 @@ static uint64_t plan_key(rast_renderer *const *rs, uint32_t n, int hidden)
     auto mix = [&h](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
     mix(n);
     mix((uint64_t)hidden);
+    // which entries share a source mesh (vertex groups): only views share meshes
+    bool views = false;
+    for (uint32_t j = 0; j < n; ++j) views = views || rs[j]->scene_src != nullptr;
+    std::unordered_map<const rast_mesh *, uint32_t> first;  // mesh -> first entry using it
+    uint32_t entry = 0;
     for (uint32_t j = 0; j < n; ++j) {
         const rast_renderer *r = rs[j];
         mix((uint64_t)r->W << 32 | r->H);
         mix((uint64_t)r->tile << 8 | (uint64_t)r->last_single << 4 | (uint64_t)job_learned(r) << 2 | job_shades(r));
         const std::vector<rast_mesh *> &ms = scene_meshes(const_cast<rast_renderer *>(r));
         mix(ms.size());
-        for (const rast_mesh *m : ms) mix((uint64_t)m->nv << 32 | m->ntri);
+        for (const rast_mesh *m : ms) {
+            mix((uint64_t)m->nv << 32 | m->ntri);
+            if (views) mix(first.emplace(m, entry).first->second);
+            ++entry;
+        }
     }
     return h;
 }
@@ -697,7 +707,11 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     P.block0.assign((size_t)n * ST_COUNT, 0);
     P.nblocks.assign((size_t)n * ST_COUNT, 0);
     std::vector<uint16_t> tables[ST_COUNT];
-    std::vector<uint32_t> vs_mesh, tri_mesh;
+    // vertex groups: the jobs' entries of one source mesh (views of one scene)
+    // share vertex blocks, which read each vertex once for all of them
+    std::vector<uint32_t> vs_group, tri_mesh, group_block0;
+    std::vector<std::vector<uint32_t>> members;
+    std::unordered_map<const rast_mesh *, uint32_t> group_of;
     P.vs_block0.clear();
     P.tri_block0.clear();
     uint64_t grid[ST_COUNT] = {};
@@ -725,9 +739,20 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         if (single) ++L.cap_jobs;
         else ++L.exact_jobs;
         for (rast_mesh *m : scene_meshes(r)) {
-            P.vs_block0.push_back((uint32_t)vs_mesh.size());
+            auto it = group_of.find(m);
+            uint32_t g;
+            if (it == group_of.end()) {
+                g = (uint32_t)members.size();
+                group_of.emplace(m, g);
+                members.emplace_back();
+                group_block0.push_back((uint32_t)vs_group.size());
+                vs_group.insert(vs_group.end(), ceil_div(m->nv, 256), g);
+            } else {
+                g = it->second;
+            }
+            members[g].push_back(mesh_entry);
+            P.vs_block0.push_back(group_block0[g]);
             P.tri_block0.push_back((uint32_t)tri_mesh.size());
-            vs_mesh.insert(vs_mesh.end(), ceil_div(m->nv, 256), mesh_entry);
             tri_mesh.insert(tri_mesh.end(), ceil_div(m->ntri, 256), mesh_entry);
             ++mesh_entry;
         }
@@ -735,21 +760,35 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     // a grid's threads must stay below 2^32 (256 threads per block in every stage but the scan)
     for (int st = 0; st < ST_COUNT; ++st)
         if (grid[st] >= (1ull << 24)) return rast_set_error(RAST_E_INVALID_ARG, "launch too large (stage %d)", st);
-    if (vs_mesh.size() >= (1ull << 24) || tri_mesh.size() >= (1ull << 24))
+    if (vs_group.size() >= (1ull << 24) || tri_mesh.size() >= (1ull << 24))
         return rast_set_error(RAST_E_INVALID_ARG, "launch too large (meshes)");
     for (int st = 0; st < ST_COUNT; ++st) L.grid[st] = (uint32_t)grid[st];
     P.njobs = n;
     P.nmeshes = mesh_entry;
-    L.vs_blocks = (uint32_t)vs_mesh.size();
+    L.vs_blocks = (uint32_t)vs_group.size();
     L.tri_blocks = (uint32_t)tri_mesh.size();
-    // device tables: [vs block -> mesh][tri block -> mesh], [stage tables]
-    RAST_TRY(dev_alloc(P.d_block_mesh, (vs_mesh.size() + tri_mesh.size() + 2) * 4));
+    // device tables: [vs block -> group][tri block -> mesh][groups (8-byte aligned)][group entries],
+    // [stage tables]
+    std::vector<uint32_t> tab(vs_group);
+    tab.insert(tab.end(), tri_mesh.begin(), tri_mesh.end());
+    if (tab.size() & 1) tab.push_back(0);
+    const size_t groups_at = tab.size();
+    tab.resize(groups_at + 2 * members.size());
+    uint32_t ne = 0;
+    for (size_t g = 0; g < members.size(); ++g) {
+        tab[groups_at + 2 * g] = ne;
+        tab[groups_at + 2 * g + 1] = (uint32_t)members[g].size();
+        ne += (uint32_t)members[g].size();
+    }
+    const size_t entries_at = tab.size();
+    for (const auto &mm : members) tab.insert(tab.end(), mm.begin(), mm.end());
+    RAST_TRY(dev_alloc(P.d_block_mesh, (tab.size() + 2) * 4));
     uint32_t *bm = (uint32_t *)P.d_block_mesh.p;
-    if (!vs_mesh.empty()) HIP_TRY(hipMemcpy(bm, vs_mesh.data(), vs_mesh.size() * 4, hipMemcpyHostToDevice));
-    if (!tri_mesh.empty())
-        HIP_TRY(hipMemcpy(bm + vs_mesh.size(), tri_mesh.data(), tri_mesh.size() * 4, hipMemcpyHostToDevice));
-    L.vs_block_mesh = bm;
-    L.tri_block_mesh = bm + vs_mesh.size();
+    if (!tab.empty()) HIP_TRY(hipMemcpy(bm, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+    L.vs_block_group = bm;
+    L.tri_block_mesh = bm + vs_group.size();
+    L.vs_groups = reinterpret_cast<const uint2 *>(bm + groups_at);
+    L.vs_entries = bm + entries_at;
     if (n > 1) {
         size_t total = 0;
         for (int st = 0; st < ST_COUNT; ++st) total += tables[st].size();

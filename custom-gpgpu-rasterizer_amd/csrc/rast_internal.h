@@ -46,7 +46,7 @@ struct DevMesh {
     uint32_t num_tris;
     uint32_t vert_base;       // first global vertex index
     uint32_t tri_base;        // first global triangle ID
-    uint32_t vs_block0;       // first block of this mesh in the vertex launch
+    uint32_t vs_block0;       // first block of this mesh's vertex group in the vertex launch
     uint32_t tri_block0;      // first block of this mesh in the triangle launch
     int32_t topology;
     int32_t has_diffuse;

@@ -390,63 +390,70 @@ __global__ __launch_bounds__(256) void k_clear_keys(unsigned long long *__restri
 
 // VS_Kernel / GetNDCVertex (CCR/CUDARenderer.cu:673-700, CCR/CUDAROPs.cu:99-118).
 __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ meshes,
-                                                const uint32_t *__restrict__ block_mesh,
+                                                const uint32_t *__restrict__ block_group,
+                                                const uint2 *__restrict__ groups,
+                                                const uint32_t *__restrict__ entries,
                                                 const JobDesc *__restrict__ jobs)
 {
-    const uint32_t mi = block_mesh[blockIdx.x];
-    const DevMesh &M = meshes[mi];
-    const JobDesc &J = jobs[M.job];
-    const FrameParams &fp = J.fp;
-    float4 *__restrict__ pos4 = J.pos4;
-    float4 *__restrict__ attr = J.attr;
-    uint32_t *__restrict__ density = J.density;
-    const uint32_t v = (blockIdx.x - M.vs_block0) * 256u + threadIdx.x;
-    if (v >= M.num_verts) return;
-    const float4 *iv = reinterpret_cast<const float4 *>(M.ivert + (size_t)v * 16);
+    // a group is one source mesh drawn by one or more jobs (views of one
+    // scene): the vertex is read once and transformed for each of them
+    const uint2 gr = groups[block_group[blockIdx.x]];
+    const DevMesh &M0 = meshes[entries[gr.x]];
+    const uint32_t v = (blockIdx.x - M0.vs_block0) * 256u + threadIdx.x;
+    if (v >= M0.num_verts) return;
+    const float4 *iv = reinterpret_cast<const float4 *>(M0.ivert + (size_t)v * 16);
     const float4 q0 = ldg(iv), q1 = ldg(iv + 1), q2 = ldg(iv + 2), q3 = ldg(iv + 3);
     const float px = q0.x, py = q0.y, pz = q0.z;
     const float u = q0.w, vv = q1.x;
     const float nx = q1.y, ny = q1.z, nz = q1.w;
     const float tx = q2.x, ty = q2.y, tz = q2.z;
     const float cr = q2.w, cg = q3.x, cb = q3.y;
-    const float *W = M.wvp, *Wd = M.world;
-    float o0 = m4(W, 0, 0) * px + m4(W, 0, 1) * py + m4(W, 0, 2) * pz + m4(W, 0, 3);
-    float o1 = m4(W, 1, 0) * px + m4(W, 1, 1) * py + m4(W, 1, 2) * pz + m4(W, 1, 3);
-    float o2 = m4(W, 2, 0) * px + m4(W, 2, 1) * py + m4(W, 2, 2) * pz + m4(W, 2, 3);
-    const float o3 = m4(W, 3, 0) * px + m4(W, 3, 1) * py + m4(W, 3, 2) * pz + m4(W, 3, 3);
-    o0 /= o3;
-    o1 /= o3;
-    o2 /= o3;
-    const uint32_t g = M.vert_base + v;
-    stg(pos4 + g, make_float4(o0, o1, o2, o3));
-    const float n0 = m4(Wd, 0, 0) * nx + m4(Wd, 0, 1) * ny + m4(Wd, 0, 2) * nz;
-    const float n1 = m4(Wd, 1, 0) * nx + m4(Wd, 1, 1) * ny + m4(Wd, 1, 2) * nz;
-    const float n2 = m4(Wd, 2, 0) * nx + m4(Wd, 2, 1) * ny + m4(Wd, 2, 2) * nz;
-    const float t0 = m4(Wd, 0, 0) * tx + m4(Wd, 0, 1) * ty + m4(Wd, 0, 2) * tz;
-    const float t1 = m4(Wd, 1, 0) * tx + m4(Wd, 1, 1) * ty + m4(Wd, 1, 2) * tz;
-    const float t2 = m4(Wd, 2, 0) * tx + m4(Wd, 2, 1) * ty + m4(Wd, 2, 2) * tz;
-    const float w0 = m4(Wd, 0, 0) * px + m4(Wd, 0, 1) * py + m4(Wd, 0, 2) * pz + m4(Wd, 0, 3);
-    const float w1 = m4(Wd, 1, 0) * px + m4(Wd, 1, 1) * py + m4(Wd, 1, 2) * pz + m4(Wd, 1, 3);
-    const float w2 = m4(Wd, 2, 0) * px + m4(Wd, 2, 1) * py + m4(Wd, 2, 2) * pz + m4(Wd, 2, 3);
-    float vx = w0 - M.cam_pos[0], vy = w1 - M.cam_pos[1], vz = w2 - M.cam_pos[2];
-    normalize3(vx, vy, vz);
-    // Every attribute is stored multiplied by 1/w: the interpolation of
-    // CCR/CUDAROPs.cu:347-380 forms w_k * (c_k * (1/w_k)) per pixel, and the
-    // product c_k * (1/w_k) is the same binary32 value whether it is formed
-    // here once or per pixel.  PBR shading reads the first three float4s only.
-    const float iw = 1.f / o3;
-    float4 *a = attr + (size_t)g * 4;
-    stg(a, make_float4(u * iw, vv * iw, iw, n0 * iw));
-    stg(a + 1, make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw));
-    stg(a + 2, make_float4(t2 * iw, vx * iw, vy * iw, vz * iw));
-    stg(a + 3, make_float4(cr * iw, cg * iw, cb * iw, 0.f));
-    if (density && (v & 7u) == 0) {
-        // projected-vertex density feature of the capacity model (every 8th
-        // vertex; no parity role)
-        const float sx = ((o0 + 1) / 2) * (float)fp.width;
-        const float sy = ((1 - o1) / 2) * (float)fp.height;
-        if (o3 > 0.f && sx >= 0.f && sy >= 0.f && sx < (float)fp.width && sy < (float)fp.height)
-            count_tile(density, ((uint32_t)sy / fp.tile) * fp.tiles_x + (uint32_t)sx / fp.tile);
+    for (uint32_t e = 0; e < gr.y; ++e) {
+        const DevMesh &M = meshes[entries[gr.x + e]];
+        const JobDesc &J = jobs[M.job];
+        const FrameParams &fp = J.fp;
+        float4 *__restrict__ pos4 = J.pos4;
+        float4 *__restrict__ attr = J.attr;
+        uint32_t *__restrict__ density = J.density;
+        const float *W = M.wvp, *Wd = M.world;
+        float o0 = m4(W, 0, 0) * px + m4(W, 0, 1) * py + m4(W, 0, 2) * pz + m4(W, 0, 3);
+        float o1 = m4(W, 1, 0) * px + m4(W, 1, 1) * py + m4(W, 1, 2) * pz + m4(W, 1, 3);
+        float o2 = m4(W, 2, 0) * px + m4(W, 2, 1) * py + m4(W, 2, 2) * pz + m4(W, 2, 3);
+        const float o3 = m4(W, 3, 0) * px + m4(W, 3, 1) * py + m4(W, 3, 2) * pz + m4(W, 3, 3);
+        o0 /= o3;
+        o1 /= o3;
+        o2 /= o3;
+        const uint32_t g = M.vert_base + v;
+        stg(pos4 + g, make_float4(o0, o1, o2, o3));
+        const float n0 = m4(Wd, 0, 0) * nx + m4(Wd, 0, 1) * ny + m4(Wd, 0, 2) * nz;
+        const float n1 = m4(Wd, 1, 0) * nx + m4(Wd, 1, 1) * ny + m4(Wd, 1, 2) * nz;
+        const float n2 = m4(Wd, 2, 0) * nx + m4(Wd, 2, 1) * ny + m4(Wd, 2, 2) * nz;
+        const float t0 = m4(Wd, 0, 0) * tx + m4(Wd, 0, 1) * ty + m4(Wd, 0, 2) * tz;
+        const float t1 = m4(Wd, 1, 0) * tx + m4(Wd, 1, 1) * ty + m4(Wd, 1, 2) * tz;
+        const float t2 = m4(Wd, 2, 0) * tx + m4(Wd, 2, 1) * ty + m4(Wd, 2, 2) * tz;
+        const float w0 = m4(Wd, 0, 0) * px + m4(Wd, 0, 1) * py + m4(Wd, 0, 2) * pz + m4(Wd, 0, 3);
+        const float w1 = m4(Wd, 1, 0) * px + m4(Wd, 1, 1) * py + m4(Wd, 1, 2) * pz + m4(Wd, 1, 3);
+        const float w2 = m4(Wd, 2, 0) * px + m4(Wd, 2, 1) * py + m4(Wd, 2, 2) * pz + m4(Wd, 2, 3);
+        float vx = w0 - M.cam_pos[0], vy = w1 - M.cam_pos[1], vz = w2 - M.cam_pos[2];
+        normalize3(vx, vy, vz);
+        // Every attribute is stored multiplied by 1/w: the interpolation of
+        // CCR/CUDAROPs.cu:347-380 forms w_k * (c_k * (1/w_k)) per pixel, and the
+        // product c_k * (1/w_k) is the same binary32 value whether it is formed
+        // here once or per pixel.  PBR shading reads the first three float4s only.
+        const float iw = 1.f / o3;
+        float4 *a = attr + (size_t)g * 4;
+        stg(a, make_float4(u * iw, vv * iw, iw, n0 * iw));
+        stg(a + 1, make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw));
+        stg(a + 2, make_float4(t2 * iw, vx * iw, vy * iw, vz * iw));
+        stg(a + 3, make_float4(cr * iw, cg * iw, cb * iw, 0.f));
+        if (density && (v & 7u) == 0) {
+            // projected-vertex density feature of the capacity model (every 8th
+            // vertex; no parity role)
+            const float sx = ((o0 + 1) / 2) * (float)fp.width;
+            const float sy = ((1 - o1) / 2) * (float)fp.height;
+            if (o3 > 0.f && sx >= 0.f && sy >= 0.f && sx < (float)fp.width && sy < (float)fp.height)
+                count_tile(density, ((uint32_t)sy / fp.tile) * fp.tiles_x + (uint32_t)sx / fp.tile);
+        }
     }
 }
 
@@ -1595,7 +1602,8 @@ hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s)
 hipError_t launch_vertex(const Launch &L, hipStream_t s)
 {
     if (L.vs_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_vertex, dim3(L.vs_blocks), dim3(256), 0, s, L.meshes, L.vs_block_mesh, L.jobs);
+    hipLaunchKernelGGL(k_vertex, dim3(L.vs_blocks), dim3(256), 0, s, L.meshes, L.vs_block_group, L.vs_groups,
+                       L.vs_entries, L.jobs);
     return hipGetLastError();
 }
 

@@ -16,7 +16,9 @@ struct Launch {
     const JobDesc *jobs;              // device, njobs entries
     uint32_t njobs;
     const DevMesh *meshes;            // device, all jobs' meshes
-    const uint32_t *vs_block_mesh;    // vertex block -> mesh entry
+    const uint32_t *vs_block_group;   // vertex block -> vertex group
+    const uint2 *vs_groups;           // group -> (first index into vs_entries, count)
+    const uint32_t *vs_entries;       // the mesh entries of each group (one source mesh, one entry per job)
     const uint32_t *tri_block_mesh;   // triangle block -> mesh entry
     uint32_t vs_blocks, tri_blocks;
     const uint16_t *table[ST_COUNT];  // device block -> job tables (null when njobs == 1)
