@@ -1134,9 +1134,11 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             off[tid] = before + incl - area;
             if (whole) wlist[pbefore + popc_below(wm)] = tid;
             __syncthreads();
-            // compacted items
+            // compacted items; slots past this chunk's triangles start at
+            // nitems, so the owner search stops at the last staged one
+            const uint32_t n_here = n_tile - base < (uint32_t)NT ? n_tile - base : (uint32_t)NT;
             for (uint32_t kk = tid; kk < nitems; kk += NT) {
-                const uint32_t j = find_item_owner(off, NT, kk);
+                const uint32_t j = find_item_owner(off, n_here, kk);
                 const TriLds &mt = stage[j];
                 int px, py;
                 item_pixel(mt, kk - off[j], px, py);
