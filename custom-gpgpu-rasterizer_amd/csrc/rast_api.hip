@@ -728,7 +728,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         nb[ST_FBALLOC] = single ? ceil_div(ntiles, 256) : 0;
         nb[ST_FBFILL] = single ? std::min<uint32_t>(ceil_div(ntris, 256), 256) : 0;
         const int res = r->tile == 8 ? ST_RES8 : (r->tile == 16 ? ST_RES16 : ST_RES32);
-        nb[res] = ceil_div(ntiles, 8) * 8;  // multiple of 8: the XCD order holds inside the job
+        nb[res] = ntiles;
         nb[ST_SHADE] = job_shades(r) ? ceil_div(r->W, 64) * ceil_div(r->H, 4) : 0;
         for (int st = 0; st < ST_COUNT; ++st) {
             P.block0[(size_t)j * ST_COUNT + st] = (uint32_t)grid[st];

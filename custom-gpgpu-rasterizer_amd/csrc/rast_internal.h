@@ -84,7 +84,7 @@ enum Stage : int {
     ST_FILL,      // exact second pass (grid-stride over binned triangles)
     ST_FBALLOC,   // single-pass fallback allocation (256 tiles per block)
     ST_FBFILL,    // single-pass fallback placement (grid-stride over entries)
-    ST_RES8,      // resolve, 8x8 tiles (one block per tile, XCD-ordered)
+    ST_RES8,      // resolve, 8x8 tiles (one block per tile, in launch order)
     ST_RES16,     // resolve, 16x16 tiles
     ST_RES32,     // resolve, 32x32 tiles
     ST_SHADE,     // 64x4-pixel blocks

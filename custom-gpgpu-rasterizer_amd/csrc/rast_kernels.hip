@@ -129,29 +129,6 @@ __device__ __forceinline__ void gatomic_min(unsigned long long *p, unsigned long
     __hip_atomic_fetch_min(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Order-free append of one element per thread with pred set, one atomic per
-// 256-thread block.  Every thread of the block must call it.
-__device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool pred, uint32_t *lds)
-{
-    const uint64_t m = __ballot(pred);
-    const uint32_t wave = threadIdx.x >> 6;
-    if (lane_id() == 0) lds[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t a = lds[0], b = lds[1], c = lds[2], d = lds[3];
-        const uint32_t tot = a + b + c + d;
-        const uint32_t base = tot ? gatomic_add(counter, tot) : 0u;
-        lds[0] = base;
-        lds[1] = base + a;
-        lds[2] = base + a + b;
-        lds[3] = base + a + b + c;
-    }
-    __syncthreads();
-    const uint32_t slot = lds[wave] + popc_below(m);
-    __syncthreads();
-    return slot;
-}
-
 // Groups the active lanes of the wave by tile: returns the mask of lanes with
 // the same t (the lane's group).  One ballot per distinct tile, no atomics.
 __device__ __forceinline__ uint64_t match_tile(uint32_t t)
@@ -169,7 +146,9 @@ __device__ __forceinline__ uint64_t match_tile(uint32_t t)
     return group;
 }
 
-// block_append for three predicates at once (two barriers).
+// Order-free append for three predicates at once: every thread with a
+// predicate set gets a slot of its counter, one atomic per counter per
+// 256-thread block (two barriers; every thread of the block must call it).
 __device__ __forceinline__ void block_append3(uint32_t *c0, bool p0, uint32_t *c1, bool p1, uint32_t *c2, bool p2,
                                               uint32_t *lds, uint32_t &s0, uint32_t &s1, uint32_t &s2)
 {
@@ -1020,7 +999,6 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     float *__restrict__ depth = J.depth;
     float *__restrict__ tile_min_depth = J.tile_min_depth;
     const uint32_t small_limit = J.small_limit;
-    const uint32_t grid_tiles = J.nblocks[STAGE];
     constexpr int NPIX = T * T;
     constexpr int PPT = NPIX / NT;
     constexpr int NW = NT / 64;
@@ -1037,9 +1015,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     const uint32_t ntiles = fp.tiles_x * fp.tiles_y;
     // Tiles in launch order: the hardware spreads consecutive workgroups over
     // the 8 XCDs, so neighbouring tiles run on different XCDs at once.  Giving
-    // each XCD a contiguous band instead measured 30% slower here (the bands'
-    // key, vis and depth traffic then concentrates on fewer memory channels).
-    (void)grid_tiles;
+    // each XCD a contiguous band instead measured 30% slower here.
     const uint32_t tile = local;
     if (tile >= ntiles) return;
     const uint32_t txi = tile % fp.tiles_x, tyi = tile / fp.tiles_x;
