@@ -246,8 +246,35 @@ struct TriLds {
                       // screen-list triangles: 1 / area
 };
 
+// Setup's micro triangles in LDS: TriLds without the edge vectors, which are
+// recomputed (the same binary32 differences) where they are needed.  56 bytes
+// instead of 80 leave room for eight waves per SIMD in the setup pass.
+struct MicroLds {
+    float sx0, sy0, sx1, sy1, sx2, sy2;
+    float z0, z1, z2, area;
+    uint32_t xr, yr;  // x0 | x1 << 16, y0 | y1 << 16 (absolute pixels)
+    uint32_t id;
+    float inv;        // 1 / (x1 - x0), for item -> pixel decoding
+};
+
+__device__ __forceinline__ void edge_vectors(const TriLds &t, float &ax, float &ay, float &bx, float &by, float &cx,
+                                             float &cy)
+{
+    ax = t.ax; ay = t.ay; bx = t.bx; by = t.by; cx = t.cx; cy = t.cy;
+}
+
+// The same differences as set_edges.
+__device__ __forceinline__ void edge_vectors(const MicroLds &t, float &ax, float &ay, float &bx, float &by, float &cx,
+                                             float &cy)
+{
+    ax = t.sx1 - t.sx0; ay = t.sy1 - t.sy0;
+    bx = t.sx2 - t.sx1; by = t.sy2 - t.sy1;
+    cx = t.sx0 - t.sx2; cy = t.sy0 - t.sy2;
+}
+
 // Item q of a triangle's pixel range [x0,x1) x [y0,y1), row-major.
-__device__ __forceinline__ void item_pixel(const TriLds &t, uint32_t q, int &px, int &py)
+template <class Tri>
+__device__ __forceinline__ void item_pixel(const Tri &t, uint32_t q, int &px, int &py)
 {
     const int x0 = (int)(t.xr & 0xFFFFu), x1 = (int)(t.xr >> 16), y0 = (int)(t.yr & 0xFFFFu);
     const int w = x1 - x0;
@@ -272,16 +299,20 @@ __device__ __forceinline__ uint32_t find_item_owner(const uint32_t *off, uint32_
 }
 
 // The three edge functions of CCR/CUDAROPs.cu:120-145 at pixel (px, py).
-__device__ __forceinline__ void edges(const TriLds &t, float px, float py, float &w0, float &w1, float &w2)
+template <class Tri>
+__device__ __forceinline__ void edges(const Tri &t, float px, float py, float &w0, float &w1, float &w2)
 {
-    w2 = (px - t.sx0) * t.ay - (py - t.sy0) * t.ax;
-    w0 = (px - t.sx1) * t.by - (py - t.sy1) * t.bx;
-    w1 = (px - t.sx2) * t.cy - (py - t.sy2) * t.cx;
+    float ax, ay, bx, by, cx, cy;
+    edge_vectors(t, ax, ay, bx, by, cx, cy);
+    w2 = (px - t.sx0) * ay - (py - t.sy0) * ax;
+    w0 = (px - t.sx1) * by - (py - t.sy1) * bx;
+    w1 = (px - t.sx2) * cy - (py - t.sy2) * cx;
 }
 
 // Depth of RasterizePixelAtomic (CCR/CUDAROPs.cu:329-345) with the D2 window:
 // weights /= |area|, z = w0 z0 + w1 z1 + w2 z2, accept 0 <= z <= 1.
-__device__ __forceinline__ bool depth_of(const TriLds &t, float w0, float w1, float w2, float &z)
+template <class Tri>
+__device__ __forceinline__ bool depth_of(const Tri &t, float w0, float w1, float w2, float &z)
 {
     const float a0 = w0 / t.area, a1 = w1 / t.area, a2 = w2 / t.area;
     const float zi = (a0 * t.z0) + (a1 * t.z1) + (a2 * t.z2);
@@ -291,7 +322,8 @@ __device__ __forceinline__ bool depth_of(const TriLds &t, float w0, float w1, fl
 }
 
 // Fragment test: all three edge functions >= 0, then the depth.
-__device__ __forceinline__ bool fragment(const TriLds &t, float px, float py, float &z)
+template <class Tri>
+__device__ __forceinline__ bool fragment(const Tri &t, float px, float py, float &z)
 {
     float w0, w1, w2;
     edges(t, px, py, w0, w1, w2);
@@ -445,7 +477,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                                                const JobDesc *__restrict__ jobs, uint32_t nblocks)
 {
     __shared__ uint32_t lds_append[12];
-    __shared__ TriLds mstage[256];
+    __shared__ MicroLds mstage[256];
     __shared__ uint32_t moff[256];
     __shared__ unsigned long long mmask[4 * kMaskChunks];
     // workgroups in launch order (consecutive blocks spread over the XCDs;
@@ -571,16 +603,18 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
             }
             const uint32_t total = __shfl(incl, 63);
             const uint32_t start = incl - area;
-            TriLds *ws = mstage + wave * 64;
+            MicroLds *ws = mstage + wave * 64;
             uint32_t *wo = moff + wave * 64;
             unsigned long long *wm = mmask + wave * kMaskChunks;
             if (micro) {
-                set_edges(t);
-                t.xr = (uint32_t)xmin | ((uint32_t)xmax << 16);
-                t.yr = (uint32_t)ymin | ((uint32_t)ymax << 16);
-                t.id = gid;
-                t.inv = 1.0f / (float)(xmax - xmin);
-                ws[rank] = t;
+                MicroLds m;
+                m.sx0 = t.sx0; m.sy0 = t.sy0; m.sx1 = t.sx1; m.sy1 = t.sy1; m.sx2 = t.sx2; m.sy2 = t.sy2;
+                m.z0 = t.z0; m.z1 = t.z1; m.z2 = t.z2; m.area = t.area;
+                m.xr = (uint32_t)xmin | ((uint32_t)xmax << 16);
+                m.yr = (uint32_t)ymin | ((uint32_t)ymax << 16);
+                m.id = gid;
+                m.inv = 1.0f / (float)(xmax - xmin);
+                ws[rank] = m;
                 wo[rank] = start;
             }
             uint32_t before = 0;  // triangles starting before the current chunk (wave-uniform)
@@ -601,7 +635,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                     const uint32_t kk = wbase + c * 64u + lane;
                     const uint32_t j = before + (uint32_t)__popcll(m & ((2ull << lane) - 1ull)) - 1u;
                     if (kk < total) {
-                        const TriLds &mt = ws[j];
+                        const MicroLds &mt = ws[j];
                         int px, py;
                         item_pixel(mt, kk - wo[j], px, py);
                         float z;
