@@ -418,6 +418,10 @@ def main():
     if "prevframe" in results and "learned" in results:
         line["learned_over_prevframe"] = round(
             results["prevframe"]["ms_per_frame"] / results["learned"]["ms_per_frame"], 3)
+    if plain and "learned" in results:
+        # the learned single-pass frame against the plain exact two-pass binner
+        # (every triangle binned, no setup-time raster, default tile)
+        line["learned_over_plain_exact"] = round(plain["exact_ms_per_frame"] / results["learned"]["ms_per_frame"], 3)
     if roofline:
         line["shading_gpix_per_s"] = round(P / (results[head]["stages_ms"]["ms_shade"] * 1e-3) / 1e9, 2)
     print(json.dumps(line))
