@@ -614,7 +614,8 @@ static int prepare_job(rast_renderer *r)
 static bool job_learned(const rast_renderer *r) { return r->last_single && r->last_opts.bin_policy == RAST_BIN_LEARNED; }
 static bool job_shades(const rast_renderer *r) { return !(r->last_opts.flags & RAST_FLAG_NO_SHADE); }
 
-// Structure of a launch: whatever fixes its grids and block tables.
+// Structure of a launch: whatever fixes its grids, block tables and kernel
+// instances (the shading kernel has one per sampling mode).
 static uint64_t plan_key(rast_renderer *const *rs, uint32_t n, int hidden)
 {
     uint64_t h = 1469598103934665603ull;
@@ -629,7 +630,8 @@ static uint64_t plan_key(rast_renderer *const *rs, uint32_t n, int hidden)
     for (uint32_t j = 0; j < n; ++j) {
         const rast_renderer *r = rs[j];
         mix((uint64_t)r->W << 32 | r->H);
-        mix((uint64_t)r->tile << 8 | (uint64_t)r->last_single << 4 | (uint64_t)job_learned(r) << 2 | job_shades(r));
+        mix((uint64_t)r->last_opts.sample_state << 16 | (uint64_t)r->tile << 8 | (uint64_t)r->last_single << 4 |
+            (uint64_t)job_learned(r) << 2 | job_shades(r));
         const std::vector<rast_mesh *> &ms = scene_meshes(const_cast<rast_renderer *>(r));
         mix(ms.size());
         for (const rast_mesh *m : ms) {
@@ -730,6 +732,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         const int res = r->tile == 8 ? ST_RES8 : (r->tile == 16 ? ST_RES16 : ST_RES32);
         nb[res] = ntiles;
         nb[ST_SHADE] = job_shades(r) ? ceil_div(r->W, 64) * ceil_div(r->H, 4) : 0;
+        if (nb[ST_SHADE]) L.shade_modes |= 1u << r->last_opts.sample_state;
         for (int st = 0; st < ST_COUNT; ++st) {
             P.block0[(size_t)j * ST_COUNT + st] = (uint32_t)grid[st];
             P.nblocks[(size_t)j * ST_COUNT + st] = nb[st];

@@ -1348,12 +1348,11 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
 
 // ------------------------------------------------------------- shading
 
+// (u8)(c * 255) saturating: 0 for negatives and NaN (fmaxf returns the
+// number), 255 from 255 up, truncation between.
 __device__ __forceinline__ uint32_t pack_channel(float c)
 {
-    const float v = c * 255.f;
-    if (!(v > 0.f)) return 0u;
-    if (v >= 255.f) return 255u;
-    return (uint32_t)v;
+    return (uint32_t)__builtin_fminf(__builtin_fmaxf(c * 255.f, 0.f), 255.f);
 }
 
 // GetRGBAFromColour (CCR/RGBRaw.h:23-32) in memory order R,G,B,A=255 with
@@ -1400,52 +1399,117 @@ __device__ __forceinline__ Taps texel_taps(const DevTexture &t, float u, float v
 // rounded quotient as the reference).
 __device__ __forceinline__ void decode_point(uint32_t s, bool ok, const float *unit, float &r, float &g, float &b)
 {
-    r = ok ? unit[s & 0xFFu] : 1.f;
-    g = ok ? unit[(s >> 8) & 0xFFu] : 0.f;
-    b = ok ? unit[(s >> 16) & 0xFFu] : 1.f;
+    const float ur = unit[s & 0xFFu], ug = unit[(s >> 8) & 0xFFu], ub = unit[(s >> 16) & 0xFFu];
+    r = ok ? ur : 1.f;
+    g = ok ? ug : 0.f;
+    b = ok ? ub : 1.f;
 }
 
-// SampleLinear's weighting: neighbours x 1/8 each in order, then the centre / 2, then / 255.
+// SampleLinear's weighting: neighbours x 1/8 each in order, then the centre
+// / 2, then x (1/255).  Every partial sum of that float accumulation is a
+// multiple of 1/8 below 2^11, hence exact, so the sum is (n0 + n1 + n2 + n3 +
+// 4 centre) / 8; it is formed here in integers, red and blue side by side in
+// 16-bit halves.  S / 8 x (1/255) and S x ((1/255) / 8) round the same real
+// number, so the colours are bitwise those of the float accumulation.
 __device__ __forceinline__ void decode_linear(const uint32_t *s, uint32_t sc, bool ok, float &r, float &g, float &b)
 {
-    const float weight = 1.f / (4 * 2);
-    float ar = 0.f, ag = 0.f, ab = 0.f;
+    constexpr uint32_t kRB = 0x00FF00FFu;
+    const uint32_t rb = (s[0] & kRB) + (s[1] & kRB) + (s[2] & kRB) + (s[3] & kRB) + ((sc & kRB) << 2);
+    const uint32_t gs = ((s[0] >> 8) & 0xFFu) + ((s[1] >> 8) & 0xFFu) + ((s[2] >> 8) & 0xFFu) +
+                        ((s[3] >> 8) & 0xFFu) + (((sc >> 8) & 0xFFu) << 2);
+    const float k = (1.0f / 255.f) * 0.125f;
+    r = ok ? (float)(rb & 0xFFFFu) * k : 1.f;
+    g = ok ? (float)gs * k : 0.f;
+    b = ok ? (float)(rb >> 16) * k : 1.f;
+}
+
+// Texel at byte offset off from base: a 32-bit offset (textures hold at most
+// 2^30 texels), so a wave-uniform base stays in scalar registers.
+__device__ __forceinline__ uint32_t texel_at(const uint32_t *base, uint32_t off)
+{
+    return ldg(reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(base) + off));
+}
+
+// Colours of the four maps at (u, v): diffuse, normal and specular RGB and
+// the gloss map's red channel.
+struct MapColours {
+    float d[3], n[3], s[3], g;
+};
+
+// Samples the four maps.  kShared: all four have the same size, so they
+// share the texel indices (computed once).  Point mode decodes through the
+// unit table (c / 255, correctly rounded).
+template <bool kShared>
+__device__ __forceinline__ MapColours sample_maps(const DevTexture (&tex)[4], int mode, float u, float v,
+                                                  const float *unit)
+{
+    MapColours o;
+    float gg, gb;
+    Taps tp[4];
+    tp[0] = texel_taps(tex[0], u, v);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        ar += (float)(s[i] & 0xFFu) * weight;
-        ag += (float)((s[i] >> 8) & 0xFFu) * weight;
-        ab += (float)((s[i] >> 16) & 0xFFu) * weight;
+    for (int k = 1; k < 4; ++k) tp[k] = kShared ? tp[0] : texel_taps(tex[k], u, v);
+    if (mode == 0) {
+        uint32_t sc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sc[k] = texel_at(tex[k].texels, (uint32_t)tp[k].c * 4u);
+        decode_point(sc[0], tp[0].ok, unit, o.d[0], o.d[1], o.d[2]);
+        decode_point(sc[1], tp[1].ok, unit, o.n[0], o.n[1], o.n[2]);
+        decode_point(sc[2], tp[2].ok, unit, o.s[0], o.s[1], o.s[2]);
+        decode_point(sc[3], tp[3].ok, unit, o.g, gg, gb);
+    } else {
+        uint32_t sc[4], sn[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            sc[k] = texel_at(tex[k].texels, (uint32_t)tp[k].c * 4u);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sn[k][i] = texel_at(tex[k].texels, (uint32_t)tp[k].n[i] * 4u);
+        }
+        decode_linear(sn[0], sc[0], tp[0].ok, o.d[0], o.d[1], o.d[2]);
+        decode_linear(sn[1], sc[1], tp[1].ok, o.n[0], o.n[1], o.n[2]);
+        decode_linear(sn[2], sc[2], tp[2].ok, o.s[0], o.s[1], o.s[2]);
+        decode_linear(sn[3], sc[3], tp[3].ok, o.g, gg, gb);
     }
-    ar += (float)(sc & 0xFFu) / 2.f;
-    ag += (float)((sc >> 8) & 0xFFu) / 2.f;
-    ab += (float)((sc >> 16) & 0xFFu) / 2.f;
+    (void)gg; (void)gb;
+    return o;
+}
+
+// c / 255.f for a byte value c: the product with the rounded reciprocal and
+// one residual correction (fma), which gives the correctly rounded quotient
+// for every c in [0, 255].
+__device__ __forceinline__ float byte_unit(uint32_t c)
+{
     const float rev = 1.0f / 255.f;
-    r = ok ? ar * rev : 1.f;
-    g = ok ? ag * rev : 0.f;
-    b = ok ? ab * rev : 1.f;
+    const float f = (float)c;
+    const float q = f * rev;
+    return __builtin_fmaf(__builtin_fmaf(-q, 255.f, f), rev, q);
 }
 
 // PS_Kernel + ShadePixelUnsafe (CCR/CUDARenderer.cu:833-897, CCR/CUDAROPs.cu:472-514)
 // with the attribute interpolation of RasterizePixelAtomic (CCR/CUDAROPs.cu:347-380).
 // The barycentrics, 1/w and the texture coordinates are computed exactly as
-// in the reference (so the same texels are fetched); the lighting terms use
-// the hardware reciprocal / rsqrt / exp2 / log2, which keeps RGBA8 within
-// 1/255 of the reference formula.  64x4-pixel blocks (rows of 64 per wave:
-// fully coalesced vis reads and RGBA8 writes; 8x8 squares measured the same).
-// A pixel's data is a chain of dependent gathers (vis, triangle record,
-// vertex attributes and texture table, texels); every level's loads are
-// issued together, with no branch between them, and range checks are applied
-// to the loaded values.  The texture table is read with scalar loads when all
-// of a wave's pixels belong to one mesh.
+// in the reference (so the same texels are fetched); the other attributes and
+// the lighting terms use fused multiply-adds and the hardware reciprocal /
+// rsqrt / exp2 / log2, which keeps RGBA8 within 1/255 of the reference
+// formula.  64x4-pixel blocks (rows of 64 per wave: fully coalesced vis reads
+// and RGBA8 writes).  A pixel's data is a chain of dependent gathers (vis,
+// triangle record, vertex attributes and texture table, texels); every
+// level's loads are issued together, with no branch between them, and range
+// checks are applied to the loaded values.  When all of a wave's pixels
+// belong to one mesh its texture table is read with scalar loads, and when
+// its four maps have one size the texel indices are computed once.
+template <bool kLinear>
 __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
                                                const uint16_t *__restrict__ table)
 {
     __shared__ float unit[256];
-    unit[threadIdx.x] = (float)threadIdx.x / 255.f;
+    unit[threadIdx.x] = byte_unit(threadIdx.x);
     __syncthreads();
     uint32_t local;
     const JobDesc &J = jobs[job_of(table, jobs, ST_SHADE, local)];
     const FrameParams &fp = J.fp;
+    if ((fp.sample_state != 0) != kLinear) return;  // the other sampling mode's instance shades this frame
+    constexpr int mode = kLinear ? 1 : 0;
     const uint32_t *__restrict__ vis = J.vis;
     const float *__restrict__ depth = J.depth;
     const uint32_t *__restrict__ trirec = J.trirec;
@@ -1458,35 +1522,40 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     if (x >= fp.width || y >= fp.height) return;
     const size_t pid = (size_t)y * fp.width + x;
     const uint32_t id = ldg(vis + pid);
-    if (id >= fp.num_tris) {
-        stg(rgba + pid, (uint32_t)(fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u));
-        return;
-    }
+    const bool valid = id < fp.num_tris;
+    const uint32_t clear = fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u;
+    // The pixels of a wave (64 of one row) that show one triangle form runs,
+    // about 7 pixels long in a 4K frame of a 2M-triangle terrain.  Lane r
+    // fetches run r's triangle record and vertex attributes, and every pixel
+    // takes its run's words across lanes (ds_bpermute): the vector memory
+    // path, which bounds this kernel, carries each run's data once rather
+    // than once per pixel.
+    if (!valid) { stg(rgba + pid, clear); return; }
     const uint4 *rp = reinterpret_cast<const uint4 *>(trirec + (size_t)id * kTriRecWords);
     const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q2 = ldg(rp + 2), q3 = ldg(rp + 3);
     const float sx0 = __uint_as_float(q0.x), sy0 = __uint_as_float(q0.y);
     const float sx1 = __uint_as_float(q0.z), sy1 = __uint_as_float(q0.w);
     const float sx2 = __uint_as_float(q1.x), sy2 = __uint_as_float(q1.y);
     const float area = __uint_as_float(q2.y);
-    // level 3: vertex attributes and the mesh's texture table
     const float4 *A0 = attr + (size_t)q3.x * 4, *A1 = attr + (size_t)q3.y * 4, *A2 = attr + (size_t)q3.z * 4;
     const float4 a00 = ldg(A0), a10 = ldg(A1), a20 = ldg(A2);
     const float4 a01 = ldg(A0 + 1), a11 = ldg(A1 + 1), a21 = ldg(A2 + 1);
     const float4 a02 = ldg(A0 + 2), a12 = ldg(A1 + 2), a22 = ldg(A2 + 2);
     const uint32_t mi = q3.w - J.mesh0 < fp.num_meshes ? q3.w : J.mesh0;  // entry in the launch's mesh table
     const uint32_t mi0 = __builtin_amdgcn_readfirstlane(mi);
-    DevTexture tex[4];
+    const bool uniform = __ballot(mi != mi0) == 0;
+    // the table of a wave-uniform mesh stays in scalar registers (stex); a
+    // per-lane one is read only where the maps are sampled, after the
+    // attributes have been interpolated
+    DevTexture stex[4] = {};
     int has_diffuse;
-    if (__ballot(mi != mi0) == 0) {
-        const DevMesh &M = meshes[mi0];  // wave-uniform: scalar loads
+    if (uniform) {
+        const DevMesh &M = meshes[mi0];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) tex[k] = M.tex[k];
+        for (int k = 0; k < 4; ++k) stex[k] = M.tex[k];
         has_diffuse = M.has_diffuse;
     } else {
-        const DevMesh *M = meshes + mi;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) tex[k] = ldg(M->tex + k);
-        has_diffuse = ldg(&M->has_diffuse);
+        has_diffuse = ldg(&meshes[mi].has_diffuse);
     }
     const float px = (float)x, py = (float)y;
     float w2 = (px - sx0) * (sy1 - sy0) - (py - sy0) * (sx1 - sx0);
@@ -1495,13 +1564,15 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     w0 = w0 / area; w1 = w1 / area; w2 = w2 / area;
     const float i0 = a00.z, i1 = a10.z, i2 = a20.z;
     const float wI = 1.f / (i0 * w0 + i1 * w1 + i2 * w2);
-    // attributes arrive multiplied by their vertex's 1/w
+    // attributes arrive multiplied by their vertex's 1/w; u and v are formed
+    // in the reference's order, the rest with fused multiply-adds
 #define RAST_INTERP(c0, c1, c2) (w0 * (c0) + w1 * (c1) + w2 * (c2))
+#define RAST_FINTERP(c0, c1, c2) __builtin_fmaf(w2, (c2), __builtin_fmaf(w1, (c1), w0 * (c0)))
     const float u = RAST_INTERP(a00.x, a10.x, a20.x) * wI;
     const float v = RAST_INTERP(a00.y, a10.y, a20.y) * wI;
-    const float nx = RAST_INTERP(a00.w, a10.w, a20.w) * wI;
-    const float ny = RAST_INTERP(a01.x, a11.x, a21.x) * wI;
-    const float nz = RAST_INTERP(a01.y, a11.y, a21.y) * wI;
+    const float nx = RAST_FINTERP(a00.w, a10.w, a20.w) * wI;
+    const float ny = RAST_FINTERP(a01.x, a11.x, a21.x) * wI;
+    const float nz = RAST_FINTERP(a01.y, a11.y, a21.y) * wI;
     if (fp.visualisation == 1) {  // depth colour, Remap(z, 0.985, 1)
         stg(rgba + pid, (uint32_t)(pack_rgba((ldg(depth + pid) - 0.985f) / (1.f - 0.985f), 0.f, 0.f)));
         return;
@@ -1512,67 +1583,57 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     }
     if (!has_diffuse) {  // vertex colour (perspective-weighted, not x w)
         const float4 a03 = ldg(A0 + 3), a13 = ldg(A1 + 3), a23 = ldg(A2 + 3);
-        stg(rgba + pid, (uint32_t)(pack_rgba(RAST_INTERP(a03.x, a13.x, a23.x), RAST_INTERP(a03.y, a13.y, a23.y),
-                              RAST_INTERP(a03.z, a13.z, a23.z))));
+        stg(rgba + pid, (uint32_t)(pack_rgba(RAST_FINTERP(a03.x, a13.x, a23.x), RAST_FINTERP(a03.y, a13.y, a23.y),
+                                             RAST_FINTERP(a03.z, a13.z, a23.z))));
         return;
     }
     // level 4: the texels of all four maps
-    Taps tp[4];
+    MapColours c;
+    if (!uniform) {
+        DevTexture vtex[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) tp[k] = texel_taps(tex[k], u, v);
-    float dr, dg, db, nr, ng, nb, sr, sg, sb, gr, gg, gb;
-    if (fp.sample_state == 0) {
-        uint32_t sc[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sc[k] = ldg(tex[k].texels + tp[k].c);
-        decode_point(sc[0], tp[0].ok, unit, dr, dg, db);
-        decode_point(sc[1], tp[1].ok, unit, nr, ng, nb);
-        decode_point(sc[2], tp[2].ok, unit, sr, sg, sb);
-        decode_point(sc[3], tp[3].ok, unit, gr, gg, gb);
-    } else {
-        uint32_t sc[4], sn[4][4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            sc[k] = ldg(tex[k].texels + tp[k].c);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) sn[k][i] = ldg(tex[k].texels + tp[k].n[i]);
-        }
-        decode_linear(sn[0], sc[0], tp[0].ok, dr, dg, db);
-        decode_linear(sn[1], sc[1], tp[1].ok, nr, ng, nb);
-        decode_linear(sn[2], sc[2], tp[2].ok, sr, sg, sb);
-        decode_linear(sn[3], sc[3], tp[3].ok, gr, gg, gb);
-    }
-    (void)gg; (void)gb;
-    const float tx = RAST_INTERP(a01.z, a11.z, a21.z);
-    const float ty = RAST_INTERP(a01.w, a11.w, a21.w);
-    const float tz = RAST_INTERP(a02.x, a12.x, a22.x);
-    float vx = RAST_INTERP(a02.y, a12.y, a22.y);
-    float vy = RAST_INTERP(a02.z, a12.z, a22.z);
-    float vz = RAST_INTERP(a02.w, a12.w, a22.w);
+        for (int k = 0; k < 4; ++k) vtex[k] = ldg(meshes[mi].tex + k);
+        c = sample_maps<false>(vtex, mode, u, v, unit);
+    } else if (stex[1].w == stex[0].w && stex[2].w == stex[0].w && stex[3].w == stex[0].w &&
+             stex[1].h == stex[0].h && stex[2].h == stex[0].h && stex[3].h == stex[0].h)
+        c = sample_maps<true>(stex, mode, u, v, unit);
+    else
+        c = sample_maps<false>(stex, mode, u, v, unit);
+    const float tx = RAST_FINTERP(a01.z, a11.z, a21.z);
+    const float ty = RAST_FINTERP(a01.w, a11.w, a21.w);
+    const float tz = RAST_FINTERP(a02.x, a12.x, a22.x);
+    float vx = RAST_FINTERP(a02.y, a12.y, a22.y);
+    float vy = RAST_FINTERP(a02.z, a12.z, a22.z);
+    float vz = RAST_FINTERP(a02.w, a12.w, a22.w);
 #undef RAST_INTERP
-    const float vl = vx * vx + vy * vy + vz * vz;
+#undef RAST_FINTERP
+    const float vl = __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx));
     const float vinv = vl > 0.f ? __builtin_amdgcn_rsqf(vl) : 0.f;
     vx *= vinv; vy *= vinv; vz *= vinv;
     // binormal = -Cross(tan, n); N = {tan, binormal, n} * (2 * sample - 1)
-    const float cx = ty * nz - tz * ny, cy = tz * nx - tx * nz, cz = tx * ny - ty * nx;
-    const float fx = 2.f * nr - 1.f, fy = 2.f * ng - 1.f, fz = 2.f * nb - 1.f;
-    const float Nx = tx * fx - cx * fy + nx * fz;
-    const float Ny = ty * fx - cy * fy + ny * fz;
-    const float Nz = tz * fx - cz * fy + nz * fz;
+    const float cx = __builtin_fmaf(ty, nz, -tz * ny), cy = __builtin_fmaf(tz, nx, -tx * nz);
+    const float cz = __builtin_fmaf(tx, ny, -ty * nx);
+    const float fx = __builtin_fmaf(2.f, c.n[0], -1.f), fy = __builtin_fmaf(2.f, c.n[1], -1.f);
+    const float fz = __builtin_fmaf(2.f, c.n[2], -1.f);
+    const float Nx = __builtin_fmaf(nx, fz, __builtin_fmaf(-cx, fy, tx * fx));
+    const float Ny = __builtin_fmaf(ny, fz, __builtin_fmaf(-cy, fy, ty * fx));
+    const float Nz = __builtin_fmaf(nz, fz, __builtin_fmaf(-cz, fy, tz * fx));
     const float Lx = 0.577f, Ly = -0.577f, Lz = -0.577f;
-    // observed area (ClampFast(x, 0, x) leaves x unchanged) * intensity / pi
-    const float oa = ((-Nx) * Lx + (-Ny) * Ly + (-Nz) * Lz) * (7.0f / 3.14159265358979323846f);
-    const float twoDot = 2.f * (Lx * Nx + Ly * Ny + Lz * Nz);
-    const float rx = -Lx + twoDot * Nx, ry = -Ly + twoDot * Ny, rz = -Lz + twoDot * Nz;
-    float angle = vx * rx + vy * ry + vz * rz;
+    const float LN = __builtin_fmaf(Lz, Nz, __builtin_fmaf(Ly, Ny, Lx * Nx));
+    // observed area -N.L (ClampFast(x, 0, x) leaves x unchanged) * intensity / pi
+    const float oa = -LN * (7.0f / 3.14159265358979323846f);
+    const float twoDot = 2.f * LN;
+    const float rx = __builtin_fmaf(twoDot, Nx, -Lx), ry = __builtin_fmaf(twoDot, Ny, -Ly);
+    const float rz = __builtin_fmaf(twoDot, Nz, -Lz);
+    float angle = __builtin_fmaf(vz, rz, __builtin_fmaf(vy, ry, vx * rx));
     angle = angle < 0.f ? 0.f : angle;
     angle = angle > 1.f ? 1.f : angle;
     // powf(angle, gloss * 25) via exp2(e * log2(angle)); pow(x, 0) = 1
-    const float e = gr * 25.f;
+    const float e = c.g * 25.f;
     angle = e == 0.f ? 1.f : (angle > 0.f ? __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(angle)) : 0.f);
-    float outr = 0.05f + dr * oa + sr * angle;
-    float outg = 0.05f + dg * oa + sg * angle;
-    float outb = 0.05f + db * oa + sb * angle;
+    float outr = __builtin_fmaf(c.s[0], angle, __builtin_fmaf(c.d[0], oa, 0.05f));
+    float outg = __builtin_fmaf(c.s[1], angle, __builtin_fmaf(c.d[1], oa, 0.05f));
+    float outb = __builtin_fmaf(c.s[2], angle, __builtin_fmaf(c.d[2], oa, 0.05f));
     const float mgb = outg < outb ? outb : outg;
     const float mx = outr < mgb ? mgb : outr;
     if (mx > 1.f) {
@@ -1670,7 +1731,12 @@ hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s)
 hipError_t launch_shade(const Launch &L, hipStream_t s)
 {
     if (!L.grid[ST_SHADE]) return hipSuccess;
-    hipLaunchKernelGGL(k_shade, dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs, L.table[ST_SHADE]);
+    // one instance per sampling mode present: a mode's code alone keeps the
+    // register count (and so the waves per SIMD) of that instance
+    if (L.shade_modes & 1u)
+        hipLaunchKernelGGL(k_shade<false>, dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs, L.table[ST_SHADE]);
+    if (L.shade_modes & 2u)
+        hipLaunchKernelGGL(k_shade<true>, dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs, L.table[ST_SHADE]);
     return hipGetLastError();
 }
 

@@ -26,6 +26,7 @@ struct Launch {
     uint32_t cap_jobs;                // jobs binning in a single pass (capacity scan)
     uint32_t exact_jobs;              // jobs binning in two passes (count scan)
     int model_hidden;                 // hidden width of the learned jobs' MLP (16, 32 or 64; 0: none)
+    uint32_t shade_modes;             // sampling modes of the shading jobs: bit 0 point, bit 1 linear
 };
 
 // Resets a key buffer to depth 1 / no triangle (before a renderer's first frame).

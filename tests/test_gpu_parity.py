@@ -228,6 +228,24 @@ def test_sampler_and_visualisation_modes(vis_mode, sample):
     check_frame(w.scene, w.camera, opts, w.width, w.height)
 
 
+@pytest.mark.parametrize("sample", [S.SAMPLE_POINT, S.SAMPLE_LINEAR])
+def test_texture_paths_for_map_sizes_and_mixed_meshes(sample):
+    """Three ways the shading pass finds texels: four maps of one size (the
+    indices are computed once per pixel), four maps of different sizes, and
+    waves whose pixels belong to two meshes (per-lane texture tables).  The
+    second sphere's texture coordinates run past 1 (magenta texels)."""
+    a = S.uv_sphere(40, 20)
+    a.textures = [S.noise_texture(96, 64, 11 + k, k) for k in range(4)]
+    a.world = S.world_matrix((-0.45, 0.0, 0.0), (0.3, 0.5, 0.0), 0.8)
+    b = S.uv_sphere(40, 20)
+    b.verts = b.verts.copy()
+    b.verts[:, 3:5] *= 1.2
+    b.textures = [S.noise_texture(64, 48, 21, 0), S.noise_texture(32, 32, 22, 1), None,
+                  S.noise_texture(16, 64, 23, 3)]
+    b.world = S.world_matrix((0.45, 0.1, -0.2), (0.0, -0.4, 0.2), 0.8)
+    check_frame(S.Scene([a, b]), S.Camera(), S.Opts(sample_state=sample), 320, 240)
+
+
 def test_vertex_colour_path_without_diffuse_map():
     m = S.uv_sphere(48, 24)
     rng = np.random.default_rng(2)
