@@ -1522,15 +1522,10 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     if (x >= fp.width || y >= fp.height) return;
     const size_t pid = (size_t)y * fp.width + x;
     const uint32_t id = ldg(vis + pid);
-    const bool valid = id < fp.num_tris;
-    const uint32_t clear = fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u;
-    // The pixels of a wave (64 of one row) that show one triangle form runs,
-    // about 7 pixels long in a 4K frame of a 2M-triangle terrain.  Lane r
-    // fetches run r's triangle record and vertex attributes, and every pixel
-    // takes its run's words across lanes (ds_bpermute): the vector memory
-    // path, which bounds this kernel, carries each run's data once rather
-    // than once per pixel.
-    if (!valid) { stg(rgba + pid, clear); return; }
+    if (id >= fp.num_tris) {
+        stg(rgba + pid, (uint32_t)(fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u));
+        return;
+    }
     const uint4 *rp = reinterpret_cast<const uint4 *>(trirec + (size_t)id * kTriRecWords);
     const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q2 = ldg(rp + 2), q3 = ldg(rp + 3);
     const float sx0 = __uint_as_float(q0.x), sy0 = __uint_as_float(q0.y);
