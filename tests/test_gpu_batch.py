@@ -59,6 +59,7 @@ def test_graph_recaptures_after_option_and_size_changes():
             ("no cull", lambda r: None, S.Opts(cull_mode=S.CULL_NONE)),
             ("tile 32", lambda r: r.setup_bins(32), S.Opts()),
             ("micro 0", lambda r: r.set_tuning(0, 16), S.Opts(visualisation=S.VIS_DEPTH)),
+            ("linear", lambda r: None, S.Opts(sample_state=S.SAMPLE_LINEAR)),
             ("again", lambda r: None, S.Opts()),
         ]
         for name, change, opts in steps:
@@ -132,7 +133,9 @@ def batch_jobs(k, n_jobs):
             r = R.Renderer(W, H, tile=(8, 16, 32)[j % 3])
             r.add_scene(random_triangle_scene(100 + j, 50 + 40 * j))
             cam = screen_camera()
-        opts = S.Opts(cull_mode=(S.CULL_BACK, S.CULL_NONE)[j % 2])
+        # both sampling modes in one batch: the shading stage runs one kernel instance per mode
+        opts = S.Opts(cull_mode=(S.CULL_BACK, S.CULL_NONE)[j % 2],
+                      sample_state=(S.SAMPLE_POINT, S.SAMPLE_LINEAR)[(j // 2) % 2])
         rs.append(r)
         jobs.append((r, cam, opts, R.BIN_EXACT, 0))
     return rs, jobs
