@@ -400,20 +400,37 @@ __global__ __launch_bounds__(256) void k_clear_keys(unsigned long long *__restri
 }
 
 // VS_Kernel / GetNDCVertex (CCR/CUDARenderer.cu:673-700, CCR/CUDAROPs.cu:99-118).
+// A block's 256 vertices are 16 KB of input records and 16 KB of attribute
+// records per job, contiguous in HBM.  Both go through LDS so that global
+// loads and stores run over consecutive 16-byte words (eight full 128-byte
+// lines per instruction) instead of one 64-byte record per lane (32 lines
+// per instruction, a quarter of each).  LDS rows are 5 quads (80 bytes) apart,
+// so a wave's 16-byte row reads hit distinct banks.
 __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ meshes,
                                                 const uint32_t *__restrict__ block_group,
                                                 const uint2 *__restrict__ groups,
                                                 const uint32_t *__restrict__ entries,
                                                 const JobDesc *__restrict__ jobs)
 {
+    constexpr int kRow = 5;  // quads per LDS row (4 used)
+    __shared__ float4 rows[256 * kRow];
     // a group is one source mesh drawn by one or more jobs (views of one
     // scene): the vertex is read once and transformed for each of them
     const uint2 gr = groups[block_group[blockIdx.x]];
     const DevMesh &M0 = meshes[entries[gr.x]];
-    const uint32_t v = (blockIdx.x - M0.vs_block0) * 256u + threadIdx.x;
-    if (v >= M0.num_verts) return;
-    const float4 *iv = reinterpret_cast<const float4 *>(M0.ivert + (size_t)v * 16);
-    const float4 q0 = ldg(iv), q1 = ldg(iv + 1), q2 = ldg(iv + 2), q3 = ldg(iv + 3);
+    const uint32_t vbase = (blockIdx.x - M0.vs_block0) * 256u;
+    const uint32_t nv = M0.num_verts - vbase < 256u ? M0.num_verts - vbase : 256u;  // vertices of this block
+    const uint32_t tid = threadIdx.x;
+    const float4 *src = reinterpret_cast<const float4 *>(M0.ivert) + (size_t)vbase * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t w = (uint32_t)k * 256u + tid;  // quad w of the block: vertex w / 4, part w % 4
+        if (w < nv * 4u) rows[(w >> 2) * kRow + (w & 3u)] = ldg(src + w);
+    }
+    __syncthreads();
+    const bool live = tid < nv;
+    const float4 q0 = rows[tid * kRow], q1 = rows[tid * kRow + 1], q2 = rows[tid * kRow + 2], q3 = rows[tid * kRow + 3];
+    const uint32_t v = vbase + tid;
     const float px = q0.x, py = q0.y, pz = q0.z;
     const float u = q0.w, vv = q1.x;
     const float nx = q1.y, ny = q1.z, nz = q1.w;
@@ -435,7 +452,7 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
         o1 /= o3;
         o2 /= o3;
         const uint32_t g = M.vert_base + v;
-        stg(pos4 + g, make_float4(o0, o1, o2, o3));
+        if (live) stg(pos4 + g, make_float4(o0, o1, o2, o3));
         const float n0 = m4(Wd, 0, 0) * nx + m4(Wd, 0, 1) * ny + m4(Wd, 0, 2) * nz;
         const float n1 = m4(Wd, 1, 0) * nx + m4(Wd, 1, 1) * ny + m4(Wd, 1, 2) * nz;
         const float n2 = m4(Wd, 2, 0) * nx + m4(Wd, 2, 1) * ny + m4(Wd, 2, 2) * nz;
@@ -452,12 +469,19 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
         // product c_k * (1/w_k) is the same binary32 value whether it is formed
         // here once or per pixel.  PBR shading reads the first three float4s only.
         const float iw = 1.f / o3;
-        float4 *a = attr + (size_t)g * 4;
-        stg(a, make_float4(u * iw, vv * iw, iw, n0 * iw));
-        stg(a + 1, make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw));
-        stg(a + 2, make_float4(t2 * iw, vx * iw, vy * iw, vz * iw));
-        stg(a + 3, make_float4(cr * iw, cg * iw, cb * iw, 0.f));
-        if (density && (v & 7u) == 0) {
+        __syncthreads();  // the rows of the previous job (or of the input) are consumed
+        rows[tid * kRow] = make_float4(u * iw, vv * iw, iw, n0 * iw);
+        rows[tid * kRow + 1] = make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw);
+        rows[tid * kRow + 2] = make_float4(t2 * iw, vx * iw, vy * iw, vz * iw);
+        rows[tid * kRow + 3] = make_float4(cr * iw, cg * iw, cb * iw, 0.f);
+        __syncthreads();
+        float4 *dst = attr + (size_t)(M.vert_base + vbase) * 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t w = (uint32_t)k * 256u + tid;
+            if (w < nv * 4u) stg(dst + w, rows[(w >> 2) * kRow + (w & 3u)]);
+        }
+        if (live && density && (v & 7u) == 0) {
             // projected-vertex density feature of the capacity model (every 8th
             // vertex; no parity role)
             const float sx = ((o0 + 1) / 2) * (float)fp.width;
