@@ -733,6 +733,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         nb[res] = ntiles;
         nb[ST_SHADE] = job_shades(r) ? ceil_div(r->W, 64) * ceil_div(r->H, 4) : 0;
         if (nb[ST_SHADE]) L.shade_modes |= 1u << r->last_opts.sample_state;
+        else ++L.unshaded_jobs;
         for (int st = 0; st < ST_COUNT; ++st) {
             P.block0[(size_t)j * ST_COUNT + st] = (uint32_t)grid[st];
             P.nblocks[(size_t)j * ST_COUNT + st] = nb[st];

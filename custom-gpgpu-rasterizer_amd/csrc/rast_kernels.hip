@@ -1534,6 +1534,12 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     const FrameParams &fp = J.fp;
     if ((fp.sample_state != 0) != kLinear) return;  // the other sampling mode's instance shades this frame
     constexpr int mode = kLinear ? 1 : 0;
+    // the frame's counters are final once resolve has run: the job's first
+    // block copies them to the renderer's pinned host block (k_publish does
+    // this for jobs that are not shaded)
+    if (local == 0 && threadIdx.x < sizeof(FrameCounters) / 4)
+        reinterpret_cast<volatile uint32_t *>(J.host_counters)[threadIdx.x] =
+            reinterpret_cast<const uint32_t *>(J.counters)[threadIdx.x];
     const uint32_t *__restrict__ vis = J.vis;
     const float *__restrict__ depth = J.depth;
     const uint32_t *__restrict__ trirec = J.trirec;
@@ -1662,14 +1668,16 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     stg(rgba + pid, (uint32_t)(pack_rgba(outr, outg, outb)));
 }
 
-// Copies every job's counters to its renderer's pinned host block (one word
-// per thread), replacing a device-to-host copy per job.
+// Copies the counters of every job that is not shaded to its renderer's
+// pinned host block (one word per thread), replacing a device-to-host copy
+// per job; the shading kernel publishes the others.
 __global__ __launch_bounds__(256) void k_publish(const JobDesc *__restrict__ jobs, uint32_t njobs)
 {
     constexpr uint32_t W = sizeof(FrameCounters) / 4;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= njobs * W) return;
     const JobDesc &J = jobs[i / W];
+    if (J.nblocks[ST_SHADE]) return;
     reinterpret_cast<volatile uint32_t *>(J.host_counters)[i % W] =
         reinterpret_cast<const uint32_t *>(J.counters)[i % W];
 }
@@ -1761,6 +1769,7 @@ hipError_t launch_shade(const Launch &L, hipStream_t s)
 
 hipError_t launch_publish(const Launch &L, hipStream_t s)
 {
+    if (L.unshaded_jobs == 0) return hipSuccess;  // every job's shading kernel publishes
     const uint32_t n = L.njobs * (uint32_t)(sizeof(FrameCounters) / 4);
     hipLaunchKernelGGL(k_publish, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.jobs, L.njobs);
     return hipGetLastError();

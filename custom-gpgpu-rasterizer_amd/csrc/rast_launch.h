@@ -27,6 +27,7 @@ struct Launch {
     uint32_t exact_jobs;              // jobs binning in two passes (count scan)
     int model_hidden;                 // hidden width of the learned jobs' MLP (16, 32 or 64; 0: none)
     uint32_t shade_modes;             // sampling modes of the shading jobs: bit 0 point, bit 1 linear
+    uint32_t unshaded_jobs;           // jobs without a shading stage (their counters go out with k_publish)
 };
 
 // Resets a key buffer to depth 1 / no triangle (before a renderer's first frame).
