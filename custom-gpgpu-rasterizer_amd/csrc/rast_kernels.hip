@@ -3,7 +3,9 @@
 // Built with -ffp-contract=off and no fast-math: every float expression below
 // is evaluated as written (one IEEE binary32 operation each, left to right),
 // division and sqrt correctly rounded, so visibility and depth are bitwise the
-// CPU restatement of the reference path.  Reference files are cited relative
+// CPU restatement of the reference path (the shading's lighting terms, held to
+// RGBA8 within 1/255, use explicit fused multiply-adds and the hardware
+// reciprocal, rsqrt, exp2 and log2).  Reference files are cited relative
 // to kdedeurw/Custom-GPGPU-Rasterizer (CCR = CustomCUDARasterizer/).
 //
 // Depth resolution is order-free: every fragment becomes a 64-bit key
@@ -24,7 +26,9 @@
 //   k_fallback single pass: exact placement of entries beyond capacity
 //   k_resolve  one workgroup per tile: keys -> LDS (and cleared for the next
 //              frame), queued triangles rasterized in LDS, vis/depth written
-//   k_shade    per pixel, deferred from the visibility buffer
+//   k_shade    per pixel, deferred from the visibility buffer; one instance per
+//              sampling mode; its first block of each job publishes the
+//              frame counters to the host (k_publish for unshaded jobs)
 #include <hip/hip_runtime.h>
 
 #include "rast_launch.h"
