@@ -31,7 +31,7 @@ constexpr int kTriRecWords = 16;
 constexpr int kMaxMeshes = 1 << 16;
 constexpr int kNumFeatures = 8;
 // Tiles per workgroup of the capacity-model kernel (16 per wave).
-constexpr int kCapacityTilesPerBlock = 32;
+constexpr int kCapacityTilesPerBlock = 64;
 
 struct DevTexture {
     const uint32_t *texels;
@@ -80,7 +80,7 @@ struct FrameParams {
 // stage s serves job table_s[b], as its block b - JobDesc::block0[s].
 enum Stage : int {
     ST_INIT,      // per-tile reset (256 tiles per block)
-    ST_CAP,       // single-pass capacities (32 tiles per block)
+    ST_CAP,       // single-pass capacities (64 tiles per block)
     ST_FILL,      // exact second pass (grid-stride over binned triangles)
     ST_FBALLOC,   // single-pass fallback allocation (256 tiles per block)
     ST_FBFILL,    // single-pass fallback placement (grid-stride over entries)
