@@ -485,13 +485,33 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
             const uint32_t w = (uint32_t)k * 256u + tid;
             if (w < nv * 4u) stg(dst + w, rows[(w >> 2) * kRow + (w & 3u)]);
         }
-        if (live && density && (v & 7u) == 0) {
+        if (density) {
             // projected-vertex density feature of the capacity model (every 8th
-            // vertex; no parity role)
-            const float sx = ((o0 + 1) / 2) * (float)fp.width;
-            const float sy = ((1 - o1) / 2) * (float)fp.height;
-            if (o3 > 0.f && sx >= 0.f && sy >= 0.f && sx < (float)fp.width && sy < (float)fp.height)
-                count_tile(density, ((uint32_t)sy / fp.tile) * fp.tiles_x + (uint32_t)sx / fp.tile);
+            // vertex; no parity role): the block's 32 samples are grouped by tile
+            // in LDS, one atomic per distinct tile
+            __shared__ uint32_t dtile[32];
+            if ((tid & 7u) == 0) {
+                uint32_t t = 0xFFFFFFFFu;
+                const float sx = ((o0 + 1) / 2) * (float)fp.width;
+                const float sy = ((1 - o1) / 2) * (float)fp.height;
+                if (live && o3 > 0.f && sx >= 0.f && sy >= 0.f && sx < (float)fp.width && sy < (float)fp.height)
+                    t = ((uint32_t)sy / fp.tile) * fp.tiles_x + (uint32_t)sx / fp.tile;
+                dtile[tid >> 3] = t;
+            }
+            __syncthreads();
+            if (tid < 32) {
+                const uint32_t mine = dtile[tid];
+                bool first = mine != 0xFFFFFFFFu;
+                uint32_t cnt = 0;
+                for (uint32_t j = 0; j < 32; ++j) {
+                    const uint32_t o = dtile[j];
+                    if (o == mine) {
+                        ++cnt;
+                        if (j < tid) first = false;
+                    }
+                }
+                if (first) gatomic_add(density + mine, cnt);
+            }
         }
     }
 }
