@@ -408,16 +408,17 @@ __global__ __launch_bounds__(256) void k_clear_keys(unsigned long long *__restri
 // records per job, contiguous in HBM.  Both go through LDS so that global
 // loads and stores run over consecutive 16-byte words (eight full 128-byte
 // lines per instruction) instead of one 64-byte record per lane (32 lines
-// per instruction, a quarter of each).  LDS rows are 5 quads (80 bytes) apart,
-// so a wave's 16-byte row reads hit distinct banks.
+// per instruction, a quarter of each).  Quad q of LDS row r sits at slot
+// q ^ ((r >> 2) & 3) of the row: both the row accesses (lane r takes row r)
+// and the consecutive ones (lane w takes quad w) then hit distinct banks.
 __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ meshes,
                                                 const uint32_t *__restrict__ block_group,
                                                 const uint2 *__restrict__ groups,
                                                 const uint32_t *__restrict__ entries,
                                                 const JobDesc *__restrict__ jobs)
 {
-    constexpr int kRow = 5;  // quads per LDS row (4 used)
-    __shared__ float4 rows[256 * kRow];
+    __shared__ float4 rows[256 * 4];
+    auto at = [](uint32_t r, uint32_t q) { return r * 4u + (q ^ ((r >> 2) & 3u)); };
     // a group is one source mesh drawn by one or more jobs (views of one
     // scene): the vertex is read once and transformed for each of them
     const uint2 gr = groups[block_group[blockIdx.x]];
@@ -429,11 +430,11 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t w = (uint32_t)k * 256u + tid;  // quad w of the block: vertex w / 4, part w % 4
-        if (w < nv * 4u) rows[(w >> 2) * kRow + (w & 3u)] = ldg(src + w);
+        if (w < nv * 4u) rows[at(w >> 2, w & 3u)] = ldg(src + w);
     }
     __syncthreads();
     const bool live = tid < nv;
-    const float4 q0 = rows[tid * kRow], q1 = rows[tid * kRow + 1], q2 = rows[tid * kRow + 2], q3 = rows[tid * kRow + 3];
+    const float4 q0 = rows[at(tid, 0)], q1 = rows[at(tid, 1)], q2 = rows[at(tid, 2)], q3 = rows[at(tid, 3)];
     const uint32_t v = vbase + tid;
     const float px = q0.x, py = q0.y, pz = q0.z;
     const float u = q0.w, vv = q1.x;
@@ -474,16 +475,16 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
         // here once or per pixel.  PBR shading reads the first three float4s only.
         const float iw = 1.f / o3;
         __syncthreads();  // the rows of the previous job (or of the input) are consumed
-        rows[tid * kRow] = make_float4(u * iw, vv * iw, iw, n0 * iw);
-        rows[tid * kRow + 1] = make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw);
-        rows[tid * kRow + 2] = make_float4(t2 * iw, vx * iw, vy * iw, vz * iw);
-        rows[tid * kRow + 3] = make_float4(cr * iw, cg * iw, cb * iw, 0.f);
+        rows[at(tid, 0)] = make_float4(u * iw, vv * iw, iw, n0 * iw);
+        rows[at(tid, 1)] = make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw);
+        rows[at(tid, 2)] = make_float4(t2 * iw, vx * iw, vy * iw, vz * iw);
+        rows[at(tid, 3)] = make_float4(cr * iw, cg * iw, cb * iw, 0.f);
         __syncthreads();
         float4 *dst = attr + (size_t)(M.vert_base + vbase) * 4;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t w = (uint32_t)k * 256u + tid;
-            if (w < nv * 4u) stg(dst + w, rows[(w >> 2) * kRow + (w & 3u)]);
+            if (w < nv * 4u) stg(dst + w, rows[at(w >> 2, w & 3u)]);
         }
         if (density) {
             // projected-vertex density feature of the capacity model (every 8th
