@@ -1390,8 +1390,15 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             const float y = __shfl_xor(dmin, o);
             dmin = y < dmin ? y : dmin;
         }
-        // depths are >= 0, so their bit patterns order like the values
-        if (lane == 0) gatomic_min(reinterpret_cast<uint32_t *>(tile_min_depth) + tile, __float_as_uint(dmin));
+        // one block per tile: the block's minimum is the tile's, a plain store
+        __syncthreads();  // wdmax may still be read by the screen-list bound
+        if (lane == 0) wdmax[wave] = dmin;
+        __syncthreads();
+        if (tid == 0) {
+            float m = wdmax[0];
+            for (int w = 1; w < NW; ++w) m = wdmax[w] < m ? wdmax[w] : m;
+            stg(tile_min_depth + tile, m);
+        }
     }
 }
 
