@@ -123,11 +123,6 @@ __device__ __forceinline__ uint32_t gatomic_add(uint32_t *p, uint32_t v)
     return __hip_atomic_fetch_add(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void gatomic_min(uint32_t *p, uint32_t v)
-{
-    __hip_atomic_fetch_min(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ void gatomic_min(unsigned long long *p, unsigned long long v)
 {
     __hip_atomic_fetch_min(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
