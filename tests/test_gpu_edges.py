@@ -77,3 +77,37 @@ def test_exact_depth_ties_go_to_the_later_triangle():
                 r.render(screen_camera(), S.Opts(cull_mode=S.CULL_NONE))
                 vis, depth, _ = r.readback()
             assert_bitwise(f"vis tile {tile} micro {micro}", vis, ovis)
+
+
+def test_learned_binning_with_non_finite_vertices_gives_the_exact_frame():
+    """The projected-vertex density count (learned capacities) skips
+    vertices off screen, behind the camera or non-finite; the frames equal
+    the exact binner's bit for bit."""
+    base = random_triangle_scene(21, 400).meshes[0]
+    v = base.verts.copy()
+    v[::37, 0] = np.nan
+    v[5::41, 1] = np.inf
+    v[7::43, 2] = 2.5  # behind the camera
+    scene = S.Scene([S.Mesh(verts=v, indices=base.indices)])
+    cams = [S.Camera(position=(0.02 * k, 0.0, 0.0), forward=(0.0, 0.0, 1.0), fov_deg=90.0, near_z=0.1, far_z=100.0)
+            for k in range(4)]
+    opts = S.Opts(cull_mode=S.CULL_NONE)
+    with R.Renderer(200, 120) as ex, R.Renderer(200, 120) as lr:
+        for r in (ex, lr):
+            r.set_tuning(0, 16)  # bin every triangle
+            r.add_scene(scene)
+        model = R.BinModel(lr, hidden=16, seed=4)
+        model.attach(lr, 1.0, 0)
+        for i, cam in enumerate(cams):
+            lr.render(cam, opts, R.BIN_EXACT)
+            if i > 0:  # features need a previous frame
+                model.record(lr)
+        model.train(epochs=2, batch=256)
+        for cam in cams:
+            ex.render(cam, opts, R.BIN_EXACT)
+            lr.render(cam, opts, R.BIN_LEARNED)
+            a, b = ex.readback(), lr.readback()
+            assert_bitwise("vis", b[0], a[0])
+            assert_bitwise("depth", b[1], a[1])
+            np.testing.assert_array_equal(b[2], a[2])
+        model.close()
