@@ -148,8 +148,11 @@ __device__ __forceinline__ uint64_t match_tile(uint32_t t)
 // Order-free append for three predicates at once: every thread with a
 // predicate set gets a slot of its counter, one atomic per counter per
 // 256-thread block (two barriers; every thread of the block must call it).
+// Counter c0 only counts (its slots s0 are not formed): its atomic returns
+// nothing, so no thread waits for it.  The others wait for their atomic's
+// round trip past L2 only in blocks where their predicate holds somewhere.
 __device__ __forceinline__ void block_append3(uint32_t *c0, bool p0, uint32_t *c1, bool p1, uint32_t *c2, bool p2,
-                                              uint32_t *lds, uint32_t &s0, uint32_t &s1, uint32_t &s2)
+                                              uint32_t *lds, uint32_t &s1, uint32_t &s2)
 {
     const uint64_t m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2);
     const uint32_t wave = threadIdx.x >> 6;
@@ -164,14 +167,17 @@ __device__ __forceinline__ void block_append3(uint32_t *c0, bool p0, uint32_t *c
         uint32_t *l = lds + 4 * threadIdx.x;
         const uint32_t a = l[0], b = l[1], c = l[2], d = l[3];
         const uint32_t tot = a + b + c + d;
-        const uint32_t base = tot ? gatomic_add(cnt, tot) : 0u;
+        uint32_t base = 0;
+        if (tot) {
+            if (threadIdx.x == 0) (void)__hip_atomic_fetch_add(gptr(cnt), tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else base = gatomic_add(cnt, tot);
+        }
         l[0] = base;
         l[1] = base + a;
         l[2] = base + a + b;
         l[3] = base + a + b + c;
     }
     __syncthreads();
-    s0 = lds[wave] + popc_below(m0);
     s1 = lds[4 + wave] + popc_below(m1);
     s2 = lds[8 + wave] + popc_below(m2);
 }
@@ -692,16 +698,16 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
         }
     }
 
-    uint32_t n_ok, bslot, sslot;
-    block_append3(&ctr->num_alive, ok, &ctr->num_big, is_big, &ctr->num_binned, binned, lds_append, n_ok, bslot,
-                  sslot);
-    (void)n_ok;
+    // the survivor list feeds the exact binner's fill pass only
+    uint32_t bslot, sslot;
+    block_append3(&ctr->num_alive, ok, &ctr->num_big, is_big, &ctr->num_binned, binned && !single_pass, lds_append,
+                  bslot, sslot);
     if (is_big) {
         if (bslot < J.big_cap) stg(J.big + bslot, gid);
         else ctr->storage_overflow = 1;
     }
     if (binned) {
-        stg(J.survivors + sslot, gid);
+        if (!single_pass) stg(J.survivors + sslot, gid);
         const uint32_t w = tx1 - tx0 + 1;
         uint32_t tx = tx0, ty = ty0;
         for (uint32_t s = 0; s < span; ++s) {
