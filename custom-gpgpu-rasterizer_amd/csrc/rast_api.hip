@@ -661,6 +661,7 @@ void plan_free(LaunchPlan &p)
     if (p.h_stage) (void)hipHostFree(p.h_stage);
     p.h_stage = nullptr;
     dfree(p.d_block_mesh);
+    dfree(p.d_alive_part);
     dfree(p.d_tables);
     dfree(p.d_stage);
     p.built = false;
@@ -787,6 +788,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     const size_t entries_at = tab.size();
     for (const auto &mm : members) tab.insert(tab.end(), mm.begin(), mm.end());
     RAST_TRY(dev_alloc(P.d_block_mesh, (tab.size() + 2) * 4));
+    RAST_TRY(dev_alloc(P.d_alive_part, ((size_t)L.tri_blocks + 1) * 4));
     uint32_t *bm = (uint32_t *)P.d_block_mesh.p;
     if (!tab.empty()) HIP_TRY(hipMemcpy(bm, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
     L.vs_block_group = bm;
@@ -887,6 +889,13 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
         J.big = (uint32_t *)r->big.p;
         J.counters = (FrameCounters *)r->counters.p;
         J.host_counters = r->d_counters_host;
+        J.alive_part = (uint32_t *)P.d_alive_part.p;
+        J.alive_part0 = 0;
+        J.alive_parts = 0;
+        if (!meshes.empty()) {  // the job's setup blocks are contiguous, in mesh order
+            J.alive_part0 = P.tri_block0[me];
+            J.alive_parts = P.tri_block0[me + meshes.size() - 1] + ceil_div(meshes.back()->ntri, 256) - J.alive_part0;
+        }
         J.density = r->model ? (uint32_t *)r->density.p : nullptr;  // feature of the capacity model
         J.tile_min_depth = (float *)r->tile_min_depth[r->cur].p;
         J.keys = (unsigned long long *)r->keys.p;

@@ -137,8 +137,14 @@ struct JobDesc {
     uint2 *ovf;                  // single pass: (tile, id) entries beyond capacity
     uint32_t *survivors;
     uint32_t *big;
-    FrameCounters *counters;
+    FrameCounters *counters;     // num_alive stays 0 here: see alive_part
     FrameCounters *host_counters;  // pinned host copy, written at the end of the launch
+    // culling survivors per setup block (launch-wide, indexed by the setup
+    // block); the job's blocks are [alive_part0, alive_part0 + alive_parts).
+    // Summed when the counters are published: one shared counter updated by
+    // every block serializes those updates at its cache line.
+    uint32_t *alive_part;
+    uint32_t alive_part0, alive_parts;
     uint32_t *density;           // per tile projected-vertex count (capacity model), may be null
     float *tile_min_depth;       // per tile min depth written by the resolve pass
     unsigned long long *keys;    // per pixel (z << 32 | ~(id + 1)), reset to depth 1 / none after each resolve

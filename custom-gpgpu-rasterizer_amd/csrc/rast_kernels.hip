@@ -145,14 +145,15 @@ __device__ __forceinline__ uint64_t match_tile(uint32_t t)
     return group;
 }
 
-// Order-free append for three predicates at once: every thread with a
-// predicate set gets a slot of its counter, one atomic per counter per
-// 256-thread block (two barriers; every thread of the block must call it).
-// Counter c0 only counts (its slots s0 are not formed): its atomic returns
-// nothing, so no thread waits for it.  The others wait for their atomic's
-// round trip past L2 only in blocks where their predicate holds somewhere.
-__device__ __forceinline__ void block_append3(uint32_t *c0, bool p0, uint32_t *c1, bool p1, uint32_t *c2, bool p2,
-                                              uint32_t *lds, uint32_t &s1, uint32_t &s2)
+// Order-free append for two predicates (p1, p2), plus the block's count of
+// p0: every thread with p1 / p2 set gets a slot of its counter, one atomic
+// per counter per 256-thread block and only where the predicate holds
+// somewhere in the block (two barriers; every thread of the block must call
+// it).  The p0 count is stored to part[blockIdx.x], with no atomic: a counter
+// that every block updates serializes those updates at its cache line, and
+// every block waiting for the returned value paid for that.
+__device__ __forceinline__ void block_append(uint32_t *part, bool p0, uint32_t *c1, bool p1, uint32_t *c2, bool p2,
+                                             uint32_t *lds, uint32_t &s1, uint32_t &s2)
 {
     const uint64_t m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2);
     const uint32_t wave = threadIdx.x >> 6;
@@ -163,23 +164,49 @@ __device__ __forceinline__ void block_append3(uint32_t *c0, bool p0, uint32_t *c
     }
     __syncthreads();
     if (threadIdx.x < 3) {
-        uint32_t *cnt = threadIdx.x == 0 ? c0 : (threadIdx.x == 1 ? c1 : c2);
         uint32_t *l = lds + 4 * threadIdx.x;
         const uint32_t a = l[0], b = l[1], c = l[2], d = l[3];
         const uint32_t tot = a + b + c + d;
-        uint32_t base = 0;
-        if (tot) {
-            if (threadIdx.x == 0) (void)__hip_atomic_fetch_add(gptr(cnt), tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            else base = gatomic_add(cnt, tot);
+        if (threadIdx.x == 0) {
+            stg(part + blockIdx.x, tot);
+        } else {
+            const uint32_t base = tot ? gatomic_add(threadIdx.x == 1 ? c1 : c2, tot) : 0u;
+            l[0] = base;
+            l[1] = base + a;
+            l[2] = base + a + b;
+            l[3] = base + a + b + c;
         }
-        l[0] = base;
-        l[1] = base + a;
-        l[2] = base + a + b;
-        l[3] = base + a + b + c;
     }
     __syncthreads();
     s1 = lds[4 + wave] + popc_below(m1);
     s2 = lds[8 + wave] + popc_below(m2);
+}
+
+// Sum of a job's per-setup-block survivor counts, by one workgroup of
+// 64 * NW threads (every thread calls it; red holds NW words).
+template <int NW>
+__device__ __forceinline__ uint32_t job_alive(const JobDesc &J, uint32_t *red)
+{
+    uint32_t v = 0;
+    for (uint32_t i = threadIdx.x; i < J.alive_parts; i += 64u * NW) v += ldg(J.alive_part + J.alive_part0 + i);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) t += red[w];
+    return t;
+}
+
+// Writes the job's counters to its renderer's pinned host block, num_alive
+// being the sum of the per-block counts (threads 0 .. words - 1 write).
+__device__ __forceinline__ void publish_counters(const JobDesc &J, uint32_t alive)
+{
+    constexpr uint32_t W = sizeof(FrameCounters) / 4;
+    static_assert(offsetof(FrameCounters, num_alive) == 0, "num_alive is the first counter word");
+    if (threadIdx.x < W)
+        reinterpret_cast<volatile uint32_t *>(J.host_counters)[threadIdx.x] =
+            threadIdx.x == 0 ? alive : reinterpret_cast<const uint32_t *>(J.counters)[threadIdx.x];
 }
 
 // counters[t] += 1 for every active lane: one atomic per distinct tile, all
@@ -700,8 +727,8 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
 
     // the survivor list feeds the exact binner's fill pass only
     uint32_t bslot, sslot;
-    block_append3(&ctr->num_alive, ok, &ctr->num_big, is_big, &ctr->num_binned, binned && !single_pass, lds_append,
-                  bslot, sslot);
+    block_append(J.alive_part, ok, &ctr->num_big, is_big, &ctr->num_binned, binned && !single_pass, lds_append, bslot,
+                 sslot);
     if (is_big) {
         if (bslot < J.big_cap) stg(J.big + bslot, gid);
         else ctr->storage_overflow = 1;
@@ -1570,9 +1597,10 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     // the frame's counters are final once resolve has run: the job's first
     // block copies them to the renderer's pinned host block (k_publish does
     // this for jobs that are not shaded)
-    if (local == 0 && threadIdx.x < sizeof(FrameCounters) / 4)
-        reinterpret_cast<volatile uint32_t *>(J.host_counters)[threadIdx.x] =
-            reinterpret_cast<const uint32_t *>(J.counters)[threadIdx.x];
+    if (local == 0) {
+        __shared__ uint32_t red[4];
+        publish_counters(J, job_alive<4>(J, red));
+    }
     const uint32_t *__restrict__ vis = J.vis;
     const float *__restrict__ depth = J.depth;
     const uint32_t *__restrict__ trirec = J.trirec;
@@ -1702,17 +1730,14 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
 }
 
 // Copies the counters of every job that is not shaded to its renderer's
-// pinned host block (one word per thread), replacing a device-to-host copy
+// pinned host block (one workgroup per job), replacing a device-to-host copy
 // per job; the shading kernel publishes the others.
-__global__ __launch_bounds__(256) void k_publish(const JobDesc *__restrict__ jobs, uint32_t njobs)
+__global__ __launch_bounds__(256) void k_publish(const JobDesc *__restrict__ jobs)
 {
-    constexpr uint32_t W = sizeof(FrameCounters) / 4;
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= njobs * W) return;
-    const JobDesc &J = jobs[i / W];
+    __shared__ uint32_t red[4];
+    const JobDesc &J = jobs[blockIdx.x];
     if (J.nblocks[ST_SHADE]) return;
-    reinterpret_cast<volatile uint32_t *>(J.host_counters)[i % W] =
-        reinterpret_cast<const uint32_t *>(J.counters)[i % W];
+    publish_counters(J, job_alive<4>(J, red));
 }
 
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -1803,8 +1828,7 @@ hipError_t launch_shade(const Launch &L, hipStream_t s)
 hipError_t launch_publish(const Launch &L, hipStream_t s)
 {
     if (L.unshaded_jobs == 0) return hipSuccess;  // every job's shading kernel publishes
-    const uint32_t n = L.njobs * (uint32_t)(sizeof(FrameCounters) / 4);
-    hipLaunchKernelGGL(k_publish, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.jobs, L.njobs);
+    hipLaunchKernelGGL(k_publish, dim3(L.njobs), dim3(256), 0, s, L.jobs);
     return hipGetLastError();
 }
 

@@ -53,6 +53,7 @@ struct LaunchPlan {
     std::vector<uint32_t> block0, nblocks;  // per job x stage (host copies for JobDesc)
     std::vector<uint32_t> vs_block0, tri_block0;  // per mesh entry
     DBuf d_block_mesh, d_tables, d_stage;
+    DBuf d_alive_part;  // culling survivors per setup block
     void *h_stage = nullptr;  // pinned: JobDesc[njobs] then DevMesh[nmeshes]
     size_t stage_bytes = 0;
     hipGraphExec_t exec = nullptr;  // captured launch (graph mode)
