@@ -741,12 +741,18 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
             const uint32_t tl = ty * fp.tiles_x + tx;
             if (single_pass) {
                 const uint32_t slot = claim_slot(tile_count, tl);
-                if (slot < ldg(J.tile_cap + tl)) {
+                const bool fits = slot < ldg(J.tile_cap + tl);
+                const uint64_t om = __ballot(!fits);
+                if (fits) {
                     const uint32_t pos = ldg(J.tile_off + tl) + slot;
                     if (pos < J.queue_cap) stg(J.queue + pos, gid);
                 } else {
                     // capacity exceeded: keep the entry for the exact fallback
-                    const uint32_t o = gatomic_add(&ctr->ovf_entries, 1u);
+                    // (one atomic on the shared counter per wave)
+                    const int leader = __builtin_ctzll(om);
+                    uint32_t base = 0;
+                    if (lane_id() == (uint32_t)leader) base = gatomic_add(&ctr->ovf_entries, (uint32_t)__popcll(om));
+                    const uint32_t o = (uint32_t)__shfl((int)base, leader) + popc_below(om);
                     if (o < J.ovf_cap) stg(J.ovf + o, make_uint2(tl, gid));
                     else ctr->storage_overflow = 1;
                 }
