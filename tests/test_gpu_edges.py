@@ -111,3 +111,36 @@ def test_learned_binning_with_non_finite_vertices_gives_the_exact_frame():
             assert_bitwise("depth", b[1], a[1])
             np.testing.assert_array_equal(b[2], a[2])
         model.close()
+
+
+def test_culling_survivor_counts_match_the_oracle_in_frames_and_batches():
+    """num_visible is summed from per-setup-block counts when the counters
+    are published (by the shading kernel, or by k_publish for unshaded
+    frames): check it against the oracle's survivors for multi-mesh frames,
+    shaded and unshaded, alone and in a batch with a view."""
+    def scene(seed):
+        return S.Scene([random_triangle_scene(seed + k, 300 + 157 * k).meshes[0] for k in range(3)])
+
+    def want(sc, cam, cull, W, H):
+        flags, _ = O.setup(sc, cam, cull, W, H)
+        return int(flags.astype(bool).sum())
+
+    cam = screen_camera()
+    a, b = scene(31), scene(41)
+    with R.Renderer(160, 96) as ra, R.Renderer(200, 120) as rb:
+        ra.add_scene(a)
+        rb.add_scene(b)
+        for flags in (0, R.FLAG_NO_SHADE, R.FLAG_GRAPH):
+            for cull in (S.CULL_BACK, S.CULL_NONE):
+                ra.render(cam, S.Opts(cull_mode=cull), R.BIN_EXACT, flags)
+                assert ra.stats()["num_visible"] == want(a, cam, cull, 160, 96), (flags, cull)
+        v = ra.view(96, 64)
+        try:
+            jobs = [(ra, cam, S.Opts(), R.BIN_EXACT, 0), (rb, cam, S.Opts(cull_mode=S.CULL_NONE), R.BIN_EXACT, 0),
+                    (v, cam, S.Opts(), R.BIN_EXACT, R.FLAG_NO_SHADE)]
+            R.render_batch(jobs)
+            assert ra.stats()["num_visible"] == want(a, cam, S.CULL_BACK, 160, 96)
+            assert rb.stats()["num_visible"] == want(b, cam, S.CULL_NONE, 200, 120)
+            assert v.stats()["num_visible"] == want(a, cam, S.CULL_BACK, 96, 64)
+        finally:
+            v.close()
