@@ -1303,6 +1303,7 @@ extern "C" int rast_render_batch(const rast_job *jobs, uint32_t n)
             HIP_TRY(hipStreamWaitEvent(g_batch.stream, t->join, 0));
         }
         RAST_TRY(clear_keys_if_dirty(t, g_batch.stream));
+        t->keys_dirty = true;  // until the batch (whose resolve pass resets the keys) is enqueued
     }
     RAST_TRY(run_plan(P, g_batch.stream, true, nullptr));
     const uint64_t seq = g_bsync.recorded.load() + 1;
@@ -1310,6 +1311,7 @@ extern "C" int rast_render_batch(const rast_job *jobs, uint32_t n)
     g_bsync.recorded.store(seq);
     for (rast_renderer *t : rs) {
         t->batch_seq = seq;  // later use of t waits for this batch (renderer_wait)
+        t->keys_dirty = false;
         finish_enqueue(t);
     }
     return RAST_OK;

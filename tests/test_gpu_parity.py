@@ -271,3 +271,39 @@ def test_multi_mesh_ids_and_world_matrices():
     c = random_triangle_scene(23, 60).meshes[0]
     scene = S.Scene([a, b, c])
     check_frame(scene, screen_camera(), S.Opts(cull_mode=S.CULL_NONE), 160, 96)
+
+
+def test_frame_sequence_matches_oracle_every_frame():
+    """Eleven frames of one renderer with a moving camera, each equal to the
+    oracle's: no depth key a frame leaves behind may show in the next one.
+    The sequence mixes eager frames, graph replays, a batch and the
+    previous-frame single-pass policy, and includes a frame that sees
+    nothing, whose buffers must come out empty."""
+    scene = random_triangle_scene(31, 400)
+    W, H = 160, 96
+    steps = []
+    for k in range(11):
+        if k == 6:
+            cam = S.Camera(position=(0.0, 0.0, 0.0), forward=(0.0, 0.0, -1.0), fov_deg=90.0,
+                           near_z=0.1, far_z=100.0)  # looking away: no fragment at all
+        else:
+            cam = S.Camera(position=(0.04 * k, -0.03 * k, 0.08 * k), forward=(0.0, 0.0, 1.0), fov_deg=90.0,
+                           near_z=0.1, far_z=100.0)
+        steps.append(cam)
+    opts = S.Opts(cull_mode=S.CULL_NONE)
+    with R.Renderer(W, H) as r:
+        r.add_scene(scene)
+        for k, cam in enumerate(steps):
+            if k in (3, 4):
+                r.render(cam, opts, R.BIN_EXACT, R.FLAG_GRAPH)
+            elif k == 8:
+                R.render_batch([(r, cam, opts, R.BIN_EXACT, 0)])
+            elif k == 9:
+                r.render(cam, opts, R.BIN_PREVFRAME)
+            else:
+                r.render(cam, opts)
+            vis, depth, rgba = r.readback()
+            ovis, odepth, orgba, _ = O.render(scene, cam, opts, W, H, threads=4)
+            assert_bitwise(f"vis, frame {k}", vis, ovis)
+            assert_bitwise(f"depth, frame {k}", depth, odepth)
+            assert_rgba_close(rgba, orgba)
