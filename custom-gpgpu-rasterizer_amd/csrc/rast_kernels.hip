@@ -65,6 +65,13 @@ __device__ __forceinline__ T ldg(const T *p)
 // load again): load them as the compiler's own 4-vectors.
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint2 ldg(const uint2 *p)
+{
+    const u32x2_t v = *(const __attribute__((address_space(1))) u32x2_t *)p;
+    return make_uint2(v.x, v.y);
+}
 
 __device__ __forceinline__ uint4 ldg(const uint4 *p)
 {
@@ -916,7 +923,13 @@ __global__ __launch_bounds__(256) void k_fallback_alloc(const JobDesc *__restric
     }
 }
 
-// Step 2 (per overflow entry): exact placement of the entries that did not fit.
+// Step 2 (per overflow entry): exact placement of the entries that did not
+// fit.  Entries of one tile come in runs (setup appends them per wave), and
+// returned atomics on one address serialize, so each wave takes a tile's
+// cursor once per distinct tile: the lanes of that tile get consecutive
+// slots from one atomic, and all of the wave's tiles go out in one atomic
+// instruction.  The order within a tile's list does not change the
+// frame (the depth keys order fragments by depth, then triangle ID).
 __global__ __launch_bounds__(256) void k_fallback_fill(const JobDesc *__restrict__ jobs,
                                                        const uint16_t *__restrict__ table)
 {
@@ -930,12 +943,40 @@ __global__ __launch_bounds__(256) void k_fallback_fill(const JobDesc *__restrict
     uint32_t n = J.counters->ovf_entries;
     n = n < J.ovf_cap ? n : J.ovf_cap;
     const uint32_t stride = J.nblocks[ST_FBFILL] * 256u;
-    for (uint32_t i = local * 256u + threadIdx.x; i < n; i += stride) {
-        const uint2 e = ovf[i];
-        const uint32_t po = tile_pool[e.x];
-        if (po == 0xFFFFFFFFu) continue;
-        const uint32_t pos = po + atomicAdd(fb_cursor + e.x, 1u);
-        if (pos < pool_cap) pool[pos] = e.y;
+    const uint32_t lane = lane_id();
+    // b is wave-uniform (stride and the wave's offset are multiples of 64)
+    for (uint32_t b = local * 256u + (threadIdx.x & ~63u); b < n; b += stride) {
+        const uint32_t i = b + lane;
+        uint2 e = make_uint2(0u, 0u);
+        uint32_t po = 0xFFFFFFFFu;
+        if (i < n) {
+            e = ldg(ovf + i);
+            po = ldg(tile_pool + e.x);
+        }
+        const bool todo = po != 0xFFFFFFFFu;
+        // group the lanes by tile (register work only), then the group
+        // leaders take their slots in one atomic instruction
+        int leader_of = (int)lane;
+        uint32_t rank = 0, group = 0;
+        for (uint64_t pend = __ballot(todo); pend;) {
+            const int leader = __builtin_ctzll(pend);
+            const uint32_t lt = (uint32_t)__builtin_amdgcn_readlane((int)e.x, leader);
+            const bool mine = todo && e.x == lt;
+            const uint64_t m = __ballot(mine);
+            if (mine) {
+                leader_of = leader;
+                rank = popc_below(m);
+            }
+            if (lane == (uint32_t)leader) group = (uint32_t)__popcll(m);
+            pend &= ~m;
+        }
+        uint32_t base = 0;
+        if (group) base = gatomic_add(fb_cursor + e.x, group);
+        base = (uint32_t)__shfl((int)base, leader_of);
+        if (todo) {
+            const uint32_t pos = po + base + rank;
+            if (pos < pool_cap) stg(pool + pos, e.y);
+        }
     }
 }
 
