@@ -732,7 +732,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         nb[ST_FBFILL] = single ? std::min<uint32_t>(ceil_div(ntris, 256), 256) : 0;
         const int res = r->tile == 8 ? ST_RES8 : (r->tile == 16 ? ST_RES16 : ST_RES32);
         nb[res] = ntiles;
-        nb[ST_SHADE] = job_shades(r) ? ceil_div(r->W, 64) * ceil_div(r->H, 4) : 0;
+        nb[ST_SHADE] = job_shades(r) ? ceil_div(r->W, 64) * ceil_div(r->H, (uint32_t)kShadeRows) : 0;
         if (nb[ST_SHADE]) L.shade_modes |= 1u << r->last_opts.sample_state;
         else ++L.unshaded_jobs;
         for (int st = 0; st < ST_COUNT; ++st) {

@@ -32,6 +32,7 @@ constexpr int kMaxMeshes = 1 << 16;
 constexpr int kNumFeatures = 8;
 // Tiles per workgroup of the capacity-model kernel (16 per wave).
 constexpr int kCapacityTilesPerBlock = 64;
+constexpr int kShadeRows = 8;  // pixel rows per shading block (64 wide; a multiple of 4)
 
 struct DevTexture {
     const uint32_t *texels;
@@ -87,7 +88,7 @@ enum Stage : int {
     ST_RES8,      // resolve, 8x8 tiles (one block per tile, in launch order)
     ST_RES16,     // resolve, 16x16 tiles
     ST_RES32,     // resolve, 32x32 tiles
-    ST_SHADE,     // 64x4-pixel blocks
+    ST_SHADE,     // 64 x kShadeRows-pixel blocks
     ST_COUNT
 };
 

@@ -1622,44 +1622,24 @@ __device__ __forceinline__ float byte_unit(uint32_t c)
 // in the reference (so the same texels are fetched); the other attributes and
 // the lighting terms use fused multiply-adds and the hardware reciprocal /
 // rsqrt / exp2 / log2, which keeps RGBA8 within 1/255 of the reference
-// formula.  64x4-pixel blocks (rows of 64 per wave: fully coalesced vis reads
-// and RGBA8 writes).  A pixel's data is a chain of dependent gathers (vis,
+// formula.  A wave shades rows of 64 pixels (fully coalesced vis reads and
+// RGBA8 writes).  A pixel's data is a chain of dependent gathers (vis,
 // triangle record, vertex attributes and texture table, texels); every
 // level's loads are issued together, with no branch between them, and range
 // checks are applied to the loaded values.  When all of a wave's pixels
 // belong to one mesh its texture table is read with scalar loads, and when
 // its four maps have one size the texel indices are computed once.
 template <bool kLinear>
-__global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
-                                               const uint16_t *__restrict__ table)
+__device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
+                                            uint32_t x, uint32_t y, uint32_t id)
 {
-    __shared__ float unit[256];
-    unit[threadIdx.x] = byte_unit(threadIdx.x);
-    __syncthreads();
-    uint32_t local;
-    const JobDesc &J = jobs[job_of(table, jobs, ST_SHADE, local)];
     const FrameParams &fp = J.fp;
-    if ((fp.sample_state != 0) != kLinear) return;  // the other sampling mode's instance shades this frame
     constexpr int mode = kLinear ? 1 : 0;
-    // the frame's counters are final once resolve has run: the job's first
-    // block copies them to the renderer's pinned host block (k_publish does
-    // this for jobs that are not shaded)
-    if (local == 0) {
-        __shared__ uint32_t red[4];
-        publish_counters(J, job_alive<4>(J, red));
-    }
-    const uint32_t *__restrict__ vis = J.vis;
     const float *__restrict__ depth = J.depth;
     const uint32_t *__restrict__ trirec = J.trirec;
     const float4 *__restrict__ attr = J.attr;
     uint32_t *__restrict__ rgba = J.rgba;
-    const uint32_t strip = local;
-    const uint32_t bw = (fp.width + 63u) / 64u;
-    const uint32_t x = (strip % bw) * 64u + (threadIdx.x & 63u);
-    const uint32_t y = (strip / bw) * 4u + (threadIdx.x >> 6);
-    if (x >= fp.width || y >= fp.height) return;
     const size_t pid = (size_t)y * fp.width + x;
-    const uint32_t id = ldg(vis + pid);
     if (id >= fp.num_tris) {
         stg(rgba + pid, (uint32_t)(fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u));
         return;
@@ -1774,6 +1754,44 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
         outr *= rev; outg *= rev; outb *= rev;
     }
     stg(rgba + pid, (uint32_t)(pack_rgba(outr, outg, outb)));
+}
+
+
+// A block shades 64 x kShadeRows pixels: lane x of wave w takes the pixels
+// (x, w), (x, w + 4), ...  The vis reads of all of them go out first, so each
+// further pixel of a lane waits for three dependent gathers instead of four.
+template <bool kLinear>
+__global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
+                                               const uint16_t *__restrict__ table)
+{
+    __shared__ float unit[256];
+    unit[threadIdx.x] = byte_unit(threadIdx.x);
+    __syncthreads();
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, ST_SHADE, local)];
+    const FrameParams &fp = J.fp;
+    if ((fp.sample_state != 0) != kLinear) return;  // the other sampling mode's instance shades this frame
+    // the frame's counters are final once resolve has run: the job's first
+    // block copies them to the renderer's pinned host block (k_publish does
+    // this for jobs that are not shaded)
+    if (local == 0) {
+        __shared__ uint32_t red[4];
+        publish_counters(J, job_alive<4>(J, red));
+    }
+    constexpr int P = kShadeRows / 4;
+    const uint32_t bw = (fp.width + 63u) / 64u;
+    const uint32_t x = (local % bw) * 64u + (threadIdx.x & 63u);
+    const uint32_t y0 = (local / bw) * (uint32_t)kShadeRows + (threadIdx.x >> 6);
+    if (x >= fp.width || y0 >= fp.height) return;
+    uint32_t ids[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const uint32_t y = y0 + 4u * k;
+        ids[k] = y < fp.height ? ldg(J.vis + (size_t)y * fp.width + x) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        if (y0 + 4u * k < fp.height) shade_pixel<kLinear>(meshes, J, unit, x, y0 + 4u * k, ids[k]);
 }
 
 // Copies the counters of every job that is not shaded to its renderer's
