@@ -27,6 +27,9 @@ namespace rast {
 
 constexpr uint32_t kVisNone = 0xFFFFFFFFu;
 constexpr int kIVertexFloats = 14;
+// Setup record, 16 u32 per triangle: sx0 sy0 sx1 sy1 | sx2 sy2 z0 area |
+// z2 z1 bbox-lo bbox-hi | i0 i1 i2 mesh (area next to the screen positions,
+// so shading reads quads 0, 1 and 3 only)
 constexpr int kTriRecWords = 16;
 constexpr int kMaxMeshes = 1 << 16;
 constexpr int kNumFeatures = 8;

@@ -406,8 +406,8 @@ __device__ __forceinline__ void load_tri(const uint32_t *__restrict__ rec, uint3
     t.sx0 = __uint_as_float(q0.x); t.sy0 = __uint_as_float(q0.y);
     t.sx1 = __uint_as_float(q0.z); t.sy1 = __uint_as_float(q0.w);
     t.sx2 = __uint_as_float(q1.x); t.sy2 = __uint_as_float(q1.y);
-    t.z0 = __uint_as_float(q1.z); t.z1 = __uint_as_float(q1.w);
-    t.z2 = __uint_as_float(q2.x); t.area = __uint_as_float(q2.y);
+    t.z0 = __uint_as_float(q1.z); t.area = __uint_as_float(q1.w);
+    t.z2 = __uint_as_float(q2.x); t.z1 = __uint_as_float(q2.y);
     t.xr = q2.z;  // bbox lo (xmin | ymin << 16)
     t.yr = q2.w;  // bbox hi (xmax | ymax << 16)
     set_edges(t);
@@ -646,8 +646,8 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                 stg(r, make_uint4(__float_as_uint(t.sx0), __float_as_uint(t.sy0), __float_as_uint(t.sx1),
                                   __float_as_uint(t.sy1)));
                 stg(r + 1, make_uint4(__float_as_uint(t.sx2), __float_as_uint(t.sy2), __float_as_uint(t.z0),
-                                      __float_as_uint(t.z1)));
-                stg(r + 2, make_uint4(__float_as_uint(t.z2), __float_as_uint(t.area),
+                                      __float_as_uint(t.area)));
+                stg(r + 2, make_uint4(__float_as_uint(t.z2), __float_as_uint(t.z1),
                                       (uint32_t)xmin | ((uint32_t)ymin << 16), (uint32_t)xmax | ((uint32_t)ymax << 16)));
                 stg(r + 3, make_uint4(i0, i1, i2, mi));
             }
@@ -1645,11 +1645,11 @@ __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, 
         return;
     }
     const uint4 *rp = reinterpret_cast<const uint4 *>(trirec + (size_t)id * kTriRecWords);
-    const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q2 = ldg(rp + 2), q3 = ldg(rp + 3);
+    const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q3 = ldg(rp + 3);  // q2 holds z2, z1 and the bbox
     const float sx0 = __uint_as_float(q0.x), sy0 = __uint_as_float(q0.y);
     const float sx1 = __uint_as_float(q0.z), sy1 = __uint_as_float(q0.w);
     const float sx2 = __uint_as_float(q1.x), sy2 = __uint_as_float(q1.y);
-    const float area = __uint_as_float(q2.y);
+    const float area = __uint_as_float(q1.w);
     const float4 *A0 = attr + (size_t)q3.x * 4, *A1 = attr + (size_t)q3.y * 4, *A2 = attr + (size_t)q3.z * 4;
     const float4 a00 = ldg(A0), a10 = ldg(A1), a20 = ldg(A2);
     const float4 a01 = ldg(A0 + 1), a11 = ldg(A1 + 1), a21 = ldg(A2 + 1);

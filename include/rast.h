@@ -234,7 +234,8 @@ int rast_frame_buffers(rast_renderer *r, void **vis, void **depth, void **rgba);
 enum {
     RAST_DEBUG_VERTEX_POS = 0,  /* float4 per vertex (ndc x, y, z, clip w), all meshes */
     RAST_DEBUG_VERTEX_ATTR = 1, /* 16 floats per vertex */
-    RAST_DEBUG_TRI_RECORDS = 2, /* 16 words per triangle (valid for survivors) */
+    RAST_DEBUG_TRI_RECORDS = 2, /* 16 words per triangle (valid for survivors): sx0 sy0 sx1 sy1 sx2 sy2 z0 area
+                                   z2 z1 bbox-lo bbox-hi i0 i1 i2 mesh */
     RAST_DEBUG_TRI_ALIVE = 3,   /* u8 per triangle */
     RAST_DEBUG_TILE_COUNTS = 4, /* u32 per tile: queue length */
     RAST_DEBUG_BIG_LIST = 5,    /* u32 count followed by the IDs */

@@ -102,7 +102,9 @@ def test_setup_records_bitwise():
     oflags, orec = O.setup(w.scene, w.camera, w.opts.cull_mode, w.width, w.height)
     np.testing.assert_array_equal(alive, oflags)
     live = oflags.astype(bool)
-    assert_bitwise("trirec", rec[live][:, :12], orec[live])
+    # GPU words 0-11: sx0 sy0 sx1 sy1 sx2 sy2 z0 area z2 z1 lo hi; oracle: ... z0 z1 z2 area lo hi
+    gpu_in_oracle_order = rec[live][:, [0, 1, 2, 3, 4, 5, 6, 9, 8, 7, 10, 11]]
+    assert_bitwise("trirec", np.ascontiguousarray(gpu_in_oracle_order), orec[live])
 
 
 @pytest.mark.parametrize("tile", [8, 16, 32])
