@@ -313,6 +313,7 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
     plan_free(r->plan);
     rast_model_detach_renderer(r->model, r);
     for (rast_mesh *m : r->meshes) free_mesh(m);
+    for (TexQuad &q : r->tex_quads) (void)hipFree(q.texels);
     for (rast_texture *t : r->textures) {
         if (t->texels) (void)hipFree(t->texels);
         delete t;
@@ -516,9 +517,18 @@ extern "C" int rast_texture_destroy(rast_texture *t)
     (void)hipSetDevice(r->device);
     HIP_TRY(renderer_wait(r));
     for (rast_renderer *v : r->views) HIP_TRY(renderer_wait(v));
-    for (rast_mesh *m : r->meshes)
+    for (rast_mesh *m : r->meshes) {
         for (auto &slot : m->tex)
             if (slot == t) slot = nullptr;
+        m->texq = nullptr;  // set again by the next frame
+    }
+    for (size_t i = r->tex_quads.size(); i-- > 0;) {
+        const TexQuad &q = r->tex_quads[i];
+        if (q.t[0] == t || q.t[1] == t || q.t[2] == t || q.t[3] == t) {
+            (void)hipFree(q.texels);
+            r->tex_quads.erase(r->tex_quads.begin() + (std::ptrdiff_t)i);
+        }
+    }
     r->textures.erase(std::remove(r->textures.begin(), r->textures.end(), t), r->textures.end());
     if (t->texels) (void)hipFree(t->texels);
     delete t;
@@ -598,11 +608,49 @@ static int validate_camera(const rast_camera *cam)
 
 // Host part of a renderer's next frame (r->last_cam / last_opts): scene
 // layout, storage and the binning mode.
+// Points each mesh whose four maps exist and share one size at their
+// interleaved copy, building it on first use (once per set of four
+// textures; freed with any of them).  Sets above 2^28 texels keep the
+// separate maps: shading addresses the copy with 32-bit byte offsets.
+static int ensure_tex_quads(rast_renderer *r)
+{
+    rast_renderer *owner = scene_owner(r);
+    for (rast_mesh *m : scene_meshes(r)) {
+        m->texq = nullptr;
+        rast_texture *const *t = m->tex;
+        if (!t[0] || !t[1] || !t[2] || !t[3]) continue;
+        const uint32_t w = t[0]->w, h = t[0]->h;
+        bool same = true;
+        for (int k = 1; k < 4; ++k) same = same && t[k]->w == w && t[k]->h == h;
+        if (!same || (uint64_t)w * h > (1ull << 28)) continue;
+        TexQuad *found = nullptr;
+        for (TexQuad &q : owner->tex_quads)
+            if (q.t[0] == t[0] && q.t[1] == t[1] && q.t[2] == t[2] && q.t[3] == t[3]) found = &q;
+        if (!found) {
+            TexQuad q = { { t[0], t[1], t[2], t[3] }, nullptr };
+            const size_t n = (size_t)w * h;
+            HIP_TRY(hipMalloc((void **)&q.texels, n * sizeof(uint4)));
+            hipError_t e = launch_interleave_maps(t[0]->texels, t[1]->texels, t[2]->texels, t[3]->texels, q.texels, n,
+                                                  r->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+            if (e != hipSuccess) {
+                (void)hipFree(q.texels);
+                return rast_set_error(RAST_E_HIP, "interleaving the maps: %s", hipGetErrorString(e));
+            }
+            owner->tex_quads.push_back(q);
+            found = &owner->tex_quads.back();
+        }
+        m->texq = found->texels;
+    }
+    return RAST_OK;
+}
+
 static int prepare_job(rast_renderer *r)
 {
     HIP_TRY(hipSetDevice(r->device));
     if (scene_meshes(r).empty()) return rast_set_error(RAST_E_STATE, "rast_render: no meshes loaded");
     if (r->layout_version != scene_owner(r)->scene_version) refresh_layout(r);
+    RAST_TRY(ensure_tex_quads(r));
     RAST_TRY(ensure_frame_storage(r, ceil_div(r->W, r->tile) * ceil_div(r->H, r->tile)));
     r->frame_had_prev = r->have_prev;
     bool single = r->last_opts.bin_policy != RAST_BIN_EXACT && r->have_prev;
@@ -932,6 +980,7 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
                 if (m->tex[q]) d.tex[q] = { m->tex[q]->texels, m->tex[q]->w, m->tex[q]->h };
                 else d.tex[q] = { dflt + q, 1u, 1u };
             }
+            d.texq = m->texq;
             vb += m->nv;
         }
     }

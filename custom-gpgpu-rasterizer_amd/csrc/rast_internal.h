@@ -61,6 +61,7 @@ struct DevMesh {
     float cam_pos[4];         // camera position (xyz) and forward (xyz) of the frame
     float cam_fwd[4];
     DevTexture tex[4];        // diffuse, normal, specular, gloss (stand-ins filled in)
+    const uint4 *texq;        // the four maps interleaved, one 16-byte texel each (null unless one size)
 };
 
 // Per-frame values that do not depend on the camera (the camera travels in

@@ -433,6 +433,16 @@ __global__ __launch_bounds__(256) void k_frame_init(const JobDesc *__restrict__ 
     if (i < sizeof(FrameCounters) / 4) reinterpret_cast<uint32_t *>(J.counters)[i] = 0;
 }
 
+// Texel i of the interleaved copy holds texel i of the diffuse, normal,
+// specular and gloss maps (one size), in that order.
+__global__ __launch_bounds__(256) void k_interleave_maps(const uint32_t *__restrict__ d, const uint32_t *__restrict__ n,
+                                                         const uint32_t *__restrict__ sp, const uint32_t *__restrict__ g,
+                                                         uint4 *__restrict__ out, size_t count)
+{
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256u)
+        out[i] = make_uint4(d[i], n[i], sp[i], g[i]);
+}
+
 __global__ __launch_bounds__(256) void k_clear_keys(unsigned long long *__restrict__ keys, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u) keys[i] = kClearKey;
@@ -1605,6 +1615,36 @@ __device__ __forceinline__ MapColours sample_maps(const DevTexture (&tex)[4], in
     return o;
 }
 
+// sample_maps for the interleaved copy of four same-size maps (texel
+// indices from the diffuse map's size): one 16-byte load per tap instead of
+// one 4-byte load per tap and map; the same texels, decoded the same way.
+__device__ __forceinline__ MapColours sample_quads(const uint4 *texq, const DevTexture &t0, int mode, float u, float v,
+                                                   const float *unit)
+{
+    MapColours o;
+    float gg, gb;
+    const Taps tp = texel_taps(t0, u, v);
+    const char *base = reinterpret_cast<const char *>(texq);
+    auto quad = [base](int i) { return ldg(reinterpret_cast<const uint4 *>(base + (uint32_t)i * 16u)); };
+    if (mode == 0) {
+        const uint4 c = quad(tp.c);
+        decode_point(c.x, tp.ok, unit, o.d[0], o.d[1], o.d[2]);
+        decode_point(c.y, tp.ok, unit, o.n[0], o.n[1], o.n[2]);
+        decode_point(c.z, tp.ok, unit, o.s[0], o.s[1], o.s[2]);
+        decode_point(c.w, tp.ok, unit, o.g, gg, gb);
+    } else {
+        const uint4 c = quad(tp.c), n0 = quad(tp.n[0]), n1 = quad(tp.n[1]), n2 = quad(tp.n[2]), n3 = quad(tp.n[3]);
+        const uint32_t sd[4] = { n0.x, n1.x, n2.x, n3.x }, sn[4] = { n0.y, n1.y, n2.y, n3.y };
+        const uint32_t ss[4] = { n0.z, n1.z, n2.z, n3.z }, sg[4] = { n0.w, n1.w, n2.w, n3.w };
+        decode_linear(sd, c.x, tp.ok, o.d[0], o.d[1], o.d[2]);
+        decode_linear(sn, c.y, tp.ok, o.n[0], o.n[1], o.n[2]);
+        decode_linear(ss, c.z, tp.ok, o.s[0], o.s[1], o.s[2]);
+        decode_linear(sg, c.w, tp.ok, o.g, gg, gb);
+    }
+    (void)gg; (void)gb;
+    return o;
+}
+
 // c / 255.f for a byte value c: the product with the rounded reciprocal and
 // one residual correction (fma), which gives the correctly rounded quotient
 // for every c in [0, 255].
@@ -1628,7 +1668,8 @@ __device__ __forceinline__ float byte_unit(uint32_t c)
 // level's loads are issued together, with no branch between them, and range
 // checks are applied to the loaded values.  When all of a wave's pixels
 // belong to one mesh its texture table is read with scalar loads, and when
-// its four maps have one size the texel indices are computed once.
+// its four maps have one size the texel indices are computed once and the
+// texels come from the maps' interleaved copy (one 16-byte load per tap).
 template <bool kLinear>
 __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
                                             uint32_t x, uint32_t y, uint32_t id)
@@ -1661,11 +1702,13 @@ __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, 
     // per-lane one is read only where the maps are sampled, after the
     // attributes have been interpolated
     DevTexture stex[4] = {};
+    const uint4 *texq = nullptr;
     int has_diffuse;
     if (uniform) {
         const DevMesh &M = meshes[mi0];
 #pragma unroll
         for (int k = 0; k < 4; ++k) stex[k] = M.tex[k];
+        texq = M.texq;
         has_diffuse = M.has_diffuse;
     } else {
         has_diffuse = ldg(&meshes[mi].has_diffuse);
@@ -1707,6 +1750,8 @@ __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, 
 #pragma unroll
         for (int k = 0; k < 4; ++k) vtex[k] = ldg(meshes[mi].tex + k);
         c = sample_maps<false>(vtex, mode, u, v, unit);
+    } else if (texq) {
+        c = sample_quads(texq, stex[0], mode, u, v, unit);
     } else if (stex[1].w == stex[0].w && stex[2].w == stex[0].w && stex[3].w == stex[0].w &&
              stex[1].h == stex[0].h && stex[2].h == stex[0].h && stex[3].h == stex[0].h)
         c = sample_maps<true>(stex, mode, u, v, unit);
@@ -1812,6 +1857,15 @@ inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 hipError_t launch_frame_init(const Launch &L, hipStream_t s)
 {
     hipLaunchKernelGGL(k_frame_init, dim3(L.grid[ST_INIT]), dim3(256), 0, s, L.jobs, L.table[ST_INIT]);
+    return hipGetLastError();
+}
+
+hipError_t launch_interleave_maps(const uint32_t *d, const uint32_t *n, const uint32_t *sp, const uint32_t *g,
+                                  uint4 *out, size_t count, hipStream_t s)
+{
+    const size_t blocks = (count + 255) / 256;
+    hipLaunchKernelGGL(k_interleave_maps, dim3(blocks < 4096 ? (uint32_t)blocks : 4096u), dim3(256), 0, s, d, n, sp, g,
+                       out, count);
     return hipGetLastError();
 }
 

@@ -32,6 +32,8 @@ struct Launch {
 
 // Resets a key buffer to depth 1 / no triangle (before a renderer's first frame).
 hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s);
+hipError_t launch_interleave_maps(const uint32_t *d, const uint32_t *n, const uint32_t *sp, const uint32_t *g,
+                                  uint4 *out, size_t count, hipStream_t s);
 hipError_t launch_frame_init(const Launch &L, hipStream_t s);
 hipError_t launch_vertex(const Launch &L, hipStream_t s);
 hipError_t launch_capacity(const Launch &L, hipStream_t s);  // rast_mlp.hip

@@ -74,6 +74,14 @@ struct rast_texture {
     uint32_t w = 0, h = 0;
 };
 
+// The four maps of a mesh interleaved (texel i = diffuse, normal, specular,
+// gloss words of texel i), built when all four exist and share one size;
+// shading then reads one 16-byte texel instead of four 4-byte ones.
+struct TexQuad {
+    rast_texture *t[4];
+    uint4 *texels;
+};
+
 struct rast_mesh {
     rast_renderer *r = nullptr;
     float *vert16 = nullptr;  // 16 floats per vertex (IVertex + 2 pad floats)
@@ -82,6 +90,7 @@ struct rast_mesh {
     int topology = 0;
     float world[16];
     rast_texture *tex[4] = { nullptr, nullptr, nullptr, nullptr };
+    const uint4 *texq = nullptr;  // interleaved maps for tex[0..3], set by prepare_job
     uint64_t tri_base = 0;
 };
 
@@ -92,6 +101,7 @@ struct rast_renderer {
     uint32_t tile = 16, big_threshold = 256, small_limit = 16;
     std::vector<rast_mesh *> meshes;
     std::vector<rast_texture *> textures;
+    std::vector<TexQuad> tex_quads;        // interleaved map sets of this renderer's textures
     rast_renderer *scene_src = nullptr;   // set for views: the renderer whose meshes they draw
     std::vector<rast_renderer *> views;    // views drawing this renderer's meshes
     uint64_t scene_version = 1;            // bumped when the mesh list changes

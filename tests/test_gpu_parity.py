@@ -309,3 +309,40 @@ def test_frame_sequence_matches_oracle_every_frame():
             assert_bitwise(f"vis, frame {k}", vis, ovis)
             assert_bitwise(f"depth, frame {k}", depth, odepth)
             assert_rgba_close(rgba, orgba)
+
+
+@pytest.mark.parametrize("sample", [S.SAMPLE_POINT, S.SAMPLE_LINEAR])
+def test_interleaved_maps_follow_texture_changes(sample):
+    """Four maps of one size are sampled from an interleaved copy built on
+    first use.  The frames stay the oracle's when the mesh is bound to another
+    set of maps, when one map of the set is destroyed (its stand-in takes
+    over and the copy is dropped) and when the first set is bound again."""
+    m = S.uv_sphere(40, 20)
+    first = [S.noise_texture(96, 64, 31 + k, k) for k in range(4)]
+    second = [S.noise_texture(96, 64, 41 + k, k) for k in range(4)]
+    m.textures = first
+    scene = S.Scene([m])
+    cam, opts, W, H = S.Camera(), S.Opts(sample_state=sample), 320, 240
+    lib = R.lib()
+
+    def same_as_oracle(r, textures, what):
+        m.textures = textures
+        r.render(cam, opts)
+        vis, depth, rgba = r.readback()
+        ovis, odepth, orgba, _ = O.render(scene, cam, opts, W, H, threads=4)
+        assert_bitwise(f"vis, {what}", vis, ovis)
+        assert_bitwise(f"depth, {what}", depth, odepth)
+        assert_rgba_close(rgba, orgba)
+
+    with R.Renderer(W, H) as r:
+        r.add_scene(scene)
+        mh = r.meshes[0]
+        same_as_oracle(r, first, "first set")
+        hs = [r.texture(t) for t in second]
+        R.check(lib.rast_mesh_set_textures(mh, *hs), "rast_mesh_set_textures")
+        same_as_oracle(r, second, "second set")
+        R.check(lib.rast_texture_destroy(hs[1]), "rast_texture_destroy")
+        del r.textures[id(second[1])]
+        same_as_oracle(r, [second[0], None, second[2], second[3]], "normal map destroyed")
+        R.check(lib.rast_mesh_set_textures(mh, *[r.texture(t) for t in first]), "rast_mesh_set_textures")
+        same_as_oracle(r, first, "first set again")
