@@ -228,6 +228,9 @@ extern "C" int rast_camera_frame(const rast_camera *cam, uint32_t width, uint32_
 
 // Timing events of one frame, recorded between the stages when RAST_FLAG_TIMING is set.
 enum { EV_START, EV_VERTEX, EV_MODEL, EV_SCANCAP, EV_SETUP, EV_SCANCNT, EV_FILL, EV_FALLBACK, EV_RASTER, EV_SHADE, EV_COUNT };
+// the start / stop events of rast_timer_start / rast_timer_stop
+enum { EV_TIMER_START = EV_COUNT, EV_TIMER_STOP, EV_ALL };
+static_assert(EV_ALL <= (int)(sizeof(((rast_renderer *)nullptr)->ev) / sizeof(hipEvent_t)), "renderer event slots");
 
 static int alloc_framebuffers(rast_renderer *r)
 {
@@ -260,7 +263,7 @@ static int new_renderer(int device, uint32_t width, uint32_t height, rast_render
     int rc = RAST_OK;
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) rc = RAST_E_HIP;
     if (rc == RAST_OK && hipEventCreateWithFlags(&r->join, hipEventDisableTiming) != hipSuccess) rc = RAST_E_HIP;
-    for (int i = 0; rc == RAST_OK && i < EV_COUNT; ++i)
+    for (int i = 0; rc == RAST_OK && i < EV_ALL; ++i)
         if (hipEventCreate(&r->ev[i]) != hipSuccess) rc = RAST_E_HIP;
     if (rc == RAST_OK && hipHostMalloc((void **)&r->h_counters, sizeof(FrameCounters)) != hipSuccess) rc = RAST_E_HIP;
     if (rc == RAST_OK && hipHostGetDevicePointer((void **)&r->d_counters_host, r->h_counters, 0) != hipSuccess)
@@ -298,6 +301,19 @@ static void free_mesh(rast_mesh *m)
     delete m;
 }
 
+// Frees a mesh's device memory and detaches it from its renderer; the handle
+// stays allocated until rast_mesh_destroy.
+static void orphan_mesh(rast_mesh *m)
+{
+    if (m->vert16) (void)hipFree(m->vert16);
+    if (m->idx) (void)hipFree(m->idx);
+    m->vert16 = nullptr;
+    m->idx = nullptr;
+    for (auto &t : m->tex) t = nullptr;
+    m->texq = nullptr;
+    m->r = nullptr;
+}
+
 extern "C" int rast_renderer_destroy(rast_renderer *r)
 {
     if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_renderer_destroy: null renderer");
@@ -312,11 +328,17 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
     }
     plan_free(r->plan);
     rast_model_detach_renderer(r->model, r);
-    for (rast_mesh *m : r->meshes) free_mesh(m);
+    // Mesh and texture handles outlive the renderer as empty handles: their
+    // device memory goes now, the handle itself with rast_mesh_destroy /
+    // rast_texture_destroy, so a caller may destroy the renderer before the
+    // objects that hold those handles (the reference app deletes its
+    // CUDARenderer before its CUDATextureManager, CCR/main.cpp:418,529).
+    for (rast_mesh *m : r->meshes) orphan_mesh(m);
     for (TexQuad &q : r->tex_quads) (void)hipFree(q.texels);
     for (rast_texture *t : r->textures) {
         if (t->texels) (void)hipFree(t->texels);
-        delete t;
+        t->texels = nullptr;
+        t->r = nullptr;
     }
     DBuf *bufs[] = { &r->pos4, &r->attr, &r->trirec,
                      &r->alive, &r->survivors, &r->tile_count[0], &r->tile_count[1], &r->tile_off, &r->tile_cap,
@@ -420,6 +442,10 @@ extern "C" int rast_mesh_destroy(rast_mesh *m)
 {
     if (!m) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_destroy: null mesh");
     rast_renderer *r = m->r;
+    if (!r) {  // its renderer is gone (and with it the device memory)
+        delete m;
+        return RAST_OK;
+    }
     (void)hipSetDevice(r->device);
     HIP_TRY(renderer_wait(r));
     for (rast_renderer *v : r->views) {
@@ -445,6 +471,7 @@ extern "C" int rast_mesh_set_textures(rast_mesh *m, rast_texture *d, rast_textur
                                       rast_texture *g)
 {
     if (!m) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_set_textures: null mesh");
+    if (!m->r) return rast_set_error(RAST_E_STATE, "rast_mesh_set_textures: the mesh's renderer was destroyed");
     rast_texture *t[4] = { d, n, s, g };
     for (int i = 0; i < 4; ++i)
         if (t[i] && t[i]->r != m->r) return rast_set_error(RAST_E_INVALID_ARG, "texture belongs to another renderer");
@@ -462,6 +489,7 @@ extern "C" int rast_mesh_num_triangles(const rast_mesh *m, uint32_t *out)
 extern "C" int rast_mesh_triangle_base(const rast_mesh *m, uint64_t *out)
 {
     if (!m || !out) return rast_set_error(RAST_E_INVALID_ARG, "rast_mesh_triangle_base: null argument");
+    if (!m->r) return rast_set_error(RAST_E_STATE, "rast_mesh_triangle_base: the mesh's renderer was destroyed");
     uint64_t base = 0;
     for (rast_mesh *q : m->r->meshes) {
         if (q == m) break;
@@ -514,6 +542,10 @@ extern "C" int rast_texture_destroy(rast_texture *t)
 {
     if (!t) return rast_set_error(RAST_E_INVALID_ARG, "rast_texture_destroy: null texture");
     rast_renderer *r = t->r;
+    if (!r) {  // its renderer is gone (and with it the texels)
+        delete t;
+        return RAST_OK;
+    }
     (void)hipSetDevice(r->device);
     HIP_TRY(renderer_wait(r));
     for (rast_renderer *v : r->views) HIP_TRY(renderer_wait(v));
@@ -629,7 +661,15 @@ static int ensure_tex_quads(rast_renderer *r)
         if (!found) {
             TexQuad q = { { t[0], t[1], t[2], t[3] }, nullptr };
             const size_t n = (size_t)w * h;
-            HIP_TRY(hipMalloc((void **)&q.texels, n * sizeof(uint4)));
+            const hipError_t me = hipMalloc((void **)&q.texels, n * sizeof(uint4));
+            if (me == hipErrorOutOfMemory) {
+                // the copy is an optimisation only: without room for it the
+                // frame samples the four separate maps
+                (void)hipGetLastError();
+                q.texels = nullptr;
+                continue;
+            }
+            HIP_TRY(me);
             hipError_t e = launch_interleave_maps(t[0]->texels, t[1]->texels, t[2]->texels, t[3]->texels, q.texels, n,
                                                   r->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
@@ -1221,6 +1261,10 @@ extern "C" int rast_get_stats(rast_renderer *r, rast_stats *out)
 extern "C" int rast_frame_buffers(rast_renderer *r, void **vis, void **depth, void **rgba)
 {
     if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_frame_buffers: null renderer");
+    // the frame is finished and checked first (re-rendered if its queue
+    // storage overflowed), as for rast_readback
+    RAST_TRY(rast_sync(r));
+    if (!r->frame_valid) return rast_set_error(RAST_E_STATE, "rast_frame_buffers: no rendered frame");
     if (vis) *vis = r->vis.p;
     if (depth) *depth = r->depth.p;
     if (rgba) *rgba = r->rgba.p;
@@ -1264,9 +1308,53 @@ extern "C" int rast_renderer_set_binmodel(rast_renderer *r, rast_binmodel *m, fl
 {
     if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_renderer_set_binmodel: null renderer");
     if (!(margin >= 1.f) || margin > 64.f) return rast_set_error(RAST_E_INVALID_ARG, "margin %g not in [1, 64]", margin);
+    if (slack > (1u << 20)) return rast_set_error(RAST_E_INVALID_ARG, "slack %u above 2^20", slack);
     r->model = m;
     r->margin = margin;
     r->slack = slack;
+    return RAST_OK;
+}
+
+// ------------------------------------------------------------ timer, hooks
+
+extern "C" int rast_timer_start(rast_renderer *r)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_timer_start: null renderer");
+    HIP_TRY(hipSetDevice(r->device));
+    HIP_TRY(hipEventRecord(r->ev[EV_TIMER_START], r->stream));
+    r->timer_started = true;
+    return RAST_OK;
+}
+
+extern "C" int rast_timer_stop(rast_renderer *r, float *ms)
+{
+    if (!r || !ms) return rast_set_error(RAST_E_INVALID_ARG, "rast_timer_stop: null argument");
+    if (!r->timer_started) return rast_set_error(RAST_E_STATE, "rast_timer_stop: timer not started");
+    HIP_TRY(hipSetDevice(r->device));
+    // a frame from a batch ran on the batch stream: the stop event follows it
+    if (r->batch_seq) HIP_TRY(hipStreamWaitEvent(r->stream, g_bsync.done, 0));
+    HIP_TRY(hipEventRecord(r->ev[EV_TIMER_STOP], r->stream));
+    HIP_TRY(hipEventSynchronize(r->ev[EV_TIMER_STOP]));
+    HIP_TRY(hipEventElapsedTime(ms, r->ev[EV_TIMER_START], r->ev[EV_TIMER_STOP]));
+    return RAST_OK;
+}
+
+extern "C" int rast_draw_texture(rast_renderer *r, const rast_texture *t, int mode, int stretch, int sample_state)
+{
+    if (!r || !t) return rast_set_error(RAST_E_INVALID_ARG, "rast_draw_texture: null argument");
+    if (!t->r || !t->texels) return rast_set_error(RAST_E_STATE, "rast_draw_texture: the texture's renderer was destroyed");
+    if (mode != RAST_DRAW_TEXTURE && mode != RAST_DRAW_TEXTURE_GLOBAL)
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_draw_texture: unknown mode %d", mode);
+    if (sample_state != RAST_SAMPLE_POINT && sample_state != RAST_SAMPLE_LINEAR)
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_draw_texture: sample_state %d", sample_state);
+    HIP_TRY(hipSetDevice(r->device));
+    if (r->frame_pending) RAST_TRY(check_frame(r));
+    HIP_TRY(renderer_wait(r));  // earlier frames (also from batches) are done with the buffers
+    HIP_TRY(launch_draw_texture((uint32_t *)r->vis.p, (float *)r->depth.p, (uint32_t *)r->rgba.p, r->W, r->H,
+                                t->texels, t->w, t->h, mode, stretch != 0, sample_state, r->stream));
+    r->stream_busy = true;
+    r->frame_pending = false;
+    r->frame_valid = true;
     return RAST_OK;
 }
 

@@ -598,7 +598,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
         uint32_t i0, i1, i2;
         if (M.topology == 0) {
             i0 = ldg(M.idx + 3 * (size_t)k); i1 = ldg(M.idx + 3 * (size_t)k + 1); i2 = ldg(M.idx + 3 * (size_t)k + 2);
-        } else {  // strip: odd triangles swap their last two indices
+        } else {  // strip (TA_Kernel, CCR/CUDARenderer.cu:733-746): odd triangles swap their last two indices
             const bool odd = (k & 1u) != 0;
             i0 = ldg(M.idx + k);
             i1 = ldg(M.idx + k + (odd ? 2 : 1));
@@ -784,61 +784,79 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
 // Exclusive scan of a job's per-tile values by one 1024-thread workgroup:
 // out_off[n] = total.  is_capacity: in = capacities, the total is the
 // capacity sum; otherwise in = exact counts and the total is the pair count.
+// Sums are formed in 64 bits.  Capacities are hints: a tile whose region
+// would end past the queue storage keeps only the room that is left (its
+// capacity is lowered in place, the rest of its entries go to the exact
+// fallback), so capacities never overflow the storage.  Exact counts beyond
+// the storage set storage_overflow (the frame is rendered again with a larger
+// queue); the offsets of such tiles are clamped to the storage end.
 __global__ __launch_bounds__(1024) void k_scan(const JobDesc *__restrict__ jobs, int is_capacity)
 {
     const JobDesc &J = jobs[blockIdx.x];  // one workgroup per job
     if ((J.single != 0) != (is_capacity != 0)) return;
     const uint32_t n = J.fp.tiles_x * J.fp.tiles_y;
-    const uint32_t *__restrict__ in = is_capacity ? J.tile_cap : J.tile_count;
+    uint32_t *__restrict__ in = is_capacity ? J.tile_cap : J.tile_count;
     uint32_t *__restrict__ out_off = J.tile_off;
     uint32_t *__restrict__ zero_cursor = is_capacity ? nullptr : J.tile_cursor;
-    const uint32_t limit = J.queue_cap;
+    const uint64_t limit = J.queue_cap;
     FrameCounters *__restrict__ ctr = J.counters;
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry;
+    __shared__ uint64_t wsum[16];
+    __shared__ uint64_t carry;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid == 0) carry = 0;
     __syncthreads();
     // chunks of 4096 values: each thread owns 4 consecutive ones (one uint4)
     for (uint32_t base = 0; base < n; base += 4096) {
         const uint32_t i = base + tid * 4;
-        uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+        uint32_t v[4] = { 0, 0, 0, 0 };
         if (i + 3 < n) {
             const uint4 q = *reinterpret_cast<const uint4 *>(in + i);
-            v0 = q.x; v1 = q.y; v2 = q.z; v3 = q.w;
+            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
         } else {
-            if (i < n) v0 = in[i];
-            if (i + 1 < n) v1 = in[i + 1];
-            if (i + 2 < n) v2 = in[i + 2];
+            for (uint32_t q = 0; q < 3; ++q)
+                if (i + q < n) v[q] = in[i + q];
         }
-        const uint32_t s = v0 + v1 + v2 + v3;
-        uint32_t incl = s;  // inclusive wave prefix
+        const uint64_t s = (uint64_t)v[0] + v[1] + v[2] + v[3];
+        uint64_t incl = s;  // inclusive wave prefix
         for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o);
+            const uint64_t y = __shfl_up(incl, o);
             if (lane >= (uint32_t)o) incl += y;
         }
         if (lane == 63) wsum[wave] = incl;
         __syncthreads();
-        uint32_t wbase = carry;
+        uint64_t wbase = carry;
         for (uint32_t w = 0; w < wave; ++w) wbase += wsum[w];
-        uint32_t run = wbase + incl - s;
-        if (i < n) { out_off[i] = run; run += v0; }
-        if (i + 1 < n) { out_off[i + 1] = run; run += v1; }
-        if (i + 2 < n) { out_off[i + 2] = run; run += v2; }
-        if (i + 3 < n) { out_off[i + 3] = run; }
+        uint64_t run = wbase + incl - s;
+        for (uint32_t q = 0; q < 4; ++q) {
+            if (i + q >= n) break;
+            out_off[i + q] = (uint32_t)(run < limit ? run : limit);
+            if (is_capacity) {
+                // the region [run, run + cap) must end inside the storage
+                const uint64_t room = run < limit ? limit - run : 0ull;
+                if (v[q] > room) in[i + q] = (uint32_t)room;
+                run += v[q] < room ? v[q] : room;
+            } else {
+                run += v[q];
+            }
+        }
         if (zero_cursor) {
             if (i + 3 < n) *reinterpret_cast<uint4 *>(zero_cursor + i) = make_uint4(0, 0, 0, 0);
             else for (uint32_t q = i; q < n && q < i + 4; ++q) zero_cursor[q] = 0;
         }
         __syncthreads();
+        // the chunk's total: the running sum of the last thread that holds values
         if (tid == 1023) carry = wbase + incl;
         __syncthreads();
     }
     if (tid == 0) {
-        out_off[n] = carry;
-        if (is_capacity) ctr->capacity_total = carry;
-        else ctr->total_pairs = carry;
-        if (carry > limit) ctr->storage_overflow = 1;
+        const uint64_t total = carry;
+        out_off[n] = (uint32_t)(total < limit ? total : limit);
+        if (is_capacity) {
+            ctr->capacity_total = (uint32_t)(total < limit ? total : limit);
+        } else {
+            ctr->total_pairs = (uint32_t)(total < 0xFFFFFFFFull ? total : 0xFFFFFFFFull);
+            if (total > limit) ctr->storage_overflow = 1;
+        }
     }
 }
 
@@ -1839,6 +1857,60 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
         if (y0 + 4u * k < fp.height) shade_pixel<kLinear>(meshes, J, unit, x, y0 + 4u * k, ids[k]);
 }
 
+// DrawTextureKernel / DrawTextureGlobalKernel (CCR/CUDARenderer.cu:1565-1617):
+// one texture over the whole frame, one pixel per thread (vis none, depth 1).
+__global__ __launch_bounds__(256) void k_draw_texture(uint32_t *__restrict__ vis, float *__restrict__ depth,
+                                                      uint32_t *__restrict__ rgba, uint32_t width, uint32_t height,
+                                                      const uint32_t *__restrict__ texels, uint32_t tw, uint32_t th,
+                                                      int mode, int stretch, int sample_state)
+{
+    const uint32_t x = blockIdx.x * 64u + (threadIdx.x & 63u);
+    const uint32_t y = blockIdx.y * 4u + (threadIdx.x >> 6);
+    if (x >= width || y >= height) return;
+    const size_t pid = (size_t)y * width + x;
+    stg(vis + pid, kVisNone);
+    stg(depth + pid, 1.0f);
+    uint32_t out;
+    if (mode == 0) {
+        // normalized coordinates, point filter, wrap addressing
+        const float u = (float)x / (float)width, v = (float)y / (float)height;
+        int ix = (int)floorf(u * (float)tw), iy = (int)floorf(v * (float)th);
+        ix %= (int)tw;
+        iy %= (int)th;
+        ix += ix < 0 ? (int)tw : 0;
+        iy += iy < 0 ? (int)th : 0;
+        out = ldg(texels + (size_t)iy * tw + (uint32_t)ix);
+    } else {
+        float u = (float)x, v = (float)y;
+        if (stretch) {
+            u /= (float)width;
+            v /= (float)height;
+        } else {
+            u /= (float)tw;
+            v /= (float)th;
+        }
+        DevTexture t;
+        t.texels = texels;
+        t.w = tw;
+        t.h = th;
+        const Taps tp = texel_taps(t, u, v);
+        float r, g, b;
+        if (sample_state == 0) {
+            const uint32_t c = ldg(texels + tp.c);
+            r = tp.ok ? byte_unit(c & 0xFFu) : 1.f;
+            g = tp.ok ? byte_unit((c >> 8) & 0xFFu) : 0.f;
+            b = tp.ok ? byte_unit((c >> 16) & 0xFFu) : 1.f;
+        } else {
+            uint32_t n[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) n[i] = ldg(texels + tp.n[i]);
+            decode_linear(n, ldg(texels + tp.c), tp.ok, r, g, b);
+        }
+        out = pack_rgba(r, g, b);
+    }
+    stg(rgba + pid, out);
+}
+
 // Copies the counters of every job that is not shaded to its renderer's
 // pinned host block (one workgroup per job), replacing a device-to-host copy
 // per job; the shading kernel publishes the others.
@@ -1941,6 +2013,15 @@ hipError_t launch_shade(const Launch &L, hipStream_t s)
         hipLaunchKernelGGL(k_shade<false>, dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs, L.table[ST_SHADE]);
     if (L.shade_modes & 2u)
         hipLaunchKernelGGL(k_shade<true>, dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs, L.table[ST_SHADE]);
+    return hipGetLastError();
+}
+
+hipError_t launch_draw_texture(uint32_t *vis, float *depth, uint32_t *rgba, uint32_t width, uint32_t height,
+                               const uint32_t *texels, uint32_t tw, uint32_t th, int mode, bool stretch,
+                               int sample_state, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_draw_texture, dim3(ceil_div(width, 64), ceil_div(height, 4)), dim3(256), 0, s, vis, depth,
+                       rgba, width, height, texels, tw, th, mode, stretch ? 1 : 0, sample_state);
     return hipGetLastError();
 }
 

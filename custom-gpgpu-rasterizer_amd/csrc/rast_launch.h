@@ -45,6 +45,11 @@ hipError_t launch_fallback(const Launch &L, hipStream_t s);
 hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s);
 hipError_t launch_shade(const Launch &L, hipStream_t s);
 hipError_t launch_publish(const Launch &L, hipStream_t s);
+// The DrawTexture / DrawTextureGlobal test hooks: a frame holding one texture
+// (vis none, depth 1); mode RAST_DRAW_TEXTURE / RAST_DRAW_TEXTURE_GLOBAL.
+hipError_t launch_draw_texture(uint32_t *vis, float *depth, uint32_t *rgba, uint32_t width, uint32_t height,
+                               const uint32_t *texels, uint32_t tw, uint32_t th, int mode, bool stretch,
+                               int sample_state, hipStream_t s);
 
 // Block -> job lookup of a shared stage grid.  The 16-bit table entry is
 // read with a vector load (there is no 16-bit scalar load); it is the same

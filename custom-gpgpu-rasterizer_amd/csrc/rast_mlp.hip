@@ -34,6 +34,17 @@ constexpr int F = kNumFeatures;
 
 __host__ __device__ inline size_t num_params(int H) { return (size_t)H * F + H + (size_t)H * H + H + H + 1; }
 
+// A tile's single-pass queue capacity from a predicted length c (>= 0, or
+// NaN): c + slack, at most `limit` (the frame's triangle count: a tile's
+// queue never holds a triangle twice).  Formed in 64 bits, so neither the
+// float conversion nor the slack addition can wrap.
+__device__ __forceinline__ uint32_t clamp_capacity(float c, uint32_t slack, uint32_t limit)
+{
+    const uint64_t base = c > 0.f ? (c < (float)limit ? (uint64_t)c : (uint64_t)limit) : 0ull;
+    const uint64_t v = base + slack;
+    return v < limit ? (uint32_t)v : limit;
+}
+
 // ---------------------------------------------------------------- features
 
 // Per-tile features (inputs of the capacity model), shared by inference and
@@ -97,7 +108,8 @@ __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t
 // block (rows padded against bank conflicts); the first layer's activations
 // pass through LDS as the second layer's A operand; the output layer sums
 // over units within each 16-lane group.  Otherwise the previous frame's
-// count: ceil(prev * margin) + slack.
+// count: ceil(prev * margin) + slack.  Either capacity is clamped to the
+// frame's triangle count.
 constexpr int kCapTiles = kCapacityTilesPerBlock;
 constexpr int kCapWaves = kCapTiles / 16;  // waves per block
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -125,8 +137,7 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
     if (!J.learned) {
         const uint32_t t = local * kCapTiles + threadIdx.x;
         if (threadIdx.x < (uint32_t)kCapTiles && t < n) {
-            const float c = ceilf((float)prev[t] * margin);
-            cap[t] = (c < 1.0e8f ? (uint32_t)c : 100000000u) + slack;
+            cap[t] = clamp_capacity(ceilf((float)prev[t] * margin), slack, fp.num_tris);
         }
         return;
     }
@@ -211,10 +222,7 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
     if (r16 < 4) {
         const uint32_t t = t0 + 4u * g + r16;
         const float out = b3 + (r16 == 0 ? o[0] : r16 == 1 ? o[1] : r16 == 2 ? o[2] : o[3]);
-        if (t < n) {
-            const float c = ceilf(expm1f(out < 20.f ? out : 20.f) * margin);
-            cap[t] = (c > 0.f ? (uint32_t)c : 0u) + slack;
-        }
+        if (t < n) cap[t] = clamp_capacity(ceilf(expm1f(out < 20.f ? out : 20.f) * margin), slack, fp.num_tris);
     }
 }
 
@@ -265,10 +273,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(uint32_t n, const float *__re
     for (int f = 0; f < F; ++f) xv[f] = x[(size_t)t * F + f];
     const float o = mlp_forward_lane<H>(w, xv, h1);
     if (pred) pred[t] = o;
-    if (cap) {
-        const float c = ceilf(expm1f(o < 20.f ? o : 20.f) * margin);
-        cap[t] = (c > 0.f ? (uint32_t)c : 0u) + slack;
-    }
+    if (cap) cap[t] = clamp_capacity(ceilf(expm1f(o < 20.f ? o : 20.f) * margin), slack, 0xFFFFFFFFu);
 }
 
 // ----------------------------------------------------------------- training

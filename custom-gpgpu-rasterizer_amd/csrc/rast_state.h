@@ -127,6 +127,7 @@ struct rast_renderer {
     // timing
     hipEvent_t ev[12] = {};
     bool timing = false;
+    bool timer_started = false;  // rast_timer_start recorded its event
     int ev_used[12] = {};
     // last frame
     rast_camera last_cam{};

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "librast.so")
 RAST_OK = 0
 VIS_NONE = 0xFFFFFFFF
 BIN_EXACT, BIN_LEARNED, BIN_PREVFRAME = 0, 1, 2
+DRAW_TEXTURE, DRAW_TEXTURE_GLOBAL = 0, 1
 FLAG_TIMING, FLAG_NO_SHADE, FLAG_GRAPH = 1, 2, 4
 DEBUG_VERTEX_POS, DEBUG_VERTEX_ATTR, DEBUG_TRI_RECORDS, DEBUG_TRI_ALIVE, DEBUG_TILE_COUNTS, DEBUG_BIG_LIST, DEBUG_TILE_CAPS = range(7)
 
@@ -35,6 +36,7 @@ EXPORTS = [
     "rast_renderer_set_binmodel", "rast_binmodel_record", "rast_binmodel_clear_data", "rast_binmodel_train",
     "rast_binmodel_get_params", "rast_binmodel_set_params", "rast_binmodel_num_params", "rast_binmodel_forward",
     "rast_binmodel_loss_grad", "rast_render_batch", "rast_renderer_create_view", "rast_texture_create_pitched",
+    "rast_timer_start", "rast_timer_stop", "rast_draw_texture",
 ]
 
 
@@ -126,6 +128,9 @@ def lib():
         L.rast_binmodel_forward.argtypes = [vp, vp, C.c_uint32, vp]
         L.rast_binmodel_loss_grad.argtypes = [vp, vp, vp, C.c_uint32, C.c_float, vp, vp]
         L.rast_render_batch.argtypes = [C.POINTER(Job), C.c_uint32]
+        L.rast_timer_start.argtypes = [vp]
+        L.rast_timer_stop.argtypes = [vp, C.POINTER(C.c_float)]
+        L.rast_draw_texture.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -232,6 +237,23 @@ class Renderer:
 
     def sync(self) -> None:
         check(lib().rast_sync(self.h), "rast_sync")
+
+    def start_timer(self) -> None:
+        """StartTimer (CCR/CUDARenderer.cu:398-402): a HIP event on the renderer's stream."""
+        check(lib().rast_timer_start(self.h), "rast_timer_start")
+
+    def stop_timer(self) -> float:
+        """StopTimer: milliseconds of device time since start_timer."""
+        ms = C.c_float(0)
+        check(lib().rast_timer_stop(self.h, C.byref(ms)), "rast_timer_stop")
+        return ms.value
+
+    def draw_texture(self, texels: np.ndarray, mode: int = DRAW_TEXTURE_GLOBAL, stretch: bool = True,
+                     sample_state: int = S.SAMPLE_POINT) -> None:
+        """DrawTexture / DrawTextureGlobal (CCR/CUDARenderer.cu:1969-2012): the
+        frame becomes the texture; read it with readback()."""
+        check(lib().rast_draw_texture(self.h, self.texture(texels), mode, int(stretch), sample_state),
+              "rast_draw_texture")
 
     def readback(self, vis=True, depth=True, rgba=True):
         out = []

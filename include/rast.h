@@ -155,8 +155,12 @@ int rast_renderer_create(int device, uint32_t width, uint32_t height, rast_rende
  * model and captured graphs; meshes and textures stay with the owner, which
  * cannot be destroyed while views exist.  Creating meshes on a view fails. */
 int rast_renderer_create_view(rast_renderer *owner, uint32_t width, uint32_t height, rast_renderer **out);
-/* ~CUDARenderer (CCR/CUDARenderer.cu:47-52): waits for the stream, frees all
- * meshes, textures and buffers of the renderer. */
+/* ~CUDARenderer (CCR/CUDARenderer.cu:47-52): waits for the stream, frees the
+ * renderer's buffers and the device memory of all its meshes and textures.
+ * Their handles stay allocated, empty: rast_mesh_destroy / rast_texture_destroy
+ * on them only frees the handle (so the objects holding them may be destroyed
+ * after the renderer, the order of CCR/main.cpp:418,529), and the other calls
+ * taking them return RAST_E_STATE or RAST_E_INVALID_ARG. */
 int rast_renderer_destroy(rast_renderer *r);
 /* SetupBins(numBins, binDim, binQueueMaxSize) (CCR/CUDARenderer.cu:199-204):
  * square tiles of tile_size pixels (8, 16 or 32); triangles whose tile
@@ -225,8 +229,11 @@ int rast_sync(rast_renderer *r);
 int rast_readback(rast_renderer *r, uint32_t *vis, float *depth, uint32_t *rgba);
 /* GetTotalNumVisibleTriangles / GetTotalNumTriangles + BENCHMARK timings. */
 int rast_get_stats(rast_renderer *r, rast_stats *out);
-/* Device pointers of the current frame's vis / depth / rgba buffers (valid
- * until the next rast_render / rast_resize), for zero-copy consumers. */
+/* Device pointers of the current frame's vis / depth / rgba buffers, for
+ * zero-copy consumers.  Waits for the frame and checks it (a frame whose
+ * queue storage overflowed is rendered again first), as rast_readback does;
+ * RAST_E_STATE if no frame was rendered.  The pointers stay valid until the
+ * next rast_render / rast_render_batch / rast_resize of the renderer. */
 int rast_frame_buffers(rast_renderer *r, void **vis, void **depth, void **rgba);
 /* Test-only inspection of intermediate buffers (the reference's test kernels
  * DrawTexture/DrawTextureGlobal play this role, CCR/CUDARenderer.h:42-45):
@@ -243,6 +250,29 @@ enum {
 };
 int rast_debug_read(rast_renderer *r, int which, void *dst, size_t bytes, size_t *size_out);
 
+/* CUDARenderer::StartTimer / StopTimer and GetBenchMarker's CUDABenchMarker
+ * (CCR/CUDARenderer.cu:398-412, CCR/CUDABenchMarker.cpp:24-35): HIP events on
+ * the renderer's stream.  rast_timer_stop records the stop event (after the
+ * renderer's last batch, if its frame came from one), waits for it and stores
+ * the milliseconds since rast_timer_start's event. */
+int rast_timer_start(rast_renderer *r);
+int rast_timer_stop(rast_renderer *r, float *ms);
+
+/* Test hooks DrawTexture / DrawTextureGlobal (CCR/CUDARenderer.cu:1969-2012,
+ * kernels :1565-1617): the renderer's frame becomes one texture (vis
+ * RAST_VIS_NONE, depth 1.0 everywhere), readable with rast_readback.
+ *   RAST_DRAW_TEXTURE: pixel (x, y) takes the texel nearest to the normalized
+ *     coordinates (x / width, y / height) under wrap addressing, texel
+ *     (floor(u * w) mod w, floor(v * h) mod h), copied unchanged (the
+ *     reference's tex2D point fetch; its R/B swap converts the texture's byte
+ *     order to its window surface's, and both are R,G,B,A here);
+ *   RAST_DRAW_TEXTURE_GLOBAL: the colour CUDATextureSampler::Sample returns
+ *     for uv = (x, y) / (width, height) (stretch != 0) or (x, y) / (w, h),
+ *     in sample_state's mode (magenta outside [0, 1]), packed as shading packs
+ *     colours.  sample_state is ignored by RAST_DRAW_TEXTURE. */
+enum { RAST_DRAW_TEXTURE = 0, RAST_DRAW_TEXTURE_GLOBAL = 1 };
+int rast_draw_texture(rast_renderer *r, const rast_texture *t, int mode, int stretch, int sample_state);
+
 /* --------------------------------------------------------- learned binner */
 /* No reference counterpart (the reference sizes queues statically,
  * CCR/DEFINES.h:21-23, CCR/main.cpp:372-389): a per-tile MLP in -> H -> H -> 1
@@ -252,7 +282,12 @@ int rast_debug_read(rast_renderer *r, int which, void *dst, size_t bytes, size_t
 int rast_binmodel_create(rast_renderer *r, uint32_t hidden, uint32_t seed, rast_binmodel **out);
 int rast_binmodel_destroy(rast_binmodel *m);
 int rast_binmodel_num_features(void);
-/* Attach (or detach with NULL) the model used by RAST_BIN_LEARNED. */
+/* Attach (or detach with NULL) the model used by RAST_BIN_LEARNED.  A tile's
+ * queue capacity is ceil(expm1(prediction) * margin) + slack (previous-frame
+ * policy: ceil(count * margin) + slack), at most the frame's triangle count;
+ * margin in [1, 64], slack at most 2^20.  Capacities are hints: the queue
+ * storage bounds their sum, and entries beyond a tile's capacity are placed
+ * by the exact fallback. */
 int rast_renderer_set_binmodel(rast_renderer *r, rast_binmodel *m, float margin, uint32_t slack);
 /* Per-tile features and exact log-count targets of the last rendered frame,
  * appended to the model's device-side training set. */
@@ -273,11 +308,13 @@ int rast_binmodel_loss_grad(rast_binmodel *m, const float *x, const float *targe
 /* One render job of a batch: no reference counterpart; it realises the
  * "persistent kernel approach" TODO of CCR/CUDARenderer.cu:284-286 and the
  * per-mesh host loop of CCR/CUDARenderer.cu:237-330.  rast_render_batch
- * enqueues all jobs as one captured HIP graph (one independent branch per
- * job, so small frames overlap on the device); the graph is cached per job
- * list and options.  Every job needs its own renderer (views of one scene
- * are fine); timing flags are ignored.  Results are read per renderer with
- * rast_readback / rast_get_stats as after rast_render. */
+ * enqueues all jobs as one captured HIP graph in which every pipeline stage
+ * is one kernel over the blocks of all jobs (stage-fused, so small frames
+ * share the chip); the graph is cached per batch structure (sizes, tile
+ * sizes, binning modes) and replayed with the new cameras and options.
+ * Every job needs its own renderer (views of one scene are fine); timing
+ * flags are ignored.  Results are read per renderer with rast_readback /
+ * rast_get_stats as after rast_render. */
 typedef struct {
     rast_renderer *target;   /* renderer providing the framebuffer and meshes */
     rast_camera camera;

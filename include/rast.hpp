@@ -22,7 +22,6 @@
 // under the same names, as the reference declares them.
 #pragma once
 
-#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -64,6 +63,25 @@ inline int enum_value(E e)
 }
 
 class CUDARenderer;
+
+// CUDABenchMarker (CCR/CUDABenchMarker.h:4-18): StartTimer / StopTimer over
+// HIP events on the renderer's stream; StopTimer waits for the stop event and
+// returns the milliseconds in between.
+class CUDABenchMarker {
+public:
+    explicit CUDABenchMarker(rast_renderer *r = nullptr) : r_(r) {}
+    virtual ~CUDABenchMarker() = default;
+    virtual void StartTimer() { check(rast_timer_start(r_), "rast_timer_start"); }
+    virtual float StopTimer()
+    {
+        float ms = 0.f;
+        check(rast_timer_stop(r_, &ms), "rast_timer_stop");
+        return ms;
+    }
+
+private:
+    rast_renderer *r_;
+};
 
 // CUDATexture (CCR/CUDATexture.h:4-28): one RGBA8 texture in HBM.
 class CUDATexture {
@@ -211,6 +229,7 @@ public:
         check(rast_renderer_create(device, width, height, &r_), "rast_renderer_create");
         width_ = width;
         height_ = height;
+        bench_ = CUDABenchMarker(r_);
     }
     // CUDARenderer(const WindowHelper&): the resolution, and the back buffer
     // that Present() fills (RGBA8, width x height).
@@ -338,13 +357,28 @@ public:
     // KernelWarmUp (CCR/CUDARenderer.cu:2015-2025): kernels are loaded with
     // the library; nothing to warm.
     void KernelWarmUp() {}
-    // StartTimer / StopTimer (CCR/CUDARenderer.cu:398-412): host wall time in
-    // milliseconds, waiting for the frames in between.
-    void StartTimer() { t0_ = std::chrono::steady_clock::now(); }
-    float StopTimer()
+    // StartTimer / StopTimer / GetBenchMarker (CCR/CUDARenderer.cu:398-412,
+    // CCR/CUDARenderer.h:71-73): HIP events on the renderer's stream.
+    void StartTimer() { bench_.StartTimer(); }
+    float StopTimer() { return bench_.StopTimer(); }
+    CUDABenchMarker &GetBenchMarker() { return bench_; }
+
+    // Test hooks DrawTexture / DrawTextureGlobal (CCR/CUDARenderer.cu:1969-2012):
+    // the frame becomes the texture, then Present().  The reference loads the
+    // texture from an image path; here it is an existing CUDATexture (the
+    // path overloads need an image decoder, see rast_io.h).
+    void DrawTexture(const CUDATexture &tex)
     {
-        check(rast_sync(r_), "rast_sync");
-        return std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0_).count();
+        check(rast_draw_texture(r_, tex.GetHandle(), RAST_DRAW_TEXTURE, 1, RAST_SAMPLE_POINT), "rast_draw_texture");
+        Present();
+    }
+    template <class SampleStateT = int>
+    void DrawTextureGlobal(const CUDATexture &tex, bool isStretchedToWindow = true, SampleStateT sampleState = SampleStateT(0))
+    {
+        check(rast_draw_texture(r_, tex.GetHandle(), RAST_DRAW_TEXTURE_GLOBAL, isStretchedToWindow ? 1 : 0,
+                                enum_value(sampleState)),
+              "rast_draw_texture");
+        Present();
     }
 
     rast_renderer *GetHandle() const { return r_; }
@@ -392,12 +426,13 @@ private:
     uint32_t width_ = 0, height_ = 0;
     uint32_t *back_buffer_ = nullptr;
     std::vector<std::unique_ptr<CUDAMesh>> meshes_;
-    std::chrono::steady_clock::time_point t0_{};
+    CUDABenchMarker bench_;
 };
 
 }  // namespace rast
 
 #ifdef RAST_REFERENCE_NAMES
+using rast::CUDABenchMarker;
 using rast::CUDAMesh;
 using rast::CUDARenderer;
 using rast::CUDATexture;

@@ -109,6 +109,10 @@ def lib():
         L.orc_render_bruteforce.argtypes = [C.POINTER(OrcMesh), C.c_int, C.POINTER(OrcCamera), C.c_int,
                                             C.c_uint32, C.c_uint32, vp, vp]
         L.orc_sample.argtypes = [C.POINTER(OrcTexture), vp, C.c_int, vp]
+        L.orc_shade_pbr.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_shade_inputs.argtypes = [C.POINTER(OrcMesh), C.c_int, C.POINTER(OrcCamera), C.c_int, C.c_uint32,
+                                       C.c_uint32, vp, vp]
+        L.orc_shade_inputs.restype = C.c_int
         L.orc_pack_rgba.argtypes = [vp]
         L.orc_pack_rgba.restype = C.c_uint32
         L.orc_mat4_mul.argtypes = [vp, vp, vp]
@@ -259,6 +263,27 @@ def sample(texels: np.ndarray, uv, mode: int):
     out = np.zeros(3, np.float32)
     lib().orc_sample(C.byref(t), _ptr(uvb), mode, _ptr(out))
     return out
+
+
+def shade_pbr(diff, nrm, spc, gls, n, tan, vd) -> np.ndarray:
+    """The oracle's ShadePixelUnsafe on given map samples and attributes."""
+    args = [np.ascontiguousarray(a, np.float32) for a in (diff, nrm, spc, gls, n, tan, vd)]
+    out = np.zeros(3, np.float32)
+    lib().orc_shade_pbr(*[_ptr(a) for a in args], _ptr(out))
+    return out
+
+
+def shade_inputs(scene, cam, cull_mode: int, width: int, height: int):
+    """Per-pixel shading inputs of the oracle's frame: (vis u32[H,W],
+    inputs f32[H,W,11] = u v n.xyz tan.xyz vd.xyz, NaN where nothing is visible)."""
+    pack = _MeshPack(scene)
+    vis = np.empty((height, width), np.uint32)
+    out = np.empty((height, width, 11), np.float32)
+    rc = lib().orc_shade_inputs(pack.arr, len(scene.meshes), C.byref(camera_struct(cam)), cull_mode, width, height,
+                                _ptr(vis), _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_shade_inputs failed: {rc}")
+    return vis, out
 
 
 def pack_rgba(rgb) -> int:

@@ -206,8 +206,15 @@ static uint32_t mesh_num_triangles(const orc_mesh *m)
     return m->num_indices >= 3 ? m->num_indices - 2 : 0;
 }
 
-/* Triangle k of a mesh: list (CCR/ERenderer.cpp:75-79) or strip with the
- * odd-triangle winding swap keyed on triangle parity (CCR/ERenderer.cpp:119-123). */
+/* Triangle k of a mesh: list (CCR/ERenderer.cpp:75-79) or strip.  Strips
+ * follow the GPU path being replaced, TA_Kernel (CCR/CUDARenderer.cu:733-746):
+ * triangle k takes indices k, k+1, k+2 and odd k swap the last two.
+ * Deviation D10: ERenderer's CPU loop (CCR/ERenderer.cpp:115-157) starts
+ * with the swap (its first triangle uses k, k+2, k+1) and toggles it only
+ * after a triangle passed culling, so a triangle's winding there depends on
+ * how many earlier triangles were culled; the index-parity rule makes every
+ * triangle independent (and is what the CUDA path intends: its strip branch
+ * declares a second `triangle` that shadows the one it culls and stores). */
 static void triangle_indices(const orc_mesh *m, uint32_t k, uint32_t idx[3])
 {
     if (m->topology == ORC_TOPO_LIST) {
@@ -620,9 +627,48 @@ typedef struct {
     const orc_texture *defaults;
 } shade_job;
 
-/* Interpolation of RasterizePixelAtomic (CCR/CUDAROPs.cu:329-380) for one
- * pixel of the winning triangle; the weights use the GPU edge form
- * (CCR/CUDAROPs.cu:120-145), which is bitwise the CPU form. */
+/* Interpolation of RasterizePixelAtomic (CCR/CUDAROPs.cu:329-380) at pixel
+ * (x, y) of triangle g, whose vertices have the VS outputs of s; the weights
+ * use the GPU edge form (CCR/CUDAROPs.cu:120-145), which is bitwise the CPU
+ * form.  uv and n are perspective-correct (x wInterpolated), tan is not, vd
+ * is normalized, the colour is weighted without 1/w (deviation D8). */
+typedef struct {
+    float uv[2], n[3], tan[3], vd[3], col[3];
+} pixel_inputs;
+
+static void interpolate_pixel(const scene_state *s, uint64_t g, uint32_t x, uint32_t y, pixel_inputs *pi,
+                              int *mesh_out, uint32_t idx_out[3])
+{
+    uint32_t k;
+    int m = locate_mesh(s, g, &k);
+    const orc_mesh *mesh = &s->meshes[m];
+    uint32_t idx[3];
+    triangle_indices(mesh, k, idx);
+    const tri_setup *t = &s->ts[g];
+    float px = (float)x, py = (float)y;
+    float w[3];
+    w[2] = (px - t->sx[0]) * (t->sy[1] - t->sy[0]) - (py - t->sy[0]) * (t->sx[1] - t->sx[0]);
+    w[0] = (px - t->sx[1]) * (t->sy[2] - t->sy[1]) - (py - t->sy[1]) * (t->sx[2] - t->sx[1]);
+    w[1] = (px - t->sx[2]) * (t->sy[0] - t->sy[2]) - (py - t->sy[2]) * (t->sx[0] - t->sx[2]);
+    w[0] /= t->area; w[1] /= t->area; w[2] /= t->area;
+    const float *a0 = s->attr[m] + 16 * (size_t)idx[0];
+    const float *a1 = s->attr[m] + 16 * (size_t)idx[1];
+    const float *a2 = s->attr[m] + 16 * (size_t)idx[2];
+    const float i0 = a0[14], i1 = a1[14], i2 = a2[14];
+    const float wI = 1.f / (i0 * w[0] + i1 * w[1] + i2 * w[2]);
+    float v[14];
+    for (int c = 0; c < 14; ++c)
+        v[c] = w[0] * (a0[c] * i0) + w[1] * (a1[c] * i1) + w[2] * (a2[c] * i2);
+    pi->uv[0] = v[0] * wI; pi->uv[1] = v[1] * wI;
+    pi->n[0] = v[2] * wI; pi->n[1] = v[3] * wI; pi->n[2] = v[4] * wI;
+    pi->tan[0] = v[5]; pi->tan[1] = v[6]; pi->tan[2] = v[7];
+    pi->vd[0] = v[8]; pi->vd[1] = v[9]; pi->vd[2] = v[10];
+    normalize3(pi->vd);
+    pi->col[0] = v[11]; pi->col[1] = v[12]; pi->col[2] = v[13];
+    if (mesh_out) *mesh_out = m;
+    if (idx_out) { idx_out[0] = idx[0]; idx_out[1] = idx[1]; idx_out[2] = idx[2]; }
+}
+
 static void *shade_worker(void *arg)
 {
     shade_job *j = (shade_job *)arg;
@@ -638,35 +684,15 @@ static void *shade_worker(void *arg)
                 else j->rgba[pid] = 0xFF000000u;
                 continue;
             }
-            uint32_t k;
-            int m = locate_mesh(s, g, &k);
-            const orc_mesh *mesh = &s->meshes[m];
+            pixel_inputs pi;
+            int m;
             uint32_t idx[3];
-            triangle_indices(mesh, k, idx);
+            interpolate_pixel(s, g, x, y, &pi, &m, idx);
+            const orc_mesh *mesh = &s->meshes[m];
             if (j->vert_bitmap) {
                 j->vert_bitmap[m][idx[0]] = 1; j->vert_bitmap[m][idx[1]] = 1; j->vert_bitmap[m][idx[2]] = 1;
             }
-            const tri_setup *t = &s->ts[g];
-            float px = (float)x, py = (float)y;
-            float w[3];
-            w[2] = (px - t->sx[0]) * (t->sy[1] - t->sy[0]) - (py - t->sy[0]) * (t->sx[1] - t->sx[0]);
-            w[0] = (px - t->sx[1]) * (t->sy[2] - t->sy[1]) - (py - t->sy[1]) * (t->sx[2] - t->sx[1]);
-            w[1] = (px - t->sx[2]) * (t->sy[0] - t->sy[2]) - (py - t->sy[2]) * (t->sx[0] - t->sx[2]);
-            w[0] /= t->area; w[1] /= t->area; w[2] /= t->area;
-            const float *a0 = s->attr[m] + 16 * (size_t)idx[0];
-            const float *a1 = s->attr[m] + 16 * (size_t)idx[1];
-            const float *a2 = s->attr[m] + 16 * (size_t)idx[2];
-            const float i0 = a0[14], i1 = a1[14], i2 = a2[14];
-            const float wI = 1.f / (i0 * w[0] + i1 * w[1] + i2 * w[2]);
-            float v[14];
-            for (int c = 0; c < 14; ++c)
-                v[c] = w[0] * (a0[c] * i0) + w[1] * (a1[c] * i1) + w[2] * (a2[c] * i2);
-            float uv[2] = { v[0] * wI, v[1] * wI };
-            float n[3] = { v[2] * wI, v[3] * wI, v[4] * wI };
-            float tan[3] = { v[5], v[6], v[7] };
-            float vd[3] = { v[8], v[9], v[10] };
-            normalize3(vd);
-            float col[3] = { v[11], v[12], v[13] };
+            float *uv = pi.uv, *n = pi.n, *tan = pi.tan, *vd = pi.vd, *col = pi.col;
             if (o->visualisation == ORC_VIS_DEPTH) { /* CCR/CUDARenderer.cu:862-879, Remap */
                 float rgb[3] = { (j->depth[pid] - 0.985f) / (1.f - 0.985f), 0.f, 0.f };
                 j->rgba[pid] = orc_pack_rgba(rgb);
@@ -776,6 +802,42 @@ int orc_render(const orc_mesh *meshes, int num_meshes, const orc_camera *cam, co
         for (int m = 0; m < num_meshes; ++m) free(vert_bm[m]);
         free(tex_bm); free(vert_bm);
     }
+    scene_free(&s);
+    return 0;
+}
+
+void orc_shade_pbr(const float diff[3], const float nrm[3], const float spc[3], const float gls[3],
+                   const float n[3], const float tan[3], const float vd[3], float out[3])
+{
+    shade_pbr(diff, nrm, spc, gls, n, tan, vd, out);
+}
+
+int orc_shade_inputs(const orc_mesh *meshes, int num_meshes, const orc_camera *cam, int cull_mode,
+                     uint32_t W, uint32_t H, uint32_t *vis, float *out)
+{
+    if (!meshes || num_meshes <= 0 || !cam || W == 0 || H == 0 || !vis || !out) return -1;
+    scene_state s;
+    if (scene_build(&s, meshes, num_meshes, cam, cull_mode, W, H, 1) != 0) return -2;
+    const size_t P = (size_t)W * H;
+    float *depth = (float *)malloc(P * sizeof(float));
+    if (!depth) { scene_free(&s); return -2; }
+    for (size_t i = 0; i < P; ++i) { vis[i] = ORC_VIS_NONE; depth[i] = 1.0f; }
+    raster_job rj = { &s, 0, H, vis, depth };
+    raster_worker(&rj);
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            const size_t pid = (size_t)x + (size_t)y * W;
+            float *o = out + 11 * pid;
+            if (vis[pid] == ORC_VIS_NONE) {
+                for (int c = 0; c < 11; ++c) o[c] = NAN;
+                continue;
+            }
+            pixel_inputs pi;
+            interpolate_pixel(&s, vis[pid], x, y, &pi, NULL, NULL);
+            o[0] = pi.uv[0]; o[1] = pi.uv[1];
+            for (int c = 0; c < 3; ++c) { o[2 + c] = pi.n[c]; o[5 + c] = pi.tan[c]; o[8 + c] = pi.vd[c]; }
+        }
+    free(depth);
     scene_free(&s);
     return 0;
 }

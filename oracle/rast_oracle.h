@@ -128,6 +128,16 @@ int orc_render_bruteforce(const orc_mesh *meshes, int num_meshes, const orc_came
                           int cull_mode, uint32_t width, uint32_t height,
                           uint32_t *vis, float *depth);
 
+/* ShadePixelUnsafe (CCR/CUDAROPs.cu:472-514) on given samples (rgb in [0,1])
+ * and interpolated n, tan, vd: the shading step of orc_render. */
+void orc_shade_pbr(const float diff[3], const float nrm[3], const float spc[3], const float gls[3],
+                   const float n[3], const float tan[3], const float vd[3], float out[3]);
+
+/* The shading inputs orc_render interpolates at every covered pixel: 11
+ * floats per pixel (u, v, n.xyz, tan.xyz, vd.xyz), NaN where vis is none. */
+int orc_shade_inputs(const orc_mesh *meshes, int num_meshes, const orc_camera *cam, int cull_mode,
+                     uint32_t width, uint32_t height, uint32_t *vis, float *out);
+
 /* Texture sampling (CCR/CUDATextureSampler.cuh:86-190): rgb in [0,1]. */
 void orc_sample(const orc_texture *t, const float uv[2], int sample_state, float rgb[3]);
 

@@ -116,10 +116,15 @@ int main()
 
     std::vector<uint32_t> vis(W * H);
     std::vector<float> depth(W * H);
+    std::vector<uint32_t> drawn(W * H);
+    // the reference app's order (CCR/main.cpp:418,440,529): the texture
+    // manager lives on the stack and outlives the renderer it took its
+    // textures from, so its textures are destroyed after the renderer
+    CUDATextureManager tm;
     {
-        CUDARenderer renderer(wh);
+        CUDARenderer *pRenderer = new CUDARenderer(wh);
+        CUDARenderer &renderer = *pRenderer;
         renderer.DisplayGPUSpecs(0);
-        CUDATextureManager tm;  // destroyed before the renderer that owns its device memory
         EXPECT(tm.AddCUDATexture(renderer.CreateTexture(checker.data(), 256, 256)) == 0, "texture id");
         renderer.LoadScene(&sg, tm);
         renderer.SetupBins(app::IPoint2{ 40, 30 }, app::IPoint2{ 16, 16 }, 256);
@@ -139,7 +144,27 @@ int main()
             threw = e.code() == RAST_E_INVALID_ARG;
         }
         EXPECT(threw, "non-square bins must be rejected");
+        // the benchmark marker times on the renderer's stream
+        renderer.GetBenchMarker().StartTimer();
+        renderer.Render(sm, tm, &cam);
+        EXPECT(renderer.GetBenchMarker().StopTimer() >= 0.f, "bench marker");
+        // DrawTextureGlobal: the checker stretched over the window (point sampling)
+        renderer.DrawTextureGlobal(*tm.GetCUDATexture(0), true, app::SampleState::Point);
+        drawn = back;
+        delete pRenderer;
     }
+    // the texture handle outlived its renderer: destroying it now is safe
+    tm.RemoveCUDATexture(0);
+    EXPECT(tm.GetCUDATexture(0) == nullptr, "texture removed");
+    for (uint32_t y = 0; y < H; y += 7)
+        for (uint32_t x = 0; x < W; x += 5) {
+            const orc_texture ot{ checker.data(), 256, 256 };
+            const float uv[2] = { (float)x / (float)W, (float)y / (float)H };
+            float c[3];
+            orc_sample(&ot, uv, ORC_SAMPLE_POINT, c);
+            const uint32_t want = orc_pack_rgba(c), got = drawn[(size_t)y * W + x];
+            EXPECT(got == want, "DrawTextureGlobal at (%u, %u): %08x, want %08x", x, y, got, want);
+        }
 
     // the oracle on the same inputs
     orc_mesh om{};

@@ -123,7 +123,8 @@ def test_frame_capacities_match_the_oracle_forward(hidden):
     against the oracle's forward pass.  The first layer's weights are zero
     except on the two tile-position features, so each tile's prediction is a
     function of its position alone; capacity = ceil(expm1(out) * margin) +
-    slack, exact except where the value sits within 1e-4 of an integer."""
+    slack (at most the triangle count), exact except where the value sits
+    within 1e-4 of an integer."""
     w = small_terrain()
     tile, margin, slack = 16, 1.25, 4
     rng = np.random.default_rng(100 + hidden)
@@ -153,9 +154,38 @@ def test_frame_capacities_match_the_oracle_forward(hidden):
     x[:, 7] = ((t % tx_n).astype(np.float32) + np.float32(0.5)) / np.float32(tx_n)
     out = O.mlp_forward(p, F, hidden, x).astype(np.float64)
     v = np.expm1(np.minimum(out, 20.0)) * margin
-    want = np.maximum(np.ceil(v), 0) + slack
+    want = np.minimum(np.maximum(np.ceil(v), 0) + slack, w.scene.num_triangles)
     near = np.abs(v - np.round(v)) < 1e-4 * np.maximum(1.0, np.abs(v))
     diff = caps.astype(np.int64) - want.astype(np.int64)
     assert np.all(diff[~near] == 0), f"{np.count_nonzero(diff[~near])} tiles differ"
     assert np.all(np.abs(diff[near]) <= 1)
     assert len(np.unique(caps)) > 20  # the predictions vary over the screen
+
+
+@pytest.mark.parametrize("policy", [R.BIN_LEARNED, R.BIN_PREVFRAME])
+def test_saturated_capacities_still_give_the_exact_frames(policy):
+    """A model whose output saturates (expm1(20) x margin 64 entries per tile,
+    far more than the triangle count, summed over the tiles far past 2^32)
+    and the largest slack: every capacity is clamped to the triangle count,
+    their sum to the queue storage (the tiles past it keep no room and fall
+    back to exact binning), and the frames equal the exact binner's."""
+    w = small_terrain()
+    _, path = S.train_test_paths(2, 3)
+    with R.Renderer(w.width, w.height, tile=8) as ex, R.Renderer(w.width, w.height, tile=8) as sp:
+        ex.add_scene(w.scene)
+        sp.add_scene(w.scene)
+        sp.set_tuning(0, 16)  # bin every triangle
+        m = R.BinModel(sp, hidden=16, seed=4)
+        p = np.zeros(m.num_params, np.float32)
+        p[-1] = 30.0  # b3: every prediction saturates
+        m.set_params(p)
+        m.attach(sp, 64.0, 1 << 20)
+        for cam in path:
+            ex.render(cam, w.opts, R.BIN_EXACT)
+            sp.render(cam, w.opts, policy)
+            a, b = ex.readback(), sp.readback()
+            for name, x, y in zip(("vis", "depth", "rgba"), a, b):
+                assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), name
+        caps = sp.debug_read(R.DEBUG_TILE_CAPS, np.uint32)
+        assert caps.max() <= w.scene.num_triangles
+        m.close()
