@@ -598,10 +598,16 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->alive, nt, &synced));
     HIP_TRY(grow(r, r->survivors, nt * 4, &synced));
     const size_t tb = ((size_t)ntiles + 64) * 4;
+    const void *prev_bufs[4] = { r->tile_count[0].p, r->tile_count[1].p, r->tile_min_depth[0].p,
+                                 r->tile_min_depth[1].p };
     for (int i = 0; i < 2; ++i) {
         HIP_TRY(grow(r, r->tile_count[i], tb, &synced));
         HIP_TRY(grow(r, r->tile_min_depth[i], tb, &synced));
     }
+    // the previous frame's per-tile counts and depths are gone if their buffers moved
+    if (prev_bufs[0] != r->tile_count[0].p || prev_bufs[1] != r->tile_count[1].p ||
+        prev_bufs[2] != r->tile_min_depth[0].p || prev_bufs[3] != r->tile_min_depth[1].p)
+        r->have_prev = false;
     HIP_TRY(grow(r, r->tile_off, tb, &synced));
     HIP_TRY(grow(r, r->tile_cap, tb, &synced));
     HIP_TRY(grow(r, r->tile_pool, tb, &synced));
@@ -616,7 +622,6 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->queue, (size_t)r->queue_cap * 4, &synced));
     HIP_TRY(grow(r, r->pool, (size_t)r->pool_cap * 4, &synced));
     HIP_TRY(grow(r, r->big, (size_t)r->big_cap * 4, &synced));
-    if (synced) r->have_prev = false;  // the previous frame's tile counts may be gone
     return RAST_OK;
 }
 
@@ -1166,6 +1171,9 @@ static int enqueue_frame(rast_renderer *r)
     return RAST_OK;
 }
 
+// Largest queue storage (entries) that single-pass capacities alone may ask for.
+constexpr uint32_t kQueueGrowLimit = 1u << 28;
+
 // Re-run the last frame after growing whichever storage overflowed.
 static int check_frame(rast_renderer *r)
 {
@@ -1175,6 +1183,11 @@ static int check_frame(rast_renderer *r)
         if (!c.storage_overflow) {
             r->frame_pending = false;
             r->frame_valid = true;
+            // single-pass capacities beyond the queue storage were cut to fit
+            // (the frame is exact all the same); the next frame gets room
+            // for them, up to 2^28 entries
+            if (r->last_single && c.capacity_wanted > r->queue_cap && r->queue_cap < kQueueGrowLimit)
+                r->queue_cap = (uint32_t)std::min<uint64_t>((uint64_t)c.capacity_wanted * 5 / 4 + 1024, kQueueGrowLimit);
             return RAST_OK;
         }
         const uint64_t need_q = std::max<uint64_t>(c.total_pairs, c.capacity_total);

@@ -106,7 +106,8 @@ struct FrameCounters {
     uint32_t ovf_entries;     // single pass: queue entries beyond capacity (fallback input)
     uint32_t pool_used;       // fallback pool entries
     uint32_t storage_overflow;// queue / pool / big list storage exhausted -> frame invalid
-    uint32_t capacity_total;
+    uint32_t capacity_total;  // single pass: capacities as used (their sum fits the queue storage)
+    uint32_t capacity_wanted; // single pass: the capacities' sum before that bound (saturated)
 };
 
 // One frame of a launch: its parameters and the device buffers of the

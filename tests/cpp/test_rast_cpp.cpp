@@ -116,7 +116,7 @@ int main()
 
     std::vector<uint32_t> vis(W * H);
     std::vector<float> depth(W * H);
-    std::vector<uint32_t> drawn(W * H);
+    std::vector<uint32_t> drawn(W * H), rendered(W * H);
     // the reference app's order (CCR/main.cpp:418,440,529): the texture
     // manager lives on the stack and outlives the renderer it took its
     // textures from, so its textures are destroyed after the renderer
@@ -133,6 +133,7 @@ int main()
         renderer.StartTimer();
         renderer.RenderAuto(sm, tm, &cam);
         const float ms = renderer.StopTimer();
+        rendered = back;  // Present() filled the back buffer
         renderer.Readback(vis.data(), depth.data(), nullptr);
         EXPECT(renderer.GetTotalNumTriangles() == 4032, "triangles %u", renderer.GetTotalNumTriangles());
         std::printf("rendered in %.3f ms, %u of %u triangles visible\n", ms, renderer.GetTotalNumVisibleTriangles(),
@@ -191,7 +192,7 @@ int main()
         std::memcpy(&b, &odepth[p], 4);
         bad_depth += a != b;
         for (int c = 0; c < 4; ++c) {
-            const int d = std::abs((int)((back[p] >> (8 * c)) & 255u) - (int)((orgba[p] >> (8 * c)) & 255u));
+            const int d = std::abs((int)((rendered[p] >> (8 * c)) & 255u) - (int)((orgba[p] >> (8 * c)) & 255u));
             maxd = d > maxd ? d : maxd;
             bad_rgba += d > 1;
         }

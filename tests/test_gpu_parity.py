@@ -346,3 +346,32 @@ def test_interleaved_maps_follow_texture_changes(sample):
         same_as_oracle(r, [second[0], None, second[2], second[3]], "normal map destroyed")
         R.check(lib.rast_mesh_set_textures(mh, *[r.texture(t) for t in first]), "rast_mesh_set_textures")
         same_as_oracle(r, first, "first set again")
+
+
+@pytest.mark.parametrize("sample", [S.SAMPLE_POINT, S.SAMPLE_LINEAR])
+def test_phong_highlight_at_full_gloss(sample):
+    """The steepest specular term: gloss 255 (exponent 25) and a camera on
+    the mirror direction of the light over a flat floor, so the angle sweeps
+    from 0 up to its maximum |L| = 0.99938 across the frame and the sum
+    saturates (MaxToOne) at the centre.  Both sides call powf (the GPU as
+    exp2(e log2 x)); RGBA8 stays within 1/255."""
+    n = 24
+    pts, uvs = [], []
+    for j in range(n + 1):
+        for i in range(n + 1):
+            pts.append((-3.0 + 6.0 * i / n, 0.0, -3.0 + 6.0 * j / n))
+            uvs.append((i / n, j / n))
+    tris = []
+    for j in range(n):
+        for i in range(n):
+            a, b, c, d = j * (n + 1) + i, j * (n + 1) + i + 1, (j + 1) * (n + 1) + i, (j + 1) * (n + 1) + i + 1
+            tris += [(a, c, b), (b, c, d)]
+    floor = mesh(pts, tris, uvs=uvs)
+    floor.verts[:, 5:8] = (0.0, 1.0, 0.0)  # normal +y
+    floor.verts[:, 8:11] = (1.0, 0.0, 0.0)  # tangent +x
+    const = lambda r, g, b: np.full((4, 4), S.rgba(r, g, b), np.uint32)
+    floor.textures = [const(40, 40, 40), const(128, 128, 255), const(255, 255, 255), const(255, 255, 255)]
+    d = np.array([1.0, 1.0, -1.0]) / np.sqrt(3.0)  # the light's mirror direction off +y
+    cam = S.Camera(position=tuple(5.0 * d), forward=tuple(d), fov_deg=60.0)
+    st = check_frame(S.Scene([floor]), cam, S.Opts(cull_mode=S.CULL_NONE, sample_state=sample), 320, 240)
+    assert st["num_visible"] > 0
