@@ -413,7 +413,15 @@ extern "C" int rast_mesh_create(rast_renderer *r, const rast_ivertex *verts, uin
                         num_verts);
     HIP_TRY(hipSetDevice(r->device));
     std::vector<float> v16((size_t)num_verts * 16, 0.f);
-    for (uint32_t i = 0; i < num_verts; ++i) memcpy(&v16[(size_t)i * 16], &verts[i], sizeof(rast_ivertex));
+    float attr_max = 0.f;
+    for (uint32_t i = 0; i < num_verts; ++i) {
+        memcpy(&v16[(size_t)i * 16], &verts[i], sizeof(rast_ivertex));
+        for (int k = 0; k < 3; ++k) {
+            const float a = std::fabs(verts[i].n[k]), b = std::fabs(verts[i].tan[k]);
+            attr_max = std::max(attr_max, std::max(a, b));
+            if (a != a || b != b) attr_max = INFINITY;
+        }
+    }
     rast_mesh *m = new (std::nothrow) rast_mesh();
     if (!m) return rast_set_error(RAST_E_OUT_OF_MEMORY, "host allocation failed");
     m->r = r;
@@ -421,6 +429,7 @@ extern "C" int rast_mesh_create(rast_renderer *r, const rast_ivertex *verts, uin
     m->ni = num_indices;
     m->ntri = ntri;
     m->topology = topology;
+    m->attr_max = attr_max;
     for (int i = 0; i < 16; ++i) m->world[i] = world ? world[i] : ((i % 5 == 0) ? 1.f : 0.f);
     hipError_t e = hipMalloc((void **)&m->vert16, v16.size() * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&m->idx, (size_t)num_indices * 4);
@@ -1015,6 +1024,13 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
             d.topology = m->topology;
             d.has_diffuse = m->tex[0] != nullptr;
             d.job = j;
+            // the rotated normal / tangent stays below attr_max times the
+            // largest row sum of the world matrix's 3x3 part
+            float rowsum = 0.f;
+            for (int row = 0; row < 3; ++row)
+                rowsum = std::max(rowsum, std::fabs(M4c(m->world, row, 0)) + std::fabs(M4c(m->world, row, 1)) +
+                                              std::fabs(M4c(m->world, row, 2)));
+            d.shade_exact = !(m->attr_max * rowsum <= 4.f);
             mat4_mul(fc.viewproj, m->world, d.wvp);  // CCR/CUDARenderer.cu:241
             memcpy(d.world, m->world, sizeof d.world);
             for (int q = 0; q < 3; ++q) {

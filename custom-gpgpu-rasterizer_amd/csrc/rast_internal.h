@@ -10,7 +10,10 @@
 //   trirec   16 words / global triangle ID (written for survivors):
 //            [0..5] screen x0,y0,x1,y1,x2,y2  [6..8] z0,z1,z2  [9] |area|
 //            [10] xmin | ymin << 16  [11] xmax | ymax << 16 (pixel range
-//            [min, max)), [12..14] global vertex indices, [15] mesh index
+//            [min, max)), [12..14] global vertex indices, [15] mesh index,
+//            bit 31 set where shading must follow the reference's operation
+//            order (a vertex near or behind the camera plane, or a mesh with
+//            large normals / tangents: the fused forms could drift there)
 //   alive    u8 / triangle
 //   tiles    count / offset / cursor / capacity u32 per tile, queue entries
 //            u32 triangle IDs, plus one screen-wide "big" list
@@ -55,7 +58,8 @@ struct DevMesh {
     int32_t topology;
     int32_t has_diffuse;
     uint32_t job;             // index of the frame (job) this mesh entry belongs to
-    uint32_t pad_;
+    uint32_t shade_exact;     // 1: the mesh's normals / tangents (times the world matrix) may exceed 4
+                              // in magnitude; its triangles are shaded in the reference's operation order
     float wvp[16];            // data[col][row]
     float world[16];
     float cam_pos[4];         // camera position (xyz) and forward (xyz) of the frame

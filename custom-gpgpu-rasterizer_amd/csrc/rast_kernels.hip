@@ -659,7 +659,13 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                                       __float_as_uint(t.area)));
                 stg(r + 2, make_uint4(__float_as_uint(t.z2), __float_as_uint(t.z1),
                                       (uint32_t)xmin | ((uint32_t)ymin << 16), (uint32_t)xmax | ((uint32_t)ymax << 16)));
-                stg(r + 3, make_uint4(i0, i1, i2, mi));
+                // shading follows the reference's operation order where the
+                // interpolated attributes can be ill-conditioned: a vertex with
+                // clip w below 0.75 (at or behind the camera plane, 1/w above
+                // 4/3: its premultiplied attributes dominate and cancel), or a
+                // mesh whose attributes may exceed 4 in magnitude
+                const bool exact = M.shade_exact || !(P0.w >= 0.75f && P1.w >= 0.75f && P2.w >= 0.75f);
+                stg(r + 3, make_uint4(i0, i1, i2, mi | (exact ? 0x80000000u : 0u)));
             }
         }
         stg(J.alive + gid, (uint8_t)(ok ? 1 : 0));
@@ -1724,11 +1730,15 @@ __device__ __forceinline__ uint32_t light_pixel(const MapColours &c, float nx, f
 }
 
 // light_pixel in the reference's own operation order (no fused
-// multiply-adds; division, sqrt and powf as the reference calls them), for
-// triangles whose interpolated attributes are ill-conditioned (a vertex at or
-// behind the camera plane, or large attributes: see shading_risky): there the
-// fused forms can drift by more than 1/255 from the reference through
-// cancellation, while this form gives the CPU path's colour up to powf.
+// multiply-adds; division and sqrt correctly rounded; powf as in
+// light_pixel), for
+// triangles whose interpolated attributes can be ill-conditioned (setup marks
+// them: a vertex near or behind the camera plane, or large attributes): there
+// the fused forms can drift by more than 1/255 from the reference through
+// cancellation, while this form gives the CPU path's colour up to the powf
+// approximation's 1e-6.  On
+// the others every quantity of the lighting stays O(1) and the fused forms
+// are within about 1e-5 of it.
 __device__ __forceinline__ uint32_t light_pixel_exact(const MapColours &c, float nx, float ny, float nz, float tx,
                                                       float ty, float tz, float vx, float vy, float vz)
 {
@@ -1748,7 +1758,10 @@ __device__ __forceinline__ uint32_t light_pixel_exact(const MapColours &c, float
     float angle = vx * rx + vy * ry + vz * rz;
     angle = angle < 0.f ? 0.f : angle;
     angle = angle > 1.f ? 1.f : angle;
-    angle = powf(angle, c.g * 25.f);
+    // powf(angle, gloss * 25) as exp2(e * log2(angle)) on the hardware units:
+    // for angle in [0, 1] and e <= 25 its error is below 1e-6 absolute
+    const float e = c.g * 25.f;
+    angle = e == 0.f ? 1.f : (angle > 0.f ? __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(angle)) : 0.f);
     float outr = (0.05f + c.d[0] * oa) + c.s[0] * angle;
     float outg = (0.05f + c.d[1] * oa) + c.s[1] * angle;
     float outb = (0.05f + c.d[2] * oa) + c.s[2] * angle;
@@ -1761,48 +1774,32 @@ __device__ __forceinline__ uint32_t light_pixel_exact(const MapColours &c, float
     return pack_rgba(outr, outg, outb);
 }
 
-// Whether a triangle's interpolated shading attributes can be ill-conditioned:
-// a vertex with 1/w outside (0, 1.5] (at or behind the camera plane: its
-// attributes, premultiplied by 1/w, then dominate and cancel) or an
-// attribute component beyond 4 in magnitude.  Elsewhere every quantity of
-// the lighting stays O(1) and the fused forms are within about 1e-5.
-__device__ __forceinline__ bool shading_risky(float iw0, float iw1, float iw2, float amax)
-{
-    const bool iw_ok = iw0 > 0.f && iw0 <= 1.5f && iw1 > 0.f && iw1 <= 1.5f && iw2 > 0.f && iw2 <= 1.5f;
-    return !(iw_ok && amax <= 4.f);
-}
-
 // PS_Kernel + ShadePixelUnsafe (CCR/CUDARenderer.cu:833-897, CCR/CUDAROPs.cu:472-514)
-// with the attribute interpolation of RasterizePixelAtomic (CCR/CUDAROPs.cu:347-380).
+// with the attribute interpolation of RasterizePixelAtomic (CCR/CUDAROPs.cu:347-380)
+// for pixel (x, y) of the triangle whose setup record quads are q0, q1, q3.
 // The barycentrics, 1/w and the texture coordinates are computed exactly as
-// in the reference (so the same texels are fetched); the other attributes and
-// the lighting terms use fused multiply-adds and the hardware reciprocal /
-// rsqrt / exp2 / log2, which keeps RGBA8 within 1/255 of the reference
-// formula.  A wave shades rows of 64 pixels (fully coalesced vis reads and
-// RGBA8 writes).  A pixel's data is a chain of dependent gathers (vis,
-// triangle record, vertex attributes and texture table, texels); every
-// level's loads are issued together, with no branch between them, and range
-// checks are applied to the loaded values.  When all of a wave's pixels
-// belong to one mesh its texture table is read with scalar loads, and when
-// its four maps have one size the texel indices are computed once and the
-// texels come from the maps' interleaved copy (one 16-byte load per tap).
-template <bool kLinear>
-__device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
-                                            uint32_t x, uint32_t y, uint32_t id)
+// in the reference (so the same texels are fetched).  kExact = false: the
+// other attributes and the lighting terms use fused multiply-adds and the
+// hardware reciprocal / rsqrt / exp2 / log2, which keeps RGBA8 within 1/255
+// of the reference formula; kExact = true (triangles setup marked as
+// ill-conditioned) follows the reference's operation order throughout.  A
+// pixel's data is a chain of dependent gathers (vis, triangle record, vertex
+// attributes and texture table, texels); every level's loads are issued
+// together, with no branch between them, and range checks are applied to
+// the loaded values.  When all of a wave's pixels belong to one mesh its
+// texture table is read with scalar loads, and when its four maps have one
+// size the texel indices are computed once and the texels come from the
+// maps' interleaved copy (one 16-byte load per tap).
+template <bool kLinear, bool kExact>
+__device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
+                                           uint32_t x, uint32_t y, const uint4 q0, const uint4 q1, const uint4 q3)
 {
     const FrameParams &fp = J.fp;
     constexpr int mode = kLinear ? 1 : 0;
     const float *__restrict__ depth = J.depth;
-    const uint32_t *__restrict__ trirec = J.trirec;
     const float4 *__restrict__ attr = J.attr;
     uint32_t *__restrict__ rgba = J.rgba;
     const size_t pid = (size_t)y * fp.width + x;
-    if (id >= fp.num_tris) {
-        stg(rgba + pid, (uint32_t)(fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u));
-        return;
-    }
-    const uint4 *rp = reinterpret_cast<const uint4 *>(trirec + (size_t)id * kTriRecWords);
-    const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q3 = ldg(rp + 3);  // q2 holds z2, z1 and the bbox
     const float sx0 = __uint_as_float(q0.x), sy0 = __uint_as_float(q0.y);
     const float sx1 = __uint_as_float(q0.z), sy1 = __uint_as_float(q0.w);
     const float sx2 = __uint_as_float(q1.x), sy2 = __uint_as_float(q1.y);
@@ -1811,7 +1808,8 @@ __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, 
     const float4 a00 = ldg(A0), a10 = ldg(A1), a20 = ldg(A2);
     const float4 a01 = ldg(A0 + 1), a11 = ldg(A1 + 1), a21 = ldg(A2 + 1);
     const float4 a02 = ldg(A0 + 2), a12 = ldg(A1 + 2), a22 = ldg(A2 + 2);
-    const uint32_t mi = q3.w - J.mesh0 < fp.num_meshes ? q3.w : J.mesh0;  // entry in the launch's mesh table
+    const uint32_t me = q3.w & 0x7FFFFFFFu;  // bit 31: the exact-shading mark
+    const uint32_t mi = me - J.mesh0 < fp.num_meshes ? me : J.mesh0;  // entry in the launch's mesh table
     const uint32_t mi0 = __builtin_amdgcn_readfirstlane(mi);
     const bool uniform = __ballot(mi != mi0) == 0;
     // the table of a wave-uniform mesh stays in scalar registers (stex); a
@@ -1836,18 +1834,11 @@ __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, 
     w0 = w0 / area; w1 = w1 / area; w2 = w2 / area;
     const float i0 = a00.z, i1 = a10.z, i2 = a20.z;
     const float wI = 1.f / (i0 * w0 + i1 * w1 + i2 * w2);
-    // premultiplied attributes: |a'| / iw bounds the attribute itself
-    const float amax = fmaxf(fmaxf(fmaxf(fabsf(a00.w), fabsf(a01.x)), fmaxf(fabsf(a01.y), fabsf(a01.z))),
-                             fmaxf(fabsf(a01.w), fabsf(a02.x)));
-    const float bmax = fmaxf(fmaxf(fmaxf(fabsf(a10.w), fabsf(a11.x)), fmaxf(fabsf(a11.y), fabsf(a11.z))),
-                             fmaxf(fabsf(a11.w), fabsf(a12.x)));
-    const float cmax = fmaxf(fmaxf(fmaxf(fabsf(a20.w), fabsf(a21.x)), fmaxf(fabsf(a21.y), fabsf(a21.z))),
-                             fmaxf(fabsf(a21.w), fabsf(a22.x)));
-    const bool risky = shading_risky(i0, i1, i2, fmaxf(amax / i0, fmaxf(bmax / i1, cmax / i2)));
     // attributes arrive multiplied by their vertex's 1/w; u and v are formed
-    // in the reference's order, the rest with fused multiply-adds
+    // in the reference's order, the rest with fused multiply-adds unless kExact
 #define RAST_INTERP(c0, c1, c2) (w0 * (c0) + w1 * (c1) + w2 * (c2))
-#define RAST_FINTERP(c0, c1, c2) (risky ? RAST_INTERP(c0, c1, c2) : __builtin_fmaf(w2, (c2), __builtin_fmaf(w1, (c1), w0 * (c0))))
+#define RAST_FINTERP(c0, c1, c2) \
+    (kExact ? RAST_INTERP(c0, c1, c2) : __builtin_fmaf(w2, (c2), __builtin_fmaf(w1, (c1), w0 * (c0))))
     const float u = RAST_INTERP(a00.x, a10.x, a20.x) * wI;
     const float v = RAST_INTERP(a00.y, a10.y, a20.y) * wI;
     const float nx = RAST_FINTERP(a00.w, a10.w, a20.w) * wI;
@@ -1889,191 +1880,39 @@ __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, 
     const float vz = RAST_FINTERP(a02.w, a12.w, a22.w);
 #undef RAST_INTERP
 #undef RAST_FINTERP
-    stg(rgba + pid, risky ? light_pixel_exact(c, nx, ny, nz, tx, ty, tz, vx, vy, vz)
-                          : light_pixel(c, nx, ny, nz, tx, ty, tz, vx, vy, vz));
+    stg(rgba + pid, kExact ? light_pixel_exact(c, nx, ny, nz, tx, ty, tz, vx, vy, vz)
+                           : light_pixel(c, nx, ny, nz, tx, ty, tz, vx, vy, vz));
 }
 
-
-// ------------------------------------------------------ tile shading
-//
-// A shading block owns 64 x kShadeRows pixels (lane x of wave w: pixels
-// (x, w + 4 k)).  Neighbouring pixels mostly show the same triangles, so the
-// block first finds the distinct triangle IDs among its pixels (an LDS hash
-// table, inserted by the first pixel of every run of equal IDs in a row),
-// numbers them in pixel order, and stages each distinct triangle once in
-// LDS: its setup record and the three vertices' attribute records, turned
-// into what every pixel of it needs (edge vectors, 1 / area, the vertices'
-// attributes).  Pixels then shade from LDS; only the texels are
-// gathered per pixel.  More than kShadeTris distinct triangles are staged in
-// chunks of kShadeTris (numbered in pixel order, so a chunk's pixels are
-// mostly whole rows).
-
-constexpr int kShadeTris = 128;             // distinct triangles staged at once
-constexpr int kShadeHash = 1024;            // hash slots (at most 512 keys: load <= 1/2)
-constexpr int kShadePix = 64 * kShadeRows;  // pixels per block
-constexpr int kShadePPT = kShadeRows / 4;   // pixels per thread
-constexpr int kTriShadeQuads = 13;          // float4s per staged triangle
-constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
-
-__device__ __forceinline__ uint32_t shade_hash(uint32_t id) { return (id * 0x9E3779B1u) >> 22; }
-
-// fl(a / b) from r = fl(1 / b): q0 = fl(a r), then two residual corrections
-// q <- q + fl(a - b q) r.  The first leaves q within one ulp of a / b, so the
-// second's residual is exact and, r being the correctly rounded reciprocal,
-// rounds to fl(a / b) (Markstein's theorem), barring underflow and overflow:
-// callers redo quotients outside [2^-100, 2^100] (or NaN: r is set to NaN
-// where b is out of range) with the division.
-__device__ __forceinline__ float div_by(float a, float b, float r)
-{
-    const float q0 = a * r;
-    const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), r, q0);
-    return __builtin_fmaf(__builtin_fmaf(-b, q1, a), r, q1);
-}
-
-__device__ __forceinline__ bool quotient_ok(float a, float q)
-{
-    const float m = fabsf(q);
-    return (m >= 0x1p-100f || a == 0.f) && m < 0x1p100f;
-}
-
-// Stages triangle id for tile shading into dst[0..12]:
-//   0: sx0 sy0 e0x e0y   (e0 = v1 - v0: the edge of w2)
-//   1: sx1 sy1 e1x e1y   (e1 = v2 - v1: w0)
-//   2: sx2 sy2 e2x e2y   (e2 = v0 - v2: w1)
-//   3: area, 1 / area (NaN outside [2^-60, 2^60]), mesh entry (bits),
-//      1 where the shading inputs are ill-conditioned (shading_risky), else 0
-//   4: u0' u1' u2' v0'   5: v1' v2' iw0 iw1   (' = times the vertex's 1/w)
-//   6..12: iw2, then per component of n', tan', vd' the three vertices'
-//          values (interpolated as w0 a0 + w1 a1 + w2 a2: a form in
-//          differences to one vertex cancels badly when a vertex lies near
-//          the camera plane and its 1/w is huge)
-// The edge vectors are the same binary32 differences the reference forms per
-// pixel (CCR/CUDAROPs.cu:131-133).
-__device__ __forceinline__ void stage_triangle(const JobDesc &J, uint32_t id, float4 *dst)
-{
-    const uint4 *rp = reinterpret_cast<const uint4 *>(J.trirec + (size_t)id * kTriRecWords);
-    const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q3 = ldg(rp + 3);
-    const float4 *A0 = J.attr + (size_t)q3.x * 4, *A1 = J.attr + (size_t)q3.y * 4, *A2 = J.attr + (size_t)q3.z * 4;
-    const float4 a00 = ldg(A0), a10 = ldg(A1), a20 = ldg(A2);
-    const float4 a01 = ldg(A0 + 1), a11 = ldg(A1 + 1), a21 = ldg(A2 + 1);
-    const float4 a02 = ldg(A0 + 2), a12 = ldg(A1 + 2), a22 = ldg(A2 + 2);
-    const float sx0 = __uint_as_float(q0.x), sy0 = __uint_as_float(q0.y);
-    const float sx1 = __uint_as_float(q0.z), sy1 = __uint_as_float(q0.w);
-    const float sx2 = __uint_as_float(q1.x), sy2 = __uint_as_float(q1.y);
-    const float area = __uint_as_float(q1.w);
-    const uint32_t mi = q3.w - J.mesh0 < J.fp.num_meshes ? q3.w : J.mesh0;  // entry in the launch's mesh table
-    const float r = (area >= 0x1p-60f && area <= 0x1p60f) ? 1.f / area : __uint_as_float(0x7FC00000u);
-    dst[0] = make_float4(sx0, sy0, sx1 - sx0, sy1 - sy0);
-    dst[1] = make_float4(sx1, sy1, sx2 - sx1, sy2 - sy1);
-    dst[2] = make_float4(sx2, sy2, sx0 - sx2, sy0 - sy2);
-    const float m0 = fmaxf(fmaxf(fmaxf(fabsf(a00.w), fabsf(a01.x)), fmaxf(fabsf(a01.y), fabsf(a01.z))),
-                           fmaxf(fmaxf(fabsf(a01.w), fabsf(a02.x)), fmaxf(fmaxf(fabsf(a02.y), fabsf(a02.z)), fabsf(a02.w))));
-    const float m1 = fmaxf(fmaxf(fmaxf(fabsf(a10.w), fabsf(a11.x)), fmaxf(fabsf(a11.y), fabsf(a11.z))),
-                           fmaxf(fmaxf(fabsf(a11.w), fabsf(a12.x)), fmaxf(fmaxf(fabsf(a12.y), fabsf(a12.z)), fabsf(a12.w))));
-    const float m2 = fmaxf(fmaxf(fmaxf(fabsf(a20.w), fabsf(a21.x)), fmaxf(fabsf(a21.y), fabsf(a21.z))),
-                           fmaxf(fmaxf(fabsf(a21.w), fabsf(a22.x)), fmaxf(fmaxf(fabsf(a22.y), fabsf(a22.z)), fabsf(a22.w))));
-    const bool risky = shading_risky(a00.z, a10.z, a20.z, fmaxf(m0 / a00.z, fmaxf(m1 / a10.z, m2 / a20.z)));
-    dst[3] = make_float4(area, r, __uint_as_float(mi), risky ? 1.f : 0.f);
-    // attribute record of a vertex: (u', v', iw, nx') (ny', nz', tx', ty') (tz', vdx', vdy', vdz')
-    dst[4] = make_float4(a00.x, a10.x, a20.x, a00.y);
-    dst[5] = make_float4(a10.y, a20.y, a00.z, a10.z);
-    dst[6] = make_float4(a20.z, a00.w, a10.w, a20.w);
-    dst[7] = make_float4(a01.x, a11.x, a21.x, a01.y);
-    dst[8] = make_float4(a11.y, a21.y, a01.z, a11.z);
-    dst[9] = make_float4(a21.z, a01.w, a11.w, a21.w);
-    dst[10] = make_float4(a02.x, a12.x, a22.x, a02.y);
-    dst[11] = make_float4(a12.y, a22.y, a02.z, a12.z);
-    dst[12] = make_float4(a22.z, a02.w, a12.w, a22.w);
-}
-
-// PBR shading of pixel (x, y) of the staged triangle T (the arithmetic of
-// shade_pixel: barycentrics, 1/w, u and v exactly as the reference forms
-// them, so the same texels are fetched; the other attributes from the
-// staged vertex values with fused multiply-adds, as shade_pixel does).  Returns false, shading nothing, when the triangle's
-// mesh has no diffuse map (the caller shades it with shade_pixel).
+// Shades pixel (x, y) showing triangle id (clear colour for none): reads the
+// triangle's setup record and runs the fused or the exact shading body as
+// setup marked the triangle (two separate code paths, so the rare exact one
+// does not raise the register count of the fused one).
 template <bool kLinear>
-__device__ __forceinline__ bool shade_staged(const float4 *T, const DevMesh *__restrict__ meshes, const float *unit,
-                                             float px, float py, uint32_t &out)
+__device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
+                                            uint32_t x, uint32_t y, uint32_t id)
 {
-    constexpr int mode = kLinear ? 1 : 0;
-    const float4 t0 = T[0], t1 = T[1], t2 = T[2], t3 = T[3];
-    // the edge functions of CCR/CUDAROPs.cu:120-145, weights / |area|
-    const float e2 = (px - t0.x) * t0.w - (py - t0.y) * t0.z;
-    const float e0 = (px - t1.x) * t1.w - (py - t1.y) * t1.z;
-    const float e1 = (px - t2.x) * t2.w - (py - t2.y) * t2.z;
-    const float area = t3.x, r = t3.y;
-    float w0 = div_by(e0, area, r), w1 = div_by(e1, area, r), w2 = div_by(e2, area, r);
-    if (!(quotient_ok(e0, w0) && quotient_ok(e1, w1) && quotient_ok(e2, w2))) {
-        w0 = e0 / area;
-        w1 = e1 / area;
-        w2 = e2 / area;
+    const FrameParams &fp = J.fp;
+    if (id >= fp.num_tris) {
+        stg(J.rgba + (size_t)y * fp.width + x, (uint32_t)(fp.visualisation == 0 ? fp.clear_rgba : 0xFF000000u));
+        return;
     }
-    const float4 t4 = T[4], t5 = T[5], t6 = T[6];
-    const float wI = 1.f / (t5.z * w0 + t5.w * w1 + t6.x * w2);
-    const float u = (w0 * t4.x + w1 * t4.y + w2 * t4.z) * wI;
-    const float v = (w0 * t4.w + w1 * t5.x + w2 * t5.y) * wI;
-    const uint32_t mi = __float_as_uint(t3.z);
-    const uint32_t mi0 = __builtin_amdgcn_readfirstlane(mi);
-    const bool uniform = __ballot(mi != mi0) == 0;
-    MapColours c;
-    if (uniform) {
-        const DevMesh &M = meshes[mi0];
-        if (!M.has_diffuse) return false;
-        DevTexture stex[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) stex[k] = M.tex[k];
-        if (M.texq) c = sample_quads(M.texq, stex[0], mode, u, v, unit);
-        else if (stex[1].w == stex[0].w && stex[2].w == stex[0].w && stex[3].w == stex[0].w &&
-                 stex[1].h == stex[0].h && stex[2].h == stex[0].h && stex[3].h == stex[0].h)
-            c = sample_maps<true>(stex, mode, u, v, unit);
-        else
-            c = sample_maps<false>(stex, mode, u, v, unit);
-    } else {
-        if (!ldg(&meshes[mi].has_diffuse)) return false;
-        DevTexture vtex[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) vtex[k] = ldg(meshes[mi].tex + k);
-        c = sample_maps<false>(vtex, mode, u, v, unit);
-    }
-    const float4 t7 = T[7], t8 = T[8], t9 = T[9], t10 = T[10], t11 = T[11], t12 = T[12];
-    if (t3.w != 0.f) {  // ill-conditioned: the reference's operation order throughout
-#define RAST_XINTERP(a0, a1, a2) (w0 * (a0) + w1 * (a1) + w2 * (a2))
-        out = light_pixel_exact(c, RAST_XINTERP(t6.y, t6.z, t6.w) * wI, RAST_XINTERP(t7.x, t7.y, t7.z) * wI,
-                                RAST_XINTERP(t7.w, t8.x, t8.y) * wI, RAST_XINTERP(t8.z, t8.w, t9.x),
-                                RAST_XINTERP(t9.y, t9.z, t9.w), RAST_XINTERP(t10.x, t10.y, t10.z),
-                                RAST_XINTERP(t10.w, t11.x, t11.y), RAST_XINTERP(t11.z, t11.w, t12.x),
-                                RAST_XINTERP(t12.y, t12.z, t12.w));
-#undef RAST_XINTERP
-        return true;
-    }
-#define RAST_VINTERP(a0, a1, a2) __builtin_fmaf(w2, (a2), __builtin_fmaf(w1, (a1), w0 * (a0)))
-    const float nx = RAST_VINTERP(t6.y, t6.z, t6.w) * wI;
-    const float ny = RAST_VINTERP(t7.x, t7.y, t7.z) * wI;
-    const float nz = RAST_VINTERP(t7.w, t8.x, t8.y) * wI;
-    const float tx = RAST_VINTERP(t8.z, t8.w, t9.x);
-    const float ty = RAST_VINTERP(t9.y, t9.z, t9.w);
-    const float tz = RAST_VINTERP(t10.x, t10.y, t10.z);
-    const float vx = RAST_VINTERP(t10.w, t11.x, t11.y);
-    const float vy = RAST_VINTERP(t11.z, t11.w, t12.x);
-    const float vz = RAST_VINTERP(t12.y, t12.z, t12.w);
-#undef RAST_VINTERP
-    out = light_pixel(c, nx, ny, nz, tx, ty, tz, vx, vy, vz);
-    return true;
+    const uint4 *rp = reinterpret_cast<const uint4 *>(J.trirec + (size_t)id * kTriRecWords);
+    const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q3 = ldg(rp + 3);  // q2 holds z2, z1 and the bbox
+    if (q3.w >> 31) shade_body<kLinear, true>(meshes, J, unit, x, y, q0, q1, q3);
+    else shade_body<kLinear, false>(meshes, J, unit, x, y, q0, q1, q3);
 }
 
+
+// A block shades 64 x kShadeRows pixels: lane x of wave w takes the pixels
+// (x, w), (x, w + 4), ...  The vis reads of all of them go out first, so each
+// further pixel of a lane waits for three dependent gathers instead of four.
 template <bool kLinear>
 __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
                                                const uint16_t *__restrict__ table)
 {
     __shared__ float unit[256];
-    __shared__ float4 tris[kShadeTris * kTriShadeQuads];
-    __shared__ uint32_t hkey[kShadeHash];
-    __shared__ uint16_t hidx[kShadeHash];
-    __shared__ uint32_t ulist[kShadePix];
-    __shared__ uint32_t wcount[kShadePPT * 4];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    unit[tid] = byte_unit(tid);
-    for (uint32_t i = tid; i < (uint32_t)kShadeHash; i += 256) hkey[i] = kHashEmpty;
+    unit[threadIdx.x] = byte_unit(threadIdx.x);
     __syncthreads();
     uint32_t local;
     const JobDesc &J = jobs[job_of(table, jobs, ST_SHADE, local)];
@@ -2086,104 +1925,20 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
         __shared__ uint32_t red[4];
         publish_counters(J, job_alive<4>(J, red));
     }
+    constexpr int P = kShadeRows / 4;
     const uint32_t bw = (fp.width + 63u) / 64u;
-    const uint32_t x = (local % bw) * 64u + lane;
-    const uint32_t yb = (local / bw) * (uint32_t)kShadeRows + wave;
-    uint32_t *__restrict__ rgba = J.rgba;
-    uint32_t ids[kShadePPT];
-    bool inside[kShadePPT], has[kShadePPT];
+    const uint32_t x = (local % bw) * 64u + (threadIdx.x & 63u);
+    const uint32_t y0 = (local / bw) * (uint32_t)kShadeRows + (threadIdx.x >> 6);
+    if (x >= fp.width || y0 >= fp.height) return;
+    uint32_t ids[P];
 #pragma unroll
-    for (int k = 0; k < kShadePPT; ++k) {
-        const uint32_t y = yb + 4u * k;
-        inside[k] = x < fp.width && y < fp.height;
-        ids[k] = inside[k] ? ldg(J.vis + (size_t)y * fp.width + x) : kVisNone;
-        has[k] = inside[k] && ids[k] < fp.num_tris;
-    }
-    if (fp.visualisation != 0) {  // depth / normal visualisation: per-pixel path
-#pragma unroll
-        for (int k = 0; k < kShadePPT; ++k)
-            if (inside[k]) shade_pixel<kLinear>(meshes, J, unit, x, yb + 4u * k, ids[k]);
-        return;
-    }
-    // distinct IDs: the first pixel of every run of equal IDs in a row inserts
-    uint32_t slot[kShadePPT];
-    bool head[kShadePPT], won[kShadePPT];
-#pragma unroll
-    for (int k = 0; k < kShadePPT; ++k) {
-        const uint32_t left = __shfl_up(ids[k], 1);
-        head[k] = has[k] && !(lane > 0 && left == ids[k]);
-        won[k] = false;
-        slot[k] = 0;
-        if (head[k]) {
-            uint32_t h = shade_hash(ids[k]);
-            for (;;) {
-                const uint32_t old = atomicCAS(&hkey[h], kHashEmpty, ids[k]);
-                if (old == kHashEmpty || old == ids[k]) {
-                    won[k] = old == kHashEmpty;
-                    break;
-                }
-                h = (h + 1u) & (uint32_t)(kShadeHash - 1);
-            }
-            slot[k] = h;
-        }
-    }
-    // number the distinct IDs in pixel order (k, then wave, then lane)
-    uint64_t wm[kShadePPT];
-#pragma unroll
-    for (int k = 0; k < kShadePPT; ++k) {
-        wm[k] = __ballot(won[k]);
-        if (lane == 0) wcount[k * 4 + wave] = (uint32_t)__popcll(wm[k]);
-    }
-    __syncthreads();
-    uint32_t total = 0, before[kShadePPT];
-#pragma unroll
-    for (int k = 0; k < kShadePPT; ++k) before[k] = 0;
-    for (int q = 0; q < kShadePPT * 4; ++q) {
-        const uint32_t c = wcount[q];
-#pragma unroll
-        for (int k = 0; k < kShadePPT; ++k)
-            if (q < k * 4 + (int)wave) before[k] += c;
-        total += c;
+    for (int k = 0; k < P; ++k) {
+        const uint32_t y = y0 + 4u * k;
+        ids[k] = y < fp.height ? ldg(J.vis + (size_t)y * fp.width + x) : 0u;
     }
 #pragma unroll
-    for (int k = 0; k < kShadePPT; ++k)
-        if (won[k]) {
-            const uint32_t idx = before[k] + popc_below(wm[k]);
-            hidx[slot[k]] = (uint16_t)idx;
-            ulist[idx] = ids[k];
-        }
-    __syncthreads();
-    uint32_t cidx[kShadePPT];
-#pragma unroll
-    for (int k = 0; k < kShadePPT; ++k) {
-        cidx[k] = 0xFFFFFFFFu;
-        if (has[k]) {
-            uint32_t h = slot[k];
-            if (!head[k]) {  // the ID is in the table (its run's head put it there)
-                h = shade_hash(ids[k]);
-                while (hkey[h] != ids[k]) h = (h + 1u) & (uint32_t)(kShadeHash - 1);
-            }
-            cidx[k] = hidx[h];
-        } else if (inside[k]) {
-            stg(rgba + (size_t)(yb + 4u * k) * fp.width + x, fp.clear_rgba);
-        }
-    }
-    for (uint32_t c0 = 0; c0 < total; c0 += (uint32_t)kShadeTris) {
-        const uint32_t n = total - c0 < (uint32_t)kShadeTris ? total - c0 : (uint32_t)kShadeTris;
-        if (c0) __syncthreads();  // the previous chunk is shaded
-        if (tid < n) stage_triangle(J, ulist[c0 + tid], tris + tid * kTriShadeQuads);
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kShadePPT; ++k) {
-            if (cidx[k] - c0 >= n) continue;  // no triangle, or one of another chunk
-            const uint32_t y = yb + 4u * k;
-            uint32_t col = 0;
-            if (shade_staged<kLinear>(tris + (cidx[k] - c0) * kTriShadeQuads, meshes, unit, (float)x, (float)y, col))
-                stg(rgba + (size_t)y * fp.width + x, col);
-            else
-                shade_pixel<kLinear>(meshes, J, unit, x, y, ids[k]);  // vertex colour
-        }
-    }
+    for (int k = 0; k < P; ++k)
+        if (y0 + 4u * k < fp.height) shade_pixel<kLinear>(meshes, J, unit, x, y0 + 4u * k, ids[k]);
 }
 
 // DrawTextureKernel / DrawTextureGlobalKernel (CCR/CUDARenderer.cu:1565-1617):

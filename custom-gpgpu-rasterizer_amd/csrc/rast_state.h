@@ -91,6 +91,7 @@ struct rast_mesh {
     float world[16];
     rast_texture *tex[4] = { nullptr, nullptr, nullptr, nullptr };
     const uint4 *texq = nullptr;  // interleaved maps for tex[0..3], set by prepare_job
+    float attr_max = 0.f;         // largest |component| of the input normals and tangents
     uint64_t tri_base = 0;
 };
 
