@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -721,6 +722,68 @@ extern "C" int rast_binmodel_forward(rast_binmodel *m, const float *x, uint32_t 
                           (float *)m->io_p.p, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
     HIP_TRY(hipMemcpy(pred, m->io_p.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return RAST_OK;
+}
+
+// ------------------------------------------------ allocation-free forward
+
+// Device staging of rast_binmodel_forward_noalloc: at most
+// kNoAllocMaxSamples samples (F floats each) and their predictions, 2.25 MiB
+// on every GPU the library runs on, held until the process ends.  In host
+// code this name is not a device address: the array's address on a device is
+// read once with hipGetSymbolAddress (that device current) and used instead.
+constexpr uint32_t kNoAllocMaxSamples = 1u << 16;
+static __device__ __align__(256) float scratch[(size_t)kNoAllocMaxSamples * (kNumFeatures + 1)];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+
+// Per-device address of `scratch`, and the lock that keeps calls on one GPU
+// from overlapping (they share that GPU's array).
+struct NoAllocState {
+    std::mutex mu;
+    float *ptr[kNoAllocDevices] = {};
+};
+NoAllocState g_noalloc;
+
+// The calling thread holds g_noalloc.mu and has made `device` current.
+int noalloc_scratch(int device, float **out)
+{
+    if (device < 0 || device >= kNoAllocDevices)
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_forward_noalloc: device %d above %d", device,
+                              kNoAllocDevices - 1);
+    if (!g_noalloc.ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return rast_set_error(RAST_E_HIP, "rast_binmodel_forward_noalloc: hipGetSymbolAddress(scratch): %s",
+                                  hipGetErrorString(e));
+        g_noalloc.ptr[device] = (float *)p;
+    }
+    *out = g_noalloc.ptr[device];
+    return RAST_OK;
+}
+
+}  // namespace
+
+extern "C" int rast_binmodel_forward_noalloc(rast_binmodel *m, const float *x, uint32_t n, float *pred)
+{
+    if (!m || !x || !pred || n == 0)
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_forward_noalloc: bad arguments");
+    if (n > kNoAllocMaxSamples)
+        return rast_set_error(RAST_E_INVALID_ARG, "rast_binmodel_forward_noalloc: %u samples above the maximum %u", n,
+                              kNoAllocMaxSamples);
+    std::lock_guard<std::mutex> lock(g_noalloc.mu);
+    HIP_TRY(hipSetDevice(m->device));
+    float *base = nullptr;
+    RAST_TRY(noalloc_scratch(m->device, &base));
+    float *sx = base, *sp = base + (size_t)kNoAllocMaxSamples * F;
+    // every word the kernel reads is written first: nothing to clear
+    HIP_TRY(hipMemcpyAsync(sx, x, (size_t)n * F * 4, hipMemcpyHostToDevice, m->stream));
+    RAST_TRY(launch_infer(m->H, n, sx, (const float *)m->params.p, 1.f, 0, nullptr, sp, m->stream));
+    HIP_TRY(hipMemcpyAsync(pred, sp, (size_t)n * 4, hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
     return RAST_OK;
 }
 

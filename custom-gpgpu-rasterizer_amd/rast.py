@@ -36,7 +36,7 @@ EXPORTS = [
     "rast_renderer_set_binmodel", "rast_binmodel_record", "rast_binmodel_clear_data", "rast_binmodel_train",
     "rast_binmodel_get_params", "rast_binmodel_set_params", "rast_binmodel_num_params", "rast_binmodel_forward",
     "rast_binmodel_loss_grad", "rast_render_batch", "rast_renderer_create_view", "rast_texture_create_pitched",
-    "rast_timer_start", "rast_timer_stop", "rast_draw_texture",
+    "rast_timer_start", "rast_timer_stop", "rast_draw_texture", "rast_binmodel_forward_noalloc",
 ]
 
 
@@ -126,6 +126,7 @@ def lib():
         L.rast_binmodel_set_params.argtypes = [vp, vp, C.c_size_t]
         L.rast_binmodel_num_params.argtypes = [vp, C.POINTER(C.c_size_t)]
         L.rast_binmodel_forward.argtypes = [vp, vp, C.c_uint32, vp]
+        L.rast_binmodel_forward_noalloc.argtypes = [vp, vp, C.c_uint32, vp]
         L.rast_binmodel_loss_grad.argtypes = [vp, vp, vp, C.c_uint32, C.c_float, vp, vp]
         L.rast_render_batch.argtypes = [C.POINTER(Job), C.c_uint32]
         L.rast_timer_start.argtypes = [vp]

@@ -301,6 +301,15 @@ int rast_binmodel_set_params(rast_binmodel *m, const float *src, size_t count);
 int rast_binmodel_num_params(rast_binmodel *m, size_t *out);
 /* Test hooks: one forward / loss-gradient evaluation on host-provided data. */
 int rast_binmodel_forward(rast_binmodel *m, const float *x, uint32_t n, float *pred);
+/* No counterpart in the reference.  rast_binmodel_forward for callers that
+ * cannot allocate (serving loops, code that must not call hipMalloc): the
+ * same kernel and result, with its device staging in a static array of the
+ * library (2.25 MiB on every GPU it runs on, held until the process ends)
+ * instead of buffers the model grows.  At most 65536 samples per call
+ * (RAST_E_INVALID_ARG above that).  Calls on one GPU share that array; the
+ * library serializes them (a call returns after its predictions are copied
+ * out), so overlapping calls from several threads wait for each other. */
+int rast_binmodel_forward_noalloc(rast_binmodel *m, const float *x, uint32_t n, float *pred);
 int rast_binmodel_loss_grad(rast_binmodel *m, const float *x, const float *target, uint32_t n, float tau,
                             float *loss, float *grad);
 
