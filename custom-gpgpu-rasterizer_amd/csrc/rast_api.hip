@@ -342,7 +342,7 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
     }
     DBuf *bufs[] = { &r->pos4, &r->attr, &r->trirec,
                      &r->alive, &r->survivors, &r->tile_count[0], &r->tile_count[1], &r->tile_off, &r->tile_cap,
-                     &r->tile_pool, &r->tile_cursor, &r->density, &r->tile_min_depth[0], &r->tile_min_depth[1], &r->keys,
+                     &r->tile_pool, &r->tile_cursor, &r->density[0], &r->density[1], &r->tile_min_depth[0], &r->tile_min_depth[1], &r->keys,
                      &r->queue, &r->pool, &r->big, &r->ovf, &r->counters, &r->vis, &r->depth,
                      &r->rgba };
     for (DBuf *b : bufs) dfree(*b);
@@ -621,7 +621,7 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->tile_cap, tb, &synced));
     HIP_TRY(grow(r, r->tile_pool, tb, &synced));
     HIP_TRY(grow(r, r->tile_cursor, tb, &synced));
-    HIP_TRY(grow(r, r->density, tb, &synced));
+    for (int i = 0; i < 2; ++i) HIP_TRY(grow(r, r->density[i], tb, &synced));
     HIP_TRY(grow(r, r->counters, sizeof(FrameCounters) + 64, &synced));
     if (r->queue_cap == 0) r->queue_cap = std::max<uint32_t>(1u << 20, std::min<uint64_t>(4ull * nt, 0x7FFFFFFFull));
     if (r->pool_cap == 0) r->pool_cap = 1u << 20;
@@ -757,6 +757,9 @@ void plan_free(LaunchPlan &p)
     }
     if (p.fork) (void)hipEventDestroy(p.fork);
     p.fork = nullptr;
+    if (p.cap_fork) (void)hipEventDestroy(p.cap_fork);
+    if (p.cap_join) (void)hipEventDestroy(p.cap_join);
+    p.cap_fork = p.cap_join = nullptr;
     if (p.done) (void)hipEventDestroy(p.done);
     p.done = nullptr;
     p.done_pending = false;
@@ -799,6 +802,8 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     if (!P.done) HIP_TRY(hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
     if (!P.fork) {
         HIP_TRY(hipEventCreateWithFlags(&P.fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&P.cap_fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&P.cap_join, hipEventDisableTiming));
         for (int i = 0; i < 2; ++i) {
             HIP_TRY(hipStreamCreateWithFlags(&P.side[i], hipStreamNonBlocking));
             HIP_TRY(hipEventCreateWithFlags(&P.joined[i], hipEventDisableTiming));
@@ -830,8 +835,11 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         nb[ST_INIT] = ceil_div(ntiles, 256);
         nb[ST_CAP] = single ? ceil_div(ntiles, kCapacityTilesPerBlock) : 0;
         nb[ST_FILL] = single ? 0 : std::min<uint32_t>(ceil_div(ntris, 256), 1024);
-        nb[ST_FBALLOC] = single ? ceil_div(ntiles, 256) : 0;
-        nb[ST_FBFILL] = single ? std::min<uint32_t>(ceil_div(ntris, 256), 256) : 0;
+        // entries beyond capacity: placed by two passes for the previous-frame
+        // policy, gathered by the resolve pass for learned capacities
+        const bool placed = single && !job_learned(r);
+        nb[ST_FBALLOC] = placed ? ceil_div(ntiles, 256) : 0;
+        nb[ST_FBFILL] = placed ? std::min<uint32_t>(ceil_div(ntris, 256), 256) : 0;
         const int res = r->tile == 8 ? ST_RES8 : (r->tile == 16 ? ST_RES16 : ST_RES32);
         nb[res] = ntiles;
         nb[ST_SHADE] = job_shades(r) ? ceil_div(r->W, 64) * ceil_div(r->H, (uint32_t)kShadeRows) : 0;
@@ -998,7 +1006,8 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
             J.alive_part0 = P.tri_block0[me];
             J.alive_parts = P.tri_block0[me + meshes.size() - 1] + ceil_div(meshes.back()->ntri, 256) - J.alive_part0;
         }
-        J.density = r->model ? (uint32_t *)r->density.p : nullptr;  // feature of the capacity model
+        J.density = r->model ? (uint32_t *)r->density[r->cur].p : nullptr;  // feature of the capacity model
+        J.prev_density = r->model ? (const uint32_t *)r->density[r->cur ^ 1].p : nullptr;
         J.tile_min_depth = (float *)r->tile_min_depth[r->cur].p;
         J.keys = (unsigned long long *)r->keys.p;
         J.vis = (uint32_t *)r->vis.p;
@@ -1066,12 +1075,25 @@ static int enqueue_launch(LaunchPlan &P, hipStream_t s, rast_renderer *timed)
     HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage, P.stage_bytes, hipMemcpyHostToDevice, s));
     RAST_TRY(mark(timed, EV_START, s));
     HIP_TRY(launch_frame_init(L, s));
-    HIP_TRY(launch_vertex(L, s));
-    RAST_TRY(mark(timed, EV_VERTEX, s));
-    HIP_TRY(launch_capacity(L, s));
-    RAST_TRY(mark(timed, EV_MODEL, s));
-    HIP_TRY(launch_scan(L, true, s));
-    RAST_TRY(mark(timed, EV_SCANCAP, s));
+    if (L.cap_jobs && !timed) {
+        // the capacities read only the previous frame's counts, depths and
+        // densities: the model and the capacity scan run on a side stream
+        // beside the vertex pass (joined before setup, which needs both)
+        HIP_TRY(hipEventRecord(P.cap_fork, s));
+        HIP_TRY(hipStreamWaitEvent(P.side[0], P.cap_fork, 0));
+        HIP_TRY(launch_capacity(L, P.side[0]));
+        HIP_TRY(launch_scan(L, true, P.side[0]));
+        HIP_TRY(hipEventRecord(P.cap_join, P.side[0]));
+        HIP_TRY(launch_vertex(L, s));
+        HIP_TRY(hipStreamWaitEvent(s, P.cap_join, 0));
+    } else {  // one stage after the other (per-stage timing)
+        HIP_TRY(launch_vertex(L, s));
+        RAST_TRY(mark(timed, EV_VERTEX, s));
+        HIP_TRY(launch_capacity(L, s));
+        RAST_TRY(mark(timed, EV_MODEL, s));
+        HIP_TRY(launch_scan(L, true, s));
+        RAST_TRY(mark(timed, EV_SCANCAP, s));
+    }
     HIP_TRY(launch_setup(L, s));
     RAST_TRY(mark(timed, EV_SETUP, s));
     HIP_TRY(launch_scan(L, false, s));
@@ -1279,7 +1301,7 @@ extern "C" int rast_get_stats(rast_renderer *r, rast_stats *out)
     s.ms_setup = elapsed(r, EV_SCANCAP, EV_SETUP);
     s.ms_count = s.ms_setup;
     s.ms_fill = elapsed(r, EV_SCANCNT, EV_FILL);
-    s.ms_fallback = elapsed(r, EV_FILL, EV_FALLBACK);
+    s.ms_fallback = elapsed(r, EV_FILL, EV_FALLBACK);  // only the previous-frame policy has these passes
     s.ms_raster = elapsed(r, EV_FALLBACK, EV_RASTER);
     s.ms_shade = elapsed(r, EV_RASTER, EV_SHADE);
     s.ms_total = elapsed(r, EV_START, EV_SHADE);

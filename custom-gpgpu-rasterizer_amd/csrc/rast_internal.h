@@ -91,8 +91,8 @@ enum Stage : int {
     ST_INIT,      // per-tile reset (256 tiles per block)
     ST_CAP,       // single-pass capacities (64 tiles per block)
     ST_FILL,      // exact second pass (grid-stride over binned triangles)
-    ST_FBALLOC,   // single-pass fallback allocation (256 tiles per block)
-    ST_FBFILL,    // single-pass fallback placement (grid-stride over entries)
+    ST_FBALLOC,   // previous-frame fallback allocation (256 tiles per block)
+    ST_FBFILL,    // previous-frame fallback placement (grid-stride over entries)
     ST_RES8,      // resolve, 8x8 tiles (one block per tile, in launch order)
     ST_RES16,     // resolve, 16x16 tiles
     ST_RES32,     // resolve, 32x32 tiles
@@ -138,7 +138,7 @@ struct JobDesc {
     uint32_t *tile_count;        // exact: counts; single pass: cursors (end = exact counts)
     uint32_t *tile_off;          // num_tiles + 1
     uint32_t *tile_cap;          // single pass capacities
-    uint32_t *tile_pool;         // per tile: fallback pool offset, or ~0u if the tile did not overflow
+    uint32_t *tile_pool;         // per tile: fallback pool offset, or ~0u (previous-frame policy)
     uint32_t *tile_cursor;       // exact fill cursors / fallback cursors
     const uint32_t *prev_count;  // previous frame's exact per-tile counts
     const float *prev_depth;     // previous frame's per-tile min depth
@@ -155,7 +155,9 @@ struct JobDesc {
     // every block serializes those updates at its cache line.
     uint32_t *alive_part;
     uint32_t alive_part0, alive_parts;
-    uint32_t *density;           // per tile projected-vertex count (capacity model), may be null
+    uint32_t *density;           // per tile projected-vertex count of this frame (capacity model), may be null
+    const uint32_t *prev_density;// the previous frame's (the capacity model's feature: it runs beside the
+                                 // vertex pass, before this frame's counts exist), may be null
     float *tile_min_depth;       // per tile min depth written by the resolve pass
     unsigned long long *keys;    // per pixel (z << 32 | ~(id + 1)), reset to depth 1 / none after each resolve
     uint32_t *vis;

@@ -23,9 +23,10 @@
 //              spanning more than big_threshold tiles go to one screen list
 //   k_scan     per-tile exclusive scan of counts / capacities
 //   k_fill     exact second pass over the binned triangles
-//   k_fallback single pass: exact placement of entries beyond capacity
 //   k_resolve  one workgroup per tile: keys -> LDS (and cleared for the next
-//              frame), queued triangles rasterized in LDS, vis/depth written
+//              frame), queued triangles rasterized in LDS (single pass: a
+//              tile whose count exceeded its capacity first gathers its
+//              remaining entries from the overflow list), vis/depth written
 //   k_shade    per pixel, deferred from the visibility buffer; one instance per
 //              sampling mode; its first block of each job publishes the
 //              frame counters to the host (k_publish for unshaded jobs)
@@ -189,14 +190,15 @@ __device__ __forceinline__ void block_append(uint32_t *part, bool p0, uint32_t *
     s2 = lds[8 + wave] + popc_below(m2);
 }
 
-// Sum of a job's per-setup-block survivor counts, by one workgroup of
-// 64 * NW threads (every thread calls it; red holds NW words).
+// Sum of a[0, n) by one workgroup of 64 * NW threads (every thread calls
+// it; red holds NW words and is reused after the call's second barrier).
 template <int NW>
-__device__ __forceinline__ uint32_t job_alive(const JobDesc &J, uint32_t *red)
+__device__ __forceinline__ uint32_t block_sum(const uint32_t *a, uint32_t n, uint32_t *red)
 {
     uint32_t v = 0;
-    for (uint32_t i = threadIdx.x; i < J.alive_parts; i += 64u * NW) v += ldg(J.alive_part + J.alive_part0 + i);
+    for (uint32_t i = threadIdx.x; i < n; i += 64u * NW) v += ldg(a + i);
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();  // red may still be read by a previous call
     if (lane_id() == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     uint32_t t = 0;
@@ -205,15 +207,25 @@ __device__ __forceinline__ uint32_t job_alive(const JobDesc &J, uint32_t *red)
     return t;
 }
 
-// Writes the job's counters to its renderer's pinned host block, num_alive
-// being the sum of the per-block counts (threads 0 .. words - 1 write).
-__device__ __forceinline__ void publish_counters(const JobDesc &J, uint32_t alive)
+// Writes the job's counters to its renderer's pinned host block (threads
+// 0 .. words - 1 write), after summing what is kept per block or per tile:
+// num_alive from the per-setup-block survivor counts and, for single-pass
+// binning, total_pairs from the final per-tile cursors (the exact counts).
+// Every thread of the workgroup (64 * NW threads) calls it.
+template <int NW>
+__device__ __forceinline__ void publish_counters(const JobDesc &J, uint32_t *red)
 {
     constexpr uint32_t W = sizeof(FrameCounters) / 4;
     static_assert(offsetof(FrameCounters, num_alive) == 0, "num_alive is the first counter word");
-    if (threadIdx.x < W)
-        reinterpret_cast<volatile uint32_t *>(J.host_counters)[threadIdx.x] =
-            threadIdx.x == 0 ? alive : reinterpret_cast<const uint32_t *>(J.counters)[threadIdx.x];
+    constexpr uint32_t kPairs = offsetof(FrameCounters, total_pairs) / 4;
+    const uint32_t alive = block_sum<NW>(J.alive_part + J.alive_part0, J.alive_parts, red);
+    const uint32_t pairs = J.single ? block_sum<NW>(J.tile_count, J.fp.tiles_x * J.fp.tiles_y, red) : 0u;
+    if (threadIdx.x < W) {
+        uint32_t v = reinterpret_cast<const uint32_t *>(J.counters)[threadIdx.x];
+        if (threadIdx.x == 0) v = alive;
+        if (threadIdx.x == kPairs && J.single) v = pairs;
+        reinterpret_cast<volatile uint32_t *>(J.host_counters)[threadIdx.x] = v;
+    }
 }
 
 // counters[t] += 1 for every active lane: one atomic per distinct tile, all
@@ -427,7 +439,7 @@ __global__ __launch_bounds__(256) void k_frame_init(const JobDesc *__restrict__ 
     if (i < n) {
         J.tile_count[i] = 0;
         J.tile_min_depth[i] = 1.0f;
-        if (J.single) J.tile_pool[i] = 0xFFFFFFFFu;
+        if (J.single && !J.learned) J.tile_pool[i] = 0xFFFFFFFFu;
         if (J.density) J.density[i] = 0;
     }
     if (i < sizeof(FrameCounters) / 4) reinterpret_cast<uint32_t *>(J.counters)[i] = 0;
@@ -898,14 +910,18 @@ __global__ __launch_bounds__(256) void k_fill(const JobDesc *__restrict__ jobs, 
     }
 }
 
-// Single-pass fallback, step 1 (per tile): the exact count of a tile is its
-// final fill cursor; the entries beyond its capacity get a pool region.  The
-// block allocates for its 256 tiles with one atomic and adds its totals with
-// one atomic each (no per-wave traffic on the shared counters).
+// Single-pass fallback of the previous-frame policy, step 1 (per tile): the
+// exact count of a tile is its final fill cursor; the entries beyond its
+// capacity get a pool region.  The block allocates for its 256 tiles with one
+// atomic and adds its overflowing tiles with one more (no per-wave traffic
+// on the shared counters).  Learned capacities overflow so rarely that the
+// resolve pass gathers a tile's few entries from the list itself; the
+// heuristic's overflow (about fifty times more entries) is placed by these
+// two passes, which read the list once in all instead of once per tile.
 __global__ __launch_bounds__(256) void k_fallback_alloc(const JobDesc *__restrict__ jobs,
                                                         const uint16_t *__restrict__ table)
 {
-    __shared__ uint32_t wsum[3][4];
+    __shared__ uint32_t wsum[2][4];
     __shared__ uint32_t base;
     uint32_t local;
     const JobDesc &J = jobs[job_of(table, jobs, ST_FBALLOC, local)];
@@ -915,39 +931,31 @@ __global__ __launch_bounds__(256) void k_fallback_alloc(const JobDesc *__restric
     FrameCounters *__restrict__ ctr = J.counters;
     const uint32_t t = local * 256u + threadIdx.x;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    uint32_t c = 0, excess = 0;
+    uint32_t excess = 0;
     if (t < n) {
-        c = count[t];
+        const uint32_t c = count[t];
         excess = c > cap[t] ? c - cap[t] : 0;
     }
-    // inclusive wave prefix of the excess; wave totals of count and overflowed tiles
+    // inclusive wave prefix of the excess; wave total of overflowed tiles
     uint32_t incl = excess;
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(incl, o);
         if (lane >= (uint32_t)o) incl += y;
     }
-    uint32_t tot = c, ovt = excess ? 1u : 0u;
-    for (int o = 32; o > 0; o >>= 1) {
-        tot += __shfl_xor(tot, o);
-        ovt += __shfl_xor(ovt, o);
-    }
+    uint32_t ovt = excess ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) ovt += __shfl_xor(ovt, o);
     if (lane == 63) wsum[0][wave] = incl;
-    if (lane == 0) {
-        wsum[1][wave] = tot;
-        wsum[2][wave] = ovt;
-    }
+    if (lane == 0) wsum[1][wave] = ovt;
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t ex = wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
-        const uint32_t pairs = wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
-        const uint32_t tiles = wsum[2][0] + wsum[2][1] + wsum[2][2] + wsum[2][3];
+        const uint32_t tiles = wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
         uint32_t b0 = 0;
         if (ex) {
             b0 = atomicAdd(&ctr->pool_used, ex);
             if (b0 + ex > J.pool_cap || b0 + ex < b0) ctr->storage_overflow = 1;
         }
         base = b0;
-        if (pairs) atomicAdd(&ctr->total_pairs, pairs);
         if (tiles) atomicAdd(&ctr->overflow_tiles, tiles);
     }
     __syncthreads();
@@ -1182,11 +1190,12 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     const uint32_t *__restrict__ tile_pool = J.tile_pool;
     const uint32_t *__restrict__ queue = J.queue;
     const uint32_t queue_cap = J.queue_cap;
-    const uint32_t *__restrict__ pool = J.pool;
+    uint32_t *__restrict__ pool = J.pool;
     const uint32_t pool_cap = J.pool_cap;
+    const uint2 *__restrict__ ovf = J.ovf;
     const uint32_t *__restrict__ big = J.big;
     const uint32_t big_cap = J.big_cap;
-    const FrameCounters *__restrict__ ctr = J.counters;
+    FrameCounters *__restrict__ ctr = J.counters;
     const uint32_t *__restrict__ trirec = J.trirec;
     unsigned long long *__restrict__ keys = J.keys;
     uint32_t *__restrict__ vis = J.vis;
@@ -1205,6 +1214,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     __shared__ uint32_t wlist[NT];
     __shared__ uint32_t wred_u[NW], wwhole[NW];
     __shared__ float wdmax[NW];
+    __shared__ uint32_t ovf_base, ovf_fill;
 
     const uint32_t ntiles = fp.tiles_x * fp.tiles_y;
     // Tiles in launch order: the hardware spreads consecutive workgroups over
@@ -1234,7 +1244,15 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
         }
     }
 
-    // list A: the tile's queue region; list B: its fallback pool region
+    // List A: the tile's queue region.  List B: for single-pass binning,
+    // the tile's entries beyond its capacity, which setup appended to the
+    // overflow list as (tile, id) pairs.  Learned capacities: the tile's
+    // block gathers them itself into a pool region it allocates (one atomic
+    // per overflowing tile; the list is read four entries per thread at a
+    // time), so no placement pass runs.  Previous-frame capacities: the
+    // fallback passes placed them in the tile's pool region.  The order of a
+    // tile's entries does not change the frame (the depth keys order
+    // fragments by depth, then ID).
     const uint32_t cnt = ldg(tile_count + tile);
     uint32_t n_a = cnt, n_b = 0;
     const uint32_t qoff = ldg(tile_off + tile);
@@ -1242,7 +1260,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     const uint32_t *list_b = pool;
     if (tile_cap) {
         const uint32_t cp = ldg(tile_cap + tile);
-        if (cnt > cp) {
+        if (cnt > cp && !J.learned) {
             n_a = cp;
             const uint32_t po = ldg(tile_pool + tile);
             if (po != 0xFFFFFFFFu) {
@@ -1250,6 +1268,48 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
                 const uint32_t room_b = po < pool_cap ? pool_cap - po : 0;
                 n_b = cnt - cp;
                 n_b = n_b < room_b ? n_b : room_b;
+            }
+        } else if (cnt > cp) {  // the same for the whole block
+            n_a = cp;
+            const uint32_t want = cnt - cp;
+            if (tid == 0) {
+                const uint32_t b = gatomic_add(&ctr->pool_used, want);
+                if (b + want > pool_cap || b + want < b) ctr->storage_overflow = 1;
+                gatomic_add(&ctr->overflow_tiles, 1u);
+                ovf_base = b;
+                ovf_fill = 0;
+            }
+            __syncthreads();
+            const uint32_t pbase = ovf_base;
+            uint32_t ne = ldg(&ctr->ovf_entries);
+            ne = ne < J.ovf_cap ? ne : J.ovf_cap;
+            for (uint32_t b0 = 0; b0 < ne; b0 += 4u * NT) {
+                uint2 e[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t i = b0 + (uint32_t)q * NT + tid;
+                    e[q] = i < ne ? ldg(ovf + i) : make_uint2(0xFFFFFFFFu, 0u);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool mine = e[q].x == tile;
+                    const uint64_t m = __ballot(mine);
+                    if (m) {
+                        const int leader = __builtin_ctzll(m);
+                        uint32_t wb = 0;
+                        if (lane == (uint32_t)leader) wb = atomicAdd(&ovf_fill, (uint32_t)__popcll(m));
+                        wb = (uint32_t)__shfl((int)wb, leader);
+                        const uint32_t pos = pbase + wb + popc_below(m);
+                        if (mine && pos < pool_cap) stg(pool + pos, e[q].y);
+                    }
+                }
+            }
+            __syncthreads();  // the gathered entries are visible to the whole block
+            const uint32_t got = ovf_fill;  // = want unless the overflow list was cut short (storage overflow)
+            if (pbase < pool_cap) {
+                list_b = pool + pbase;
+                const uint32_t room_b = pool_cap - pbase;
+                n_b = got < room_b ? got : room_b;
             }
         }
     }
@@ -1923,7 +1983,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     // this for jobs that are not shaded)
     if (local == 0) {
         __shared__ uint32_t red[4];
-        publish_counters(J, job_alive<4>(J, red));
+        publish_counters<4>(J, red);
     }
     constexpr int P = kShadeRows / 4;
     const uint32_t bw = (fp.width + 63u) / 64u;
@@ -2003,7 +2063,7 @@ __global__ __launch_bounds__(256) void k_publish(const JobDesc *__restrict__ job
     __shared__ uint32_t red[4];
     const JobDesc &J = jobs[blockIdx.x];
     if (J.nblocks[ST_SHADE]) return;
-    publish_counters(J, job_alive<4>(J, red));
+    publish_counters<4>(J, red);
 }
 
 inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }

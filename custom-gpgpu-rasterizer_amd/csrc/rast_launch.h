@@ -40,11 +40,12 @@ hipError_t launch_capacity(const Launch &L, hipStream_t s);  // rast_mlp.hip
 hipError_t launch_scan(const Launch &L, bool capacities, hipStream_t s);
 hipError_t launch_setup(const Launch &L, hipStream_t s);
 hipError_t launch_fill(const Launch &L, hipStream_t s);
-hipError_t launch_fallback(const Launch &L, hipStream_t s);
 // One tile-size group (ST_RES8 / ST_RES16 / ST_RES32); the groups are independent.
 hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s);
 hipError_t launch_shade(const Launch &L, hipStream_t s);
 hipError_t launch_publish(const Launch &L, hipStream_t s);
+// Placement of the previous-frame policy's entries beyond capacity.
+hipError_t launch_fallback(const Launch &L, hipStream_t s);
 // The DrawTexture / DrawTextureGlobal test hooks: a frame holding one texture
 // (vis none, depth 1); mode RAST_DRAW_TEXTURE / RAST_DRAW_TEXTURE_GLOBAL.
 hipError_t launch_draw_texture(uint32_t *vis, float *depth, uint32_t *rgba, uint32_t width, uint32_t height,

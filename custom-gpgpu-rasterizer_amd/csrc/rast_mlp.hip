@@ -3,8 +3,9 @@
 // A per-tile MLP in -> H -> H -> 1 (ReLU, fp32 weights and accumulation)
 // predicts log(1 + queue length) of each screen tile from features available
 // before binning: the previous frame's exact queue lengths (the tile and its
-// 3x3 neighbourhood), the previous frame's min depth, the current frame's
-// projected-vertex density and the tile position.  Capacities
+// 3x3 neighbourhood), the previous frame's min depth and projected-vertex
+// density, and the tile position: all known before this frame's vertex pass,
+// beside which the model runs.  Capacities
 // ceil(expm1(pred) * margin) + slack let binning run in one pass; entries
 // beyond a tile's capacity are re-binned exactly (rast_kernels.hip), so the
 // prediction never changes the frame.  The reference has no counterpart: it
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
     const uint32_t t0 = local * kCapTiles + w * 16u;  // this wave's first tile
     if (lane < 16) {
         float x[F];
-        if (t0 + lane < n) tile_features(fp, t0 + lane, prev, J.prev_depth, J.density, x);
+        if (t0 + lane < n) tile_features(fp, t0 + lane, prev, J.prev_depth, J.prev_density, x);
         else for (int f = 0; f < F; ++f) x[f] = 0.f;
 #pragma unroll
         for (int f = 0; f < F; ++f) xs[w][f][lane] = x[f];
@@ -668,8 +669,8 @@ extern "C" int rast_binmodel_record(rast_binmodel *m, rast_renderer *r)
         m->Y = ny;
         m->cap = ncap;
     }
-    // features of the frame just rendered: previous frame's counts/depth and
-    // this frame's density; target: this frame's exact counts
+    // features of the frame just rendered (the previous frame's counts,
+    // depth and projected-vertex density); target: this frame's exact counts
     FrameParams fp{};
     fp.tiles_x = tx;
     fp.tiles_y = ty;
@@ -677,7 +678,7 @@ extern "C" int rast_binmodel_record(rast_binmodel *m, rast_renderer *r)
     float *xdst = (float *)m->X.p + (size_t)m->n * F;
     float *ydst = (float *)m->Y.p + (size_t)m->n;
     HIP_TRY(launch_tile_features(fp, (const uint32_t *)r->tile_count[before].p, (const float *)r->tile_min_depth[before].p,
-                                 (const uint32_t *)r->density.p, xdst, r->stream));
+                                 (const uint32_t *)r->density[before].p, xdst, r->stream));
     hipLaunchKernelGGL(k_targets, dim3(cdiv(nt, 256)), dim3(256), 0, r->stream, nt,
                        (const uint32_t *)r->tile_count[last].p, ydst);
     HIP_TRY(hipGetLastError());
