@@ -342,7 +342,7 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
     }
     DBuf *bufs[] = { &r->pos4, &r->attr, &r->trirec,
                      &r->alive, &r->survivors, &r->tile_count[0], &r->tile_count[1], &r->tile_off, &r->tile_cap,
-                     &r->tile_pool, &r->tile_cursor, &r->density[0], &r->density[1], &r->tile_min_depth[0], &r->tile_min_depth[1], &r->keys,
+                     &r->tile_pool, &r->tile_cursor, &r->tile_min_depth[0], &r->tile_min_depth[1], &r->keys,
                      &r->queue, &r->pool, &r->big, &r->ovf, &r->counters, &r->vis, &r->depth,
                      &r->rgba };
     for (DBuf *b : bufs) dfree(*b);
@@ -621,7 +621,6 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->tile_cap, tb, &synced));
     HIP_TRY(grow(r, r->tile_pool, tb, &synced));
     HIP_TRY(grow(r, r->tile_cursor, tb, &synced));
-    for (int i = 0; i < 2; ++i) HIP_TRY(grow(r, r->density[i], tb, &synced));
     HIP_TRY(grow(r, r->counters, sizeof(FrameCounters) + 64, &synced));
     if (r->queue_cap == 0) r->queue_cap = std::max<uint32_t>(1u << 20, std::min<uint64_t>(4ull * nt, 0x7FFFFFFFull));
     if (r->pool_cap == 0) r->pool_cap = 1u << 20;
@@ -1006,8 +1005,6 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
             J.alive_part0 = P.tri_block0[me];
             J.alive_parts = P.tri_block0[me + meshes.size() - 1] + ceil_div(meshes.back()->ntri, 256) - J.alive_part0;
         }
-        J.density = r->model ? (uint32_t *)r->density[r->cur].p : nullptr;  // feature of the capacity model
-        J.prev_density = r->model ? (const uint32_t *)r->density[r->cur ^ 1].p : nullptr;
         J.tile_min_depth = (float *)r->tile_min_depth[r->cur].p;
         J.keys = (unsigned long long *)r->keys.p;
         J.vis = (uint32_t *)r->vis.p;

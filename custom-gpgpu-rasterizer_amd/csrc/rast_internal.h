@@ -155,9 +155,6 @@ struct JobDesc {
     // every block serializes those updates at its cache line.
     uint32_t *alive_part;
     uint32_t alive_part0, alive_parts;
-    uint32_t *density;           // per tile projected-vertex count of this frame (capacity model), may be null
-    const uint32_t *prev_density;// the previous frame's (the capacity model's feature: it runs beside the
-                                 // vertex pass, before this frame's counts exist), may be null
     float *tile_min_depth;       // per tile min depth written by the resolve pass
     unsigned long long *keys;    // per pixel (z << 32 | ~(id + 1)), reset to depth 1 / none after each resolve
     uint32_t *vis;

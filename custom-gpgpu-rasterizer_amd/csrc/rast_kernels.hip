@@ -440,7 +440,6 @@ __global__ __launch_bounds__(256) void k_frame_init(const JobDesc *__restrict__ 
         J.tile_count[i] = 0;
         J.tile_min_depth[i] = 1.0f;
         if (J.single && !J.learned) J.tile_pool[i] = 0xFFFFFFFFu;
-        if (J.density) J.density[i] = 0;
     }
     if (i < sizeof(FrameCounters) / 4) reinterpret_cast<uint32_t *>(J.counters)[i] = 0;
 }
@@ -501,10 +500,8 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
     for (uint32_t e = 0; e < gr.y; ++e) {
         const DevMesh &M = meshes[entries[gr.x + e]];
         const JobDesc &J = jobs[M.job];
-        const FrameParams &fp = J.fp;
         float4 *__restrict__ pos4 = J.pos4;
         float4 *__restrict__ attr = J.attr;
-        uint32_t *__restrict__ density = J.density;
         const float *W = M.wvp, *Wd = M.world;
         float o0 = m4(W, 0, 0) * px + m4(W, 0, 1) * py + m4(W, 0, 2) * pz + m4(W, 0, 3);
         float o1 = m4(W, 1, 0) * px + m4(W, 1, 1) * py + m4(W, 1, 2) * pz + m4(W, 1, 3);
@@ -542,34 +539,6 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
         for (int k = 0; k < 4; ++k) {
             const uint32_t w = (uint32_t)k * 256u + tid;
             if (w < nv * 4u) stg(dst + w, rows[at(w >> 2, w & 3u)]);
-        }
-        if (density) {
-            // projected-vertex density feature of the capacity model (every 8th
-            // vertex; no parity role): the block's 32 samples are grouped by tile
-            // in LDS, one atomic per distinct tile
-            __shared__ uint32_t dtile[32];
-            if ((tid & 7u) == 0) {
-                uint32_t t = 0xFFFFFFFFu;
-                const float sx = ((o0 + 1) / 2) * (float)fp.width;
-                const float sy = ((1 - o1) / 2) * (float)fp.height;
-                if (live && o3 > 0.f && sx >= 0.f && sy >= 0.f && sx < (float)fp.width && sy < (float)fp.height)
-                    t = ((uint32_t)sy / fp.tile) * fp.tiles_x + (uint32_t)sx / fp.tile;
-                dtile[tid >> 3] = t;
-            }
-            __syncthreads();
-            if (tid < 32) {
-                const uint32_t mine = dtile[tid];
-                bool first = mine != 0xFFFFFFFFu;
-                uint32_t cnt = 0;
-                for (uint32_t j = 0; j < 32; ++j) {
-                    const uint32_t o = dtile[j];
-                    if (o == mine) {
-                        ++cnt;
-                        if (j < tid) first = false;
-                    }
-                }
-                if (first) gatomic_add(density + mine, cnt);
-            }
         }
     }
 }

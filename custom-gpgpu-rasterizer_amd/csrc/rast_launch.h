@@ -69,6 +69,6 @@ __device__ __forceinline__ uint32_t job_of(const uint16_t *table, const JobDesc 
 
 // Training-set features of a rendered frame (rast_mlp.hip).
 hipError_t launch_tile_features(const FrameParams &fp, const uint32_t *prev_count, const float *prev_depth,
-                                const uint32_t *density, float *features, hipStream_t s);
+                                float *features, hipStream_t s);
 
 }  // namespace rast

@@ -3,9 +3,9 @@
 // A per-tile MLP in -> H -> H -> 1 (ReLU, fp32 weights and accumulation)
 // predicts log(1 + queue length) of each screen tile from features available
 // before binning: the previous frame's exact queue lengths (the tile and its
-// 3x3 neighbourhood), the previous frame's min depth and projected-vertex
-// density, and the tile position: all known before this frame's vertex pass,
-// beside which the model runs.  Capacities
+// 3x3 and 5x5 neighbourhoods), the previous frame's min depth and the tile
+// position: all known before this frame's vertex pass, beside which the
+// model runs.  Capacities
 // ceil(expm1(pred) * margin) + slack let binning run in one pass; entries
 // beyond a tile's capacity are re-binned exactly (rast_kernels.hip), so the
 // prediction never changes the frame.  The reference has no counterpart: it
@@ -50,54 +50,59 @@ __device__ __forceinline__ uint32_t clamp_capacity(float c, uint32_t slack, uint
 // ---------------------------------------------------------------- features
 
 // Per-tile features (inputs of the capacity model), shared by inference and
-// training-set recording.
+// training-set recording, all from the previous frame (the model runs before
+// this frame's vertex pass is done): the tile's exact queue length, the mean
+// and max of its 3x3 and of its 5x5 neighbourhood (the wider ring follows
+// camera motion), its min depth, and the tile position.
 __device__ __forceinline__ void tile_features(const FrameParams &fp, uint32_t t, const uint32_t *__restrict__ prev,
-                                              const float *__restrict__ prev_depth,
-                                              const uint32_t *__restrict__ density, float *o)
+                                              const float *__restrict__ prev_depth, float *o)
 {
     const int tx = (int)(t % fp.tiles_x), ty = (int)(t / fp.tiles_x);
-    // all loads first (clamped to the tile itself outside the grid), so they
-    // are in flight together; then the sums in neighbourhood order
-    uint32_t pc[9], dc[9];
-    bool ok[9];
+    // all loads first (tiles outside the grid count as absent), so they are
+    // in flight together; then the sums in neighbourhood order
+    uint32_t pc[25];
+    bool ok[25];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const int x = tx + k % 3 - 1, y = ty + k / 3 - 1;
+    for (int k = 0; k < 25; ++k) {
+        const int x = tx + k % 5 - 2, y = ty + k / 5 - 2;
         ok[k] = x >= 0 && y >= 0 && x < (int)fp.tiles_x && y < (int)fp.tiles_y;
         const uint32_t q = ok[k] ? (uint32_t)y * fp.tiles_x + (uint32_t)x : t;
         pc[k] = prev[q];
-        dc[k] = density ? density[q] : 0u;
     }
-    float sum = 0.f, mx = 0.f, dsum = 0.f;
-    int cnt = 0;
+    float s3 = 0.f, m3 = 0.f, s5 = 0.f, m5 = 0.f;
+    int n3 = 0, n5 = 0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < 25; ++k) {
         if (!ok[k]) continue;
         const float c = (float)pc[k];
-        sum += c;
-        mx = c > mx ? c : mx;
-        dsum += density ? (float)dc[k] : 0.f;
-        ++cnt;
+        const bool inner = k % 5 >= 1 && k % 5 <= 3 && k / 5 >= 1 && k / 5 <= 3;
+        if (inner) {
+            s3 += c;
+            m3 = c > m3 ? c : m3;
+            ++n3;
+        }
+        s5 += c;
+        m5 = c > m5 ? c : m5;
+        ++n5;
     }
-    o[0] = log1pf((float)prev[t]);
-    o[1] = log1pf(sum / (float)cnt);
-    o[2] = log1pf(mx);
-    o[3] = log1pf(density ? (float)density[t] : 0.f);
-    o[4] = log1pf(dsum / (float)cnt);
+    o[0] = log1pf((float)pc[12]);
+    o[1] = log1pf(s3 / (float)n3);
+    o[2] = log1pf(m3);
+    o[3] = log1pf(s5 / (float)n5);
+    o[4] = log1pf(m5);
     o[5] = prev_depth[t];
     o[6] = ((float)ty + 0.5f) / (float)fp.tiles_y;
     o[7] = ((float)tx + 0.5f) / (float)fp.tiles_x;
 }
 
 __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t *__restrict__ prev,
-                                                  const float *__restrict__ prev_depth,
-                                                  const uint32_t *__restrict__ density, float *__restrict__ out)
+                                                  const float *__restrict__ prev_depth, float *__restrict__ out)
 {
     const uint32_t n = fp.tiles_x * fp.tiles_y;
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= n) return;
     float f[F];
-    tile_features(fp, t, prev, prev_depth, density, f);
+    tile_features(fp, t, prev, prev_depth, f);
     for (int q = 0; q < F; ++q) out[(size_t)t * F + q] = f[q];
 }
 
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__re
     const uint32_t t0 = local * kCapTiles + w * 16u;  // this wave's first tile
     if (lane < 16) {
         float x[F];
-        if (t0 + lane < n) tile_features(fp, t0 + lane, prev, J.prev_depth, J.prev_density, x);
+        if (t0 + lane < n) tile_features(fp, t0 + lane, prev, J.prev_depth, x);
         else for (int f = 0; f < F; ++f) x[f] = 0.f;
 #pragma unroll
         for (int f = 0; f < F; ++f) xs[w][f][lane] = x[f];
@@ -504,10 +509,10 @@ using namespace rast_detail;
 
 namespace rast {
 hipError_t launch_tile_features(const FrameParams &fp, const uint32_t *prev_count, const float *prev_depth,
-                                const uint32_t *density, float *features, hipStream_t s)
+                                float *features, hipStream_t s)
 {
     const uint32_t n = fp.tiles_x * fp.tiles_y;
-    hipLaunchKernelGGL(k_features, dim3(cdiv(n, 256)), dim3(256), 0, s, fp, prev_count, prev_depth, density, features);
+    hipLaunchKernelGGL(k_features, dim3(cdiv(n, 256)), dim3(256), 0, s, fp, prev_count, prev_depth, features);
     return hipGetLastError();
 }
 }  // namespace rast
@@ -669,16 +674,16 @@ extern "C" int rast_binmodel_record(rast_binmodel *m, rast_renderer *r)
         m->Y = ny;
         m->cap = ncap;
     }
-    // features of the frame just rendered (the previous frame's counts,
-    // depth and projected-vertex density); target: this frame's exact counts
+    // features of the frame just rendered (the previous frame's counts and
+    // depths); target: this frame's exact counts
     FrameParams fp{};
     fp.tiles_x = tx;
     fp.tiles_y = ty;
     const int last = r->cur ^ 1, before = r->cur;
     float *xdst = (float *)m->X.p + (size_t)m->n * F;
     float *ydst = (float *)m->Y.p + (size_t)m->n;
-    HIP_TRY(launch_tile_features(fp, (const uint32_t *)r->tile_count[before].p, (const float *)r->tile_min_depth[before].p,
-                                 (const uint32_t *)r->density[before].p, xdst, r->stream));
+    HIP_TRY(launch_tile_features(fp, (const uint32_t *)r->tile_count[before].p, (const float *)r->tile_min_depth[before].p, xdst,
+                                 r->stream));
     hipLaunchKernelGGL(k_targets, dim3(cdiv(nt, 256)), dim3(256), 0, r->stream, nt,
                        (const uint32_t *)r->tile_count[last].p, ydst);
     HIP_TRY(hipGetLastError());

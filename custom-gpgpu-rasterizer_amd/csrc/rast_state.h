@@ -117,7 +117,7 @@ struct rast_renderer {
     uint32_t total_verts = 0, total_tris = 0;
     // frame buffers
     DBuf pos4, attr, trirec, alive, survivors;
-    DBuf tile_count[2], tile_off, tile_cap, tile_pool, tile_cursor, density[2], tile_min_depth[2];
+    DBuf tile_count[2], tile_off, tile_cap, tile_pool, tile_cursor, tile_min_depth[2];
     DBuf queue, pool, big, ovf, counters;
     uint32_t queue_cap = 0, pool_cap = 0, big_cap = 0, ovf_cap = 0;
     DBuf vis, depth, rgba, keys;
