@@ -65,37 +65,45 @@ def parse():
     return ap.parse_args()
 
 
-def time_policy(R, r, cams, opts, policy, warmup, steps, graph=True):
-    """Warm up, then time `steps` frames (graph replay); then the same frames
-    again with per-stage HIP events.  Returns (seconds, per-stage ms, stats)."""
+def time_frames(R, r, cams, opts, policy, warmup, steps, graph=True, start=0):
+    """Warm up, then time exactly `steps` frames of the path (graph replay),
+    bracketed by stream synchronizations: the frames are enqueued back to
+    back (the host does not read them), as a renderer runs.  Returns
+    seconds."""
     flags = R.FLAG_GRAPH if graph else 0
     for i in range(warmup):
-        r.render(cams[i % len(cams)], opts, policy, flags)
+        r.render(cams[(start + i) % len(cams)], opts, policy, flags)
     r.sync()
-    ovf_tiles = 0
-    cap_total = 0
-    pairs = 0
-    rerenders0 = r.stats()["rerenders"]
     t0 = time.perf_counter()
     for i in range(steps):
-        r.render(cams[(warmup + i) % len(cams)], opts, policy, flags)
-        st = r.stats()  # waits for the frame
-        ovf_tiles += st["overflow_tiles"]
-        cap_total += st["capacity_total"]
-        pairs += st["num_bin_pairs"]
+        r.render(cams[(start + warmup + i) % len(cams)], opts, policy, flags)
     r.sync()
-    secs = time.perf_counter() - t0
+    return time.perf_counter() - t0
+
+
+def stage_profile(R, r, cams, opts, policy, warmup, steps):
+    """The same frames once more, one at a time with per-stage HIP events
+    (stages one after the other), read back after every frame: per-stage
+    milliseconds and the binning statistics per frame."""
     keys = ("ms_vertex", "ms_model", "ms_setup", "ms_scan", "ms_fill", "ms_fallback", "ms_raster", "ms_shade",
             "ms_total")
     acc = {k: 0.0 for k in keys}
+    ovf_tiles = cap_total = pairs = 0
+    # the path frame before the first profiled one, so single-pass capacities
+    # come from the path's previous frame as in the timed rounds
+    r.render(cams[(warmup - 1) % len(cams)], opts, policy, R.FLAG_TIMING)
+    rerenders0 = r.stats()["rerenders"]
     for i in range(steps):
         r.render(cams[(warmup + i) % len(cams)], opts, policy, R.FLAG_TIMING)
         st = r.stats()
         for k in keys:
             acc[k] += st[k]
+        ovf_tiles += st["overflow_tiles"]
+        cap_total += st["capacity_total"]
+        pairs += st["num_bin_pairs"]
     extra = {"overflow_tiles_per_frame": ovf_tiles / steps, "capacity_entries_per_frame": cap_total / steps,
              "bin_pairs_per_frame": pairs / steps, "rerenders": r.stats()["rerenders"] - rerenders0}
-    return secs, {k: v / steps for k, v in acc.items()}, extra
+    return {k: v / steps for k, v in acc.items()}, extra
 
 
 def train_model(R, r, w, train_path, args):
@@ -109,9 +117,16 @@ def train_model(R, r, w, train_path, args):
     return model
 
 
+ROUNDS = 3
+
+
 def run_policies(R, r, w, train_path, test_path, policies, args):
     """Trains the capacity model (if the learned policy is asked for) and
-    times every policy on the test path; returns (results, train seconds)."""
+    times every policy on the test path: ROUNDS rounds, each timing every
+    policy in turn over the same `steps` frames (so drift of the device
+    clock or temperature affects all alike); a policy's figure is its median
+    round.  Returns (results, train seconds, the last round's seconds per
+    policy)."""
     results = {}
     model = None
     train_s = 0.0
@@ -120,20 +135,30 @@ def run_policies(R, r, w, train_path, test_path, policies, args):
         model = train_model(R, r, w, train_path, args)
         train_s = time.perf_counter() - t0
     code = {"exact": R.BIN_EXACT, "learned": R.BIN_LEARNED, "prevframe": R.BIN_PREVFRAME}
-    for p in policies:
-        # the projected-vertex density feeding the model is computed only while a model is attached
+
+    def select(p):
         if model is not None:
             if p == "learned":
                 model.attach(r, args.margin, args.slack)
             else:
                 model.detach(r)
-        secs, stages, extra = time_policy(R, r, test_path, w.opts, code[p], args.warmup, args.steps,
-                                          graph=not args.no_graph)
-        results[p] = {"ms_per_frame": secs * 1e3 / args.steps, "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-                      **extra}
+
+    rounds = {p: [] for p in policies}
+    for _ in range(ROUNDS):
+        for p in policies:
+            select(p)
+            rounds[p].append(time_frames(R, r, test_path, w.opts, code[p], args.warmup, args.steps,
+                                         graph=not args.no_graph))
+    for p in policies:
+        select(p)
+        stages, extra = stage_profile(R, r, test_path, w.opts, code[p], args.warmup, args.steps)
+        secs = sorted(rounds[p])[len(rounds[p]) // 2]
+        results[p] = {"ms_per_frame": secs * 1e3 / args.steps,
+                      "rounds_ms_per_frame": [round(t * 1e3 / args.steps, 4) for t in rounds[p]],
+                      "stages_ms": {k: round(v, 4) for k, v in stages.items()}, **extra}
     if model is not None:
         model.close()
-    return results, train_s
+    return results, train_s, {p: rounds[p][-1] for p in policies}
 
 
 def batch_workload(R, S, autotune, args):
@@ -214,18 +239,24 @@ def flythrough_workload(R, S, autotune, args):
 
     for f in range(3):
         R.render_batch(jobs(learned, f, R.BIN_LEARNED))
-    ovf = tiles = 0
+    for v in learned:
+        v.sync()
     t0 = time.perf_counter()
+    for f in range(len(fly.path)):  # frames back to back, as a renderer runs
+        R.render_batch(jobs(learned, f, R.BIN_LEARNED))
+    for v in learned:
+        v.sync()
+    secs = time.perf_counter() - t0
+    # second pass over the path: every frame's statistics, and every 10th
+    # frame against the exact binner
+    ovf = tiles = 0
+    checked = mismatched = 0
     for f in range(len(fly.path)):
         R.render_batch(jobs(learned, f, R.BIN_LEARNED))
         for v in learned:
             st = v.stats()
             ovf += st["overflow_tiles"]
             tiles += st["num_tiles"]
-    secs = time.perf_counter() - t0
-    checked = mismatched = 0
-    for f in range(len(fly.path)):
-        R.render_batch(jobs(learned, f, R.BIN_LEARNED))
         if f % 10 == 0:
             R.render_batch(jobs(exact, f, R.BIN_EXACT))
             for a, b in zip(learned, exact):
@@ -365,7 +396,7 @@ def main():
     tuned.apply(r)
     r.add_scene(w.scene)
     policies = [p for p in args.policies.split(",") if p]
-    results, train_s = run_policies(R, r, w, train_path, test_path, policies, args)
+    results, train_s, _ = run_policies(R, r, w, train_path, test_path, policies, args)
     r.close()
 
     plain = None
@@ -375,7 +406,7 @@ def main():
         r2 = R.Renderer(w.width, w.height)
         pp.apply(r2)
         r2.add_scene(w.scene)
-        res2, _ = run_policies(R, r2, w, train_path, test_path, ["exact", "learned"], args)
+        res2, _, _ = run_policies(R, r2, w, train_path, test_path, ["exact", "learned"], args)
         r2.close()
         plain = {"params": vars(pp), "exact_ms_per_frame": round(res2["exact"]["ms_per_frame"], 4),
                  "learned_ms_per_frame": round(res2["learned"]["ms_per_frame"], 4),
@@ -420,7 +451,8 @@ def main():
         "data": "synthetic (seeded procedural terrain, textures and camera path)",
         "config": {"workload": w.name, "triangles": ntri, "width": w.width, "height": w.height,
                    "binning": head, "params": vars(tuned), "camera_path_frames": 100,
-                   "train_frames": args.train_frames, "graph_replay": not args.no_graph},
+                   "train_frames": args.train_frames, "graph_replay": not args.no_graph,
+                   "timing": f"median of {ROUNDS} rounds of {args.steps} back-to-back frames, policies interleaved"},
         "gpix_per_s": round(P / (ms * 1e-3) / 1e9, 3),
         "policies": {k: {"mtris_per_s": round(ntri / (v["ms_per_frame"] * 1e-3) / 1e6, 2), **v}
                      for k, v in results.items()},

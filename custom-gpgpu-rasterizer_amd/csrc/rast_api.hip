@@ -746,8 +746,10 @@ static uint64_t plan_key(rast_renderer *const *rs, uint32_t n, int hidden)
 
 void plan_free(LaunchPlan &p)
 {
-    if (p.exec) (void)hipGraphExecDestroy(p.exec);
-    p.exec = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (p.exec[i]) (void)hipGraphExecDestroy(p.exec[i]);
+        p.exec[i] = nullptr;
+    }
     for (int i = 0; i < 2; ++i) {
         if (p.side[i]) (void)hipStreamDestroy(p.side[i]);
         if (p.joined[i]) (void)hipEventDestroy(p.joined[i]);
@@ -759,11 +761,13 @@ void plan_free(LaunchPlan &p)
     if (p.cap_fork) (void)hipEventDestroy(p.cap_fork);
     if (p.cap_join) (void)hipEventDestroy(p.cap_join);
     p.cap_fork = p.cap_join = nullptr;
-    if (p.done) (void)hipEventDestroy(p.done);
-    p.done = nullptr;
-    p.done_pending = false;
-    if (p.h_stage) (void)hipHostFree(p.h_stage);
-    p.h_stage = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (p.done[i]) (void)hipEventDestroy(p.done[i]);
+        p.done[i] = nullptr;
+        p.done_pending[i] = false;
+        if (p.h_stage[i]) (void)hipHostFree(p.h_stage[i]);
+        p.h_stage[i] = nullptr;
+    }
     dfree(p.d_block_mesh);
     dfree(p.d_alive_part);
     dfree(p.d_tables);
@@ -772,14 +776,21 @@ void plan_free(LaunchPlan &p)
     p.key = 0;
 }
 
-// The plan's previous launch must be done before its staging or tables change.
-static int plan_wait(LaunchPlan &P)
+// Waits for the last launch that used staging block k.
+static int plan_wait_block(LaunchPlan &P, int k)
 {
-    if (P.done_pending) {
-        HIP_TRY(hipEventSynchronize(P.done));
-        P.done_pending = false;
+    if (P.done_pending[k]) {
+        HIP_TRY(hipEventSynchronize(P.done[k]));
+        P.done_pending[k] = false;
     }
     return RAST_OK;
+}
+
+// Every launch of the plan must be done before its tables change.
+static int plan_wait(LaunchPlan &P)
+{
+    RAST_TRY(plan_wait_block(P, 0));
+    return plan_wait_block(P, 1);
 }
 
 static int dev_alloc(DBuf &b, size_t bytes)
@@ -795,10 +806,12 @@ static int dev_alloc(DBuf &b, size_t bytes)
 static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int hidden, uint64_t key)
 {
     RAST_TRY(plan_wait(P));
-    if (P.exec) (void)hipGraphExecDestroy(P.exec);
-    P.exec = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (P.exec[i]) (void)hipGraphExecDestroy(P.exec[i]);
+        P.exec[i] = nullptr;
+        if (!P.done[i]) HIP_TRY(hipEventCreateWithFlags(&P.done[i], hipEventDisableTiming));
+    }
     P.built = false;
-    if (!P.done) HIP_TRY(hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
     if (!P.fork) {
         HIP_TRY(hipEventCreateWithFlags(&P.fork, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&P.cap_fork, hipEventDisableTiming));
@@ -919,10 +932,12 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     }
     // staging block: JobDesc[n] then DevMesh[meshes]
     const size_t bytes = (size_t)n * sizeof(JobDesc) + (size_t)P.nmeshes * sizeof(DevMesh);
-    if (bytes > P.stage_bytes || !P.h_stage) {
-        if (P.h_stage) (void)hipHostFree(P.h_stage);
-        P.h_stage = nullptr;
-        HIP_TRY(hipHostMalloc(&P.h_stage, bytes));
+    if (bytes > P.stage_bytes || !P.h_stage[0] || !P.h_stage[1]) {
+        for (int i = 0; i < 2; ++i) {
+            if (P.h_stage[i]) (void)hipHostFree(P.h_stage[i]);
+            P.h_stage[i] = nullptr;
+            HIP_TRY(hipHostMalloc(&P.h_stage[i], bytes));
+        }
     }
     RAST_TRY(dev_alloc(P.d_stage, bytes));
     P.stage_bytes = bytes;
@@ -934,12 +949,14 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
 }
 
 // Writes every job's descriptor and mesh entries (this frame's cameras,
-// options and buffer parities) into the plan's pinned staging block.
+// options and buffer parities) into the staging block of the next launch,
+// once the launch before last, which used it, is done.
 static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
 {
-    RAST_TRY(plan_wait(P));
-    JobDesc *jd = (JobDesc *)P.h_stage;
-    DevMesh *dm = (DevMesh *)((char *)P.h_stage + (size_t)n * sizeof(JobDesc));
+    RAST_TRY(plan_wait_block(P, P.parity));
+    void *h = P.h_stage[P.parity];
+    JobDesc *jd = (JobDesc *)h;
+    DevMesh *dm = (DevMesh *)((char *)h + (size_t)n * sizeof(JobDesc));
     uint32_t me = 0;
     for (uint32_t j = 0; j < n; ++j) {
         rast_renderer *r = rs[j];
@@ -1069,7 +1086,7 @@ static int mark(rast_renderer *r, int which, hipStream_t s)
 static int enqueue_launch(LaunchPlan &P, hipStream_t s, rast_renderer *timed)
 {
     const Launch &L = P.L;
-    HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage, P.stage_bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage[P.parity], P.stage_bytes, hipMemcpyHostToDevice, s));
     RAST_TRY(mark(timed, EV_START, s));
     HIP_TRY(launch_frame_init(L, s));
     if (L.cap_jobs && !timed) {
@@ -1132,9 +1149,10 @@ static int capture_launch(LaunchPlan &P, hipStream_t s)
         return rc;
     }
     HIP_TRY(e);
-    const hipError_t ie = hipGraphInstantiate(&P.exec, cap, nullptr, nullptr, 0);
+    hipGraphExec_t &ex = P.exec[P.parity];
+    const hipError_t ie = hipGraphInstantiate(&ex, cap, nullptr, nullptr, 0);
     (void)hipGraphDestroy(cap);
-    if (ie != hipSuccess) P.exec = nullptr;
+    if (ie != hipSuccess) ex = nullptr;
     HIP_TRY(ie);
     return RAST_OK;
 }
@@ -1142,14 +1160,16 @@ static int capture_launch(LaunchPlan &P, hipStream_t s)
 // Runs a built and staged plan on s (replaying its graph in graph mode).
 static int run_plan(LaunchPlan &P, hipStream_t s, bool graph, rast_renderer *timed)
 {
+    const int k = P.parity;
     if (graph) {
-        if (!P.exec) RAST_TRY(capture_launch(P, s));
-        HIP_TRY(hipGraphLaunch(P.exec, s));
+        if (!P.exec[k]) RAST_TRY(capture_launch(P, s));
+        HIP_TRY(hipGraphLaunch(P.exec[k], s));
     } else {
         RAST_TRY(enqueue_launch(P, s, timed));
     }
-    HIP_TRY(hipEventRecord(P.done, s));
-    P.done_pending = true;
+    HIP_TRY(hipEventRecord(P.done[k], s));
+    P.done_pending[k] = true;
+    P.parity = k ^ 1;
     return RAST_OK;
 }
 
@@ -1243,7 +1263,10 @@ extern "C" int rast_render(rast_renderer *r, const rast_camera *cam, const rast_
     if (!r || !cam || !opts) return rast_set_error(RAST_E_INVALID_ARG, "rast_render: null argument");
     RAST_TRY(validate_opts(opts));
     RAST_TRY(validate_camera(cam));
-    if (r->frame_pending) RAST_TRY(check_frame(r));
+    // A frame nobody read is not checked: the next one replaces it and takes
+    // only capacity hints from it (its per-tile counts are exact even where
+    // its queue storage overflowed), so the host does not wait for it.
+    // Every frame that is read is checked first (rast_sync).
     r->last_cam = *cam;
     r->last_opts = *opts;
     r->prev_valid_before = r->have_prev;
@@ -1459,8 +1482,7 @@ extern "C" int rast_render_batch(const rast_job *jobs, uint32_t n)
     // host part of every job; each renderer's previous frame is checked first
     for (uint32_t i = 0; i < n; ++i) {
         rast_renderer *t = rs[i];
-        if (t->frame_pending) RAST_TRY(check_frame(t));
-        t->last_cam = jobs[i].camera;
+        t->last_cam = jobs[i].camera;  // an unread earlier frame is replaced unchecked, as in rast_render
         t->last_opts = jobs[i].opts;
         t->last_opts.flags &= ~RAST_FLAG_TIMING;
         t->prev_valid_before = t->have_prev;

@@ -54,11 +54,16 @@ struct LaunchPlan {
     std::vector<uint32_t> vs_block0, tri_block0;  // per mesh entry
     DBuf d_block_mesh, d_tables, d_stage;
     DBuf d_alive_part;  // culling survivors per setup block
-    void *h_stage = nullptr;  // pinned: JobDesc[njobs] then DevMesh[nmeshes]
+    // Two pinned staging blocks (JobDesc[njobs] then DevMesh[nmeshes]) used in
+    // turn, so the next launch is staged while the previous one still reads
+    // its block: the host never waits for the frame before last.  One
+    // captured graph per block (a graph's upload reads a fixed address).
+    void *h_stage[2] = { nullptr, nullptr };
     size_t stage_bytes = 0;
-    hipGraphExec_t exec = nullptr;  // captured launch (graph mode)
-    hipEvent_t done = nullptr;      // end of the plan's last launch (staging reusable)
-    bool done_pending = false;
+    hipGraphExec_t exec[2] = { nullptr, nullptr };  // captured launches (graph mode)
+    hipEvent_t done[2] = { nullptr, nullptr };      // end of the last launch that used each block
+    bool done_pending[2] = { false, false };
+    int parity = 0;                                 // the block the next launch uses
     // side streams running the resolve passes of different tile sizes concurrently
     hipStream_t side[2] = { nullptr, nullptr };
     hipEvent_t fork = nullptr, joined[2] = { nullptr, nullptr };

@@ -163,3 +163,40 @@ def test_batch_rejects_shared_targets():
         r.add_scene(random_triangle_scene(1, 10))
         with pytest.raises(R.RastError):
             R.render_batch([(r, screen_camera(), S.Opts(), R.BIN_EXACT, 0)] * 2)
+
+
+def test_back_to_back_launches_keep_their_own_staging():
+    """Frames and batches are enqueued without waiting for the previous one:
+    each launch stages its cameras in one of two pinned blocks in turn.  Two
+    sets of renderers with the same batch shape share one launch plan; three
+    batches in a row (sets A, B, then A again) must each render their own
+    cameras, and a renderer's unread frames are simply replaced."""
+    scene = random_triangle_scene(41, 200)
+    W, H = 96, 64
+    cams = [S.Camera(position=(0.03 * k, -0.02 * k, 0.05 * k), forward=(0.0, 0.0, 1.0), fov_deg=90.0,
+                     near_z=0.1, far_z=100.0) for k in range(6)]
+    opts = S.Opts(cull_mode=S.CULL_NONE)
+    A = [R.Renderer(W, H) for _ in range(2)]
+    B = [R.Renderer(W, H) for _ in range(2)]
+    try:
+        for r in A + B:
+            r.add_scene(scene)
+        R.render_batch([(A[0], cams[0], opts, R.BIN_EXACT, 0), (A[1], cams[1], opts, R.BIN_EXACT, 0)])
+        R.render_batch([(B[0], cams[2], opts, R.BIN_EXACT, 0), (B[1], cams[3], opts, R.BIN_EXACT, 0)])
+        R.render_batch([(A[0], cams[4], opts, R.BIN_EXACT, 0), (A[1], cams[5], opts, R.BIN_EXACT, 0)])
+        for r, cam in zip(A + B, (cams[4], cams[5], cams[2], cams[3])):
+            vis, depth, rgba = r.readback()
+            ovis, odepth, orgba, _ = O.render(scene, cam, opts, W, H, threads=4)
+            np.testing.assert_array_equal(vis, ovis)
+            np.testing.assert_array_equal(depth.view(np.uint32), odepth.view(np.uint32))
+        # single frames: five in a row (both staging blocks twice), only the last read
+        r = A[0]
+        for k in range(5):
+            r.render(cams[k], opts, R.BIN_EXACT, R.FLAG_GRAPH if k % 2 else 0)
+        vis, depth, _ = r.readback()
+        ovis, odepth, _, _ = O.render(scene, cams[4], opts, W, H, threads=4, want_rgba=False)
+        np.testing.assert_array_equal(vis, ovis)
+        np.testing.assert_array_equal(depth.view(np.uint32), odepth.view(np.uint32))
+    finally:
+        for r in A + B:
+            r.close()

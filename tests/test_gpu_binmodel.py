@@ -143,8 +143,9 @@ def test_frame_capacities_match_the_oracle_forward(hidden):
         cam = S.train_test_paths(2, 1)[1][0]
         r.render(cam, w.opts, R.BIN_EXACT)
         # the first single-pass frame may find its capacities cut to the queue
-        # storage (and the storage grown for the next frame); the second has room
+        # storage; reading it grows the storage, and the second has room
         r.render(cam, w.opts, R.BIN_LEARNED)
+        r.sync()
         r.render(cam, w.opts, R.BIN_LEARNED)
         caps = r.debug_read(R.DEBUG_TILE_CAPS, np.uint32)
         m.close()
