@@ -227,7 +227,7 @@ extern "C" int rast_camera_frame(const rast_camera *cam, uint32_t width, uint32_
 }
 
 // Timing events of one frame, recorded between the stages when RAST_FLAG_TIMING is set.
-enum { EV_START, EV_VERTEX, EV_MODEL, EV_SCANCAP, EV_SETUP, EV_SCANCNT, EV_FILL, EV_FALLBACK, EV_RASTER, EV_SHADE, EV_COUNT };
+enum { EV_START, EV_VERTEX, EV_MODEL, EV_SETUP, EV_SCANCNT, EV_FILL, EV_FALLBACK, EV_RASTER, EV_SHADE, EV_COUNT };
 // the start / stop events of rast_timer_start / rast_timer_stop
 enum { EV_TIMER_START = EV_COUNT, EV_TIMER_STOP, EV_ALL };
 static_assert(EV_ALL <= (int)(sizeof(((rast_renderer *)nullptr)->ev) / sizeof(hipEvent_t)), "renderer event slots");
@@ -758,9 +758,6 @@ void plan_free(LaunchPlan &p)
     }
     if (p.fork) (void)hipEventDestroy(p.fork);
     p.fork = nullptr;
-    if (p.cap_fork) (void)hipEventDestroy(p.cap_fork);
-    if (p.cap_join) (void)hipEventDestroy(p.cap_join);
-    p.cap_fork = p.cap_join = nullptr;
     for (int i = 0; i < 2; ++i) {
         if (p.done[i]) (void)hipEventDestroy(p.done[i]);
         p.done[i] = nullptr;
@@ -814,8 +811,6 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     P.built = false;
     if (!P.fork) {
         HIP_TRY(hipEventCreateWithFlags(&P.fork, hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&P.cap_fork, hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&P.cap_join, hipEventDisableTiming));
         for (int i = 0; i < 2; ++i) {
             HIP_TRY(hipStreamCreateWithFlags(&P.side[i], hipStreamNonBlocking));
             HIP_TRY(hipEventCreateWithFlags(&P.joined[i], hipEventDisableTiming));
@@ -1089,28 +1084,19 @@ static int enqueue_launch(LaunchPlan &P, hipStream_t s, rast_renderer *timed)
     HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage[P.parity], P.stage_bytes, hipMemcpyHostToDevice, s));
     RAST_TRY(mark(timed, EV_START, s));
     HIP_TRY(launch_frame_init(L, s));
-    if (L.cap_jobs && !timed) {
-        // the capacities read only the previous frame's counts, depths and
-        // densities: the model and the capacity scan run on a side stream
-        // beside the vertex pass (joined before setup, which needs both)
-        HIP_TRY(hipEventRecord(P.cap_fork, s));
-        HIP_TRY(hipStreamWaitEvent(P.side[0], P.cap_fork, 0));
-        HIP_TRY(launch_capacity(L, P.side[0]));
-        HIP_TRY(launch_scan(L, true, P.side[0]));
-        HIP_TRY(hipEventRecord(P.cap_join, P.side[0]));
-        HIP_TRY(launch_vertex(L, s));
-        HIP_TRY(hipStreamWaitEvent(s, P.cap_join, 0));
+    if (!timed) {
+        // the capacities read only the previous frame's counts and depths:
+        // their workgroups run in the vertex launch, beside the transform
+        HIP_TRY(launch_vertex(L, 1, s));
     } else {  // one stage after the other (per-stage timing)
-        HIP_TRY(launch_vertex(L, s));
+        HIP_TRY(launch_vertex(L, 0, s));
         RAST_TRY(mark(timed, EV_VERTEX, s));
-        HIP_TRY(launch_capacity(L, s));
+        HIP_TRY(launch_vertex(L, 2, s));
         RAST_TRY(mark(timed, EV_MODEL, s));
-        HIP_TRY(launch_scan(L, true, s));
-        RAST_TRY(mark(timed, EV_SCANCAP, s));
     }
     HIP_TRY(launch_setup(L, s));
     RAST_TRY(mark(timed, EV_SETUP, s));
-    HIP_TRY(launch_scan(L, false, s));
+    HIP_TRY(launch_scan(L, s));
     RAST_TRY(mark(timed, EV_SCANCNT, s));
     HIP_TRY(launch_fill(L, s));
     RAST_TRY(mark(timed, EV_FILL, s));
@@ -1316,9 +1302,9 @@ extern "C" int rast_get_stats(rast_renderer *r, rast_stats *out)
     s.capacity_total = r->last_single ? c.capacity_total : 0;
     s.rerenders = r->rerenders;
     s.ms_vertex = elapsed(r, EV_START, EV_VERTEX);
-    s.ms_model = elapsed(r, EV_VERTEX, EV_MODEL);
-    s.ms_scan = elapsed(r, EV_MODEL, EV_SCANCAP) + elapsed(r, EV_SETUP, EV_SCANCNT);
-    s.ms_setup = elapsed(r, EV_SCANCAP, EV_SETUP);
+    s.ms_model = elapsed(r, EV_VERTEX, EV_MODEL);  // capacities and queue regions (single pass)
+    s.ms_scan = elapsed(r, EV_SETUP, EV_SCANCNT);
+    s.ms_setup = elapsed(r, EV_MODEL, EV_SETUP);
     s.ms_count = s.ms_setup;
     s.ms_fill = elapsed(r, EV_SCANCNT, EV_FILL);
     s.ms_fallback = elapsed(r, EV_FILL, EV_FALLBACK);  // only the previous-frame policy has these passes

@@ -36,8 +36,8 @@ constexpr int kIVertexFloats = 14;
 constexpr int kTriRecWords = 16;
 constexpr int kMaxMeshes = 1 << 16;
 constexpr int kNumFeatures = 8;
-// Tiles per workgroup of the capacity-model kernel (16 per wave).
-constexpr int kCapacityTilesPerBlock = 64;
+// Tiles per capacity workgroup (one per lane of its four waves).
+constexpr int kCapacityTilesPerBlock = 256;
 constexpr int kShadeRows = 8;  // pixel rows per shading block (64 wide; a multiple of 4)
 
 struct DevTexture {
@@ -89,7 +89,7 @@ struct FrameParams {
 // stage s serves job table_s[b], as its block b - JobDesc::block0[s].
 enum Stage : int {
     ST_INIT,      // per-tile reset (256 tiles per block)
-    ST_CAP,       // single-pass capacities (64 tiles per block)
+    ST_CAP,       // single-pass capacities and queue regions (256 tiles per block)
     ST_FILL,      // exact second pass (grid-stride over binned triangles)
     ST_FBALLOC,   // previous-frame fallback allocation (256 tiles per block)
     ST_FBFILL,    // previous-frame fallback placement (grid-stride over entries)
@@ -112,6 +112,8 @@ struct FrameCounters {
     uint32_t storage_overflow;// queue / pool / big list storage exhausted -> frame invalid
     uint32_t capacity_total;  // single pass: capacities as used (their sum fits the queue storage)
     uint32_t capacity_wanted; // single pass: the capacities' sum before that bound (saturated)
+    unsigned long long capacity_top;  // single pass: queue allocation top (the capacities' full sum);
+                                      // the two fields above are derived from it when published
 };
 
 // One frame of a launch: its parameters and the device buffers of the

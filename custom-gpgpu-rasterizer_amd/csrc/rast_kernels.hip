@@ -14,14 +14,15 @@
 // CCR/ERenderer.cpp:544-554 applied in ascending triangle order.
 //
 // Pipeline (one launch per stage for all jobs and meshes of a launch):
-//   k_vertex   per vertex: clip/NDC position + shading attributes
+//   k_vertex   per vertex: clip/NDC position + shading attributes; beside
+//              it, single-pass jobs' tile capacities and queue regions
 //   k_setup    per triangle: assembly, cull, screen setup record; triangles
 //              whose bounding box has at most micro_limit pixels are
 //              rasterized on the spot with 64-bit atomic min into the key
 //              buffer; larger ones are counted per tile (exact two-pass) or
 //              written straight into per-tile queues (single pass); triangles
 //              spanning more than big_threshold tiles go to one screen list
-//   k_scan     per-tile exclusive scan of counts / capacities
+//   k_scan     per-tile exclusive scan of the exact counts
 //   k_fill     exact second pass over the binned triangles
 //   k_resolve  one workgroup per tile: keys -> LDS (and cleared for the next
 //              frame), queued triangles rasterized in LDS (single pass: a
@@ -32,6 +33,7 @@
 //              frame counters to the host (k_publish for unshaded jobs)
 #include <hip/hip_runtime.h>
 
+#include "rast_capacity.h"
 #include "rast_launch.h"
 
 namespace rast {
@@ -218,12 +220,20 @@ __device__ __forceinline__ void publish_counters(const JobDesc &J, uint32_t *red
     constexpr uint32_t W = sizeof(FrameCounters) / 4;
     static_assert(offsetof(FrameCounters, num_alive) == 0, "num_alive is the first counter word");
     constexpr uint32_t kPairs = offsetof(FrameCounters, total_pairs) / 4;
+    constexpr uint32_t kCapTotal = offsetof(FrameCounters, capacity_total) / 4;
+    constexpr uint32_t kCapWanted = offsetof(FrameCounters, capacity_wanted) / 4;
     const uint32_t alive = block_sum<NW>(J.alive_part + J.alive_part0, J.alive_parts, red);
     const uint32_t pairs = J.single ? block_sum<NW>(J.tile_count, J.fp.tiles_x * J.fp.tiles_y, red) : 0u;
     if (threadIdx.x < W) {
         uint32_t v = reinterpret_cast<const uint32_t *>(J.counters)[threadIdx.x];
         if (threadIdx.x == 0) v = alive;
-        if (threadIdx.x == kPairs && J.single) v = pairs;
+        if (J.single) {
+            // the queue regions as placed (at most the storage) and as wanted
+            const unsigned long long top = J.counters->capacity_top;
+            if (threadIdx.x == kPairs) v = pairs;
+            if (threadIdx.x == kCapTotal) v = (uint32_t)(top < J.queue_cap ? top : J.queue_cap);
+            if (threadIdx.x == kCapWanted) v = (uint32_t)(top < 0xFFFFFFFFull ? top : 0xFFFFFFFFull);
+        }
         reinterpret_cast<volatile uint32_t *>(J.host_counters)[threadIdx.x] = v;
     }
 }
@@ -467,19 +477,18 @@ __global__ __launch_bounds__(256) void k_clear_keys(unsigned long long *__restri
 // per instruction, a quarter of each).  Quad q of LDS row r sits at slot
 // q ^ ((r >> 2) & 3) of the row: both the row accesses (lane r takes row r)
 // and the consecutive ones (lane w takes quad w) then hit distinct banks.
-__global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ meshes,
-                                                const uint32_t *__restrict__ block_group,
-                                                const uint2 *__restrict__ groups,
-                                                const uint32_t *__restrict__ entries,
-                                                const JobDesc *__restrict__ jobs)
+__device__ __forceinline__ void vertex_block(const DevMesh *__restrict__ meshes,
+                                             const uint32_t *__restrict__ block_group,
+                                             const uint2 *__restrict__ groups,
+                                             const uint32_t *__restrict__ entries,
+                                             const JobDesc *__restrict__ jobs, uint32_t blk, float4 *rows)
 {
-    __shared__ float4 rows[256 * 4];
     auto at = [](uint32_t r, uint32_t q) { return r * 4u + (q ^ ((r >> 2) & 3u)); };
     // a group is one source mesh drawn by one or more jobs (views of one
     // scene): the vertex is read once and transformed for each of them
-    const uint2 gr = groups[block_group[blockIdx.x]];
+    const uint2 gr = groups[block_group[blk]];
     const DevMesh &M0 = meshes[entries[gr.x]];
-    const uint32_t vbase = (blockIdx.x - M0.vs_block0) * 256u;
+    const uint32_t vbase = (blk - M0.vs_block0) * 256u;
     const uint32_t nv = M0.num_verts - vbase < 256u ? M0.num_verts - vbase : 256u;  // vertices of this block
     const uint32_t tid = threadIdx.x;
     const float4 *src = reinterpret_cast<const float4 *>(M0.ivert) + (size_t)vbase * 4;
@@ -541,6 +550,33 @@ __global__ __launch_bounds__(256) void k_vertex(const DevMesh *__restrict__ mesh
             if (w < nv * 4u) stg(dst + w, rows[at(w >> 2, w & 3u)]);
         }
     }
+}
+
+// The vertex launch.  With single-pass jobs (H > 0) its first cap_blocks
+// workgroups compute their tiles' capacities and queue regions instead
+// (rast_capacity.h): those read only the previous frame's counts and depths,
+// so they run beside the vertex transform in the same launch; the LDS is one
+// array sized for the larger of the two uses (a capacity model of hidden
+// width 64 stages 19 KB, which keeps 8 workgroups per CU, as the vertex
+// pass's 16 KB does).  A launch of cap_blocks workgroups alone runs the
+// capacity stage by itself (per-stage timing).
+template <int H>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_vertex(const DevMesh *__restrict__ meshes,
+                                                const uint32_t *__restrict__ block_group,
+                                                const uint2 *__restrict__ groups,
+                                                const uint32_t *__restrict__ entries,
+                                                const JobDesc *__restrict__ jobs,
+                                                const uint16_t *__restrict__ cap_table, uint32_t cap_blocks)
+{
+    constexpr int kVertexF4 = 256 * 4, kCapF4 = (capacity_lds_floats<H>() + 3) / 4;
+    __shared__ float4 lds[kVertexF4 > kCapF4 ? kVertexF4 : kCapF4];
+    if constexpr (H != 0) {
+        if (blockIdx.x < cap_blocks) {
+            capacity_block<H>(jobs, cap_table, reinterpret_cast<float *>(lds));
+            return;
+        }
+    }
+    vertex_block(meshes, block_group, groups, entries, jobs, blockIdx.x - cap_blocks, lds);
 }
 
 // TA_Kernel + TB_Kernel (CCR/CUDARenderer.cu:702-796,903-988) on the CPU
@@ -768,24 +804,19 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     }
 }
 
-// Exclusive scan of a job's per-tile values by one 1024-thread workgroup:
-// out_off[n] = total.  is_capacity: in = capacities, the total is the
-// capacity sum; otherwise in = exact counts and the total is the pair count.
-// Sums are formed in 64 bits.  Capacities are hints: a tile whose region
-// would end past the queue storage keeps only the room that is left (its
-// capacity is lowered in place, the rest of its entries go to the exact
-// fallback), so capacities never overflow the storage; the host grows the
-// storage for the next frame from the sum that was wanted.  Exact counts beyond
-// the storage set storage_overflow (the frame is rendered again with a larger
-// queue); the offsets of such tiles are clamped to the storage end.
-__global__ __launch_bounds__(1024) void k_scan(const JobDesc *__restrict__ jobs, int is_capacity)
+// Exclusive scan of a job's exact per-tile counts by one 1024-thread
+// workgroup (two-pass jobs): tile_off[n] = the pair count.  Sums are formed
+// in 64 bits.  Counts beyond the queue storage set storage_overflow (the
+// frame is rendered again with a larger queue); the offsets of such tiles
+// are clamped to the storage end.  The fill cursors are zeroed on the way.
+__global__ __launch_bounds__(1024) void k_scan(const JobDesc *__restrict__ jobs)
 {
     const JobDesc &J = jobs[blockIdx.x];  // one workgroup per job
-    if ((J.single != 0) != (is_capacity != 0)) return;
+    if (J.single) return;
     const uint32_t n = J.fp.tiles_x * J.fp.tiles_y;
-    uint32_t *__restrict__ in = is_capacity ? J.tile_cap : J.tile_count;
+    const uint32_t *__restrict__ in = J.tile_count;
     uint32_t *__restrict__ out_off = J.tile_off;
-    uint32_t *__restrict__ zero_cursor = is_capacity ? nullptr : J.tile_cursor;
+    uint32_t *__restrict__ zero_cursor = J.tile_cursor;
     const uint64_t limit = J.queue_cap;
     FrameCounters *__restrict__ ctr = J.counters;
     __shared__ uint64_t wsum[16];
@@ -818,19 +849,10 @@ __global__ __launch_bounds__(1024) void k_scan(const JobDesc *__restrict__ jobs,
         for (uint32_t q = 0; q < 4; ++q) {
             if (i + q >= n) break;
             out_off[i + q] = (uint32_t)(run < limit ? run : limit);
-            if (is_capacity) {
-                // the region [run, run + cap) must end inside the storage
-                const uint64_t room = run < limit ? limit - run : 0ull;
-                if (v[q] > room) in[i + q] = (uint32_t)room;
-                run += v[q] < room ? v[q] : room;
-            } else {
-                run += v[q];
-            }
+            run += v[q];
         }
-        if (zero_cursor) {
-            if (i + 3 < n) *reinterpret_cast<uint4 *>(zero_cursor + i) = make_uint4(0, 0, 0, 0);
-            else for (uint32_t q = i; q < n && q < i + 4; ++q) zero_cursor[q] = 0;
-        }
+        if (i + 3 < n) *reinterpret_cast<uint4 *>(zero_cursor + i) = make_uint4(0, 0, 0, 0);
+        else for (uint32_t q = i; q < n && q < i + 4; ++q) zero_cursor[q] = 0;
         __syncthreads();
         // the chunk's total: the running sum of the last thread that holds values
         if (tid == 1023) carry = wbase + incl;
@@ -839,13 +861,8 @@ __global__ __launch_bounds__(1024) void k_scan(const JobDesc *__restrict__ jobs,
     if (tid == 0) {
         const uint64_t total = carry;
         out_off[n] = (uint32_t)(total < limit ? total : limit);
-        if (is_capacity) {
-            ctr->capacity_total = (uint32_t)(total < limit ? total : limit);
-            ctr->capacity_wanted = (uint32_t)(total < 0xFFFFFFFFull ? total : 0xFFFFFFFFull);
-        } else {
-            ctr->total_pairs = (uint32_t)(total < 0xFFFFFFFFull ? total : 0xFFFFFFFFull);
-            if (total > limit) ctr->storage_overflow = 1;
-        }
+        ctr->total_pairs = (uint32_t)(total < 0xFFFFFFFFull ? total : 0xFFFFFFFFull);
+        if (total > limit) ctr->storage_overflow = 1;
     }
 }
 
@@ -2061,11 +2078,33 @@ hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_vertex(const Launch &L, hipStream_t s)
+hipError_t launch_vertex(const Launch &L, int with_capacity, hipStream_t s)
 {
-    if (L.vs_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_vertex, dim3(L.vs_blocks), dim3(256), 0, s, L.meshes, L.vs_block_group, L.vs_groups,
-                       L.vs_entries, L.jobs);
+    const uint32_t cap_blocks = L.grid[ST_CAP];
+    const uint32_t vs_blocks = with_capacity == 2 ? 0u : L.vs_blocks;  // 2: the capacity stage alone
+    if (!with_capacity || !cap_blocks) {
+        if (vs_blocks == 0) return hipSuccess;
+        hipLaunchKernelGGL(k_vertex<0>, dim3(vs_blocks), dim3(256), 0, s, L.meshes, L.vs_block_group, L.vs_groups,
+                           L.vs_entries, L.jobs, (const uint16_t *)nullptr, 0u);
+        return hipGetLastError();
+    }
+    const dim3 g(cap_blocks + vs_blocks), b(256);
+    // one instance per hidden width (a launch's learned jobs share one);
+    // heuristic-only launches take the smallest
+    switch (L.model_hidden) {
+    case 32:
+        hipLaunchKernelGGL(k_vertex<32>, g, b, 0, s, L.meshes, L.vs_block_group, L.vs_groups, L.vs_entries, L.jobs,
+                           L.table[ST_CAP], cap_blocks);
+        break;
+    case 64:
+        hipLaunchKernelGGL(k_vertex<64>, g, b, 0, s, L.meshes, L.vs_block_group, L.vs_groups, L.vs_entries, L.jobs,
+                           L.table[ST_CAP], cap_blocks);
+        break;
+    default:
+        hipLaunchKernelGGL(k_vertex<16>, g, b, 0, s, L.meshes, L.vs_block_group, L.vs_groups, L.vs_entries, L.jobs,
+                           L.table[ST_CAP], cap_blocks);
+        break;
+    }
     return hipGetLastError();
 }
 
@@ -2076,10 +2115,10 @@ hipError_t launch_setup(const Launch &L, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const Launch &L, bool capacities, hipStream_t s)
+hipError_t launch_scan(const Launch &L, hipStream_t s)
 {
-    if ((capacities ? L.cap_jobs : L.exact_jobs) == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scan, dim3(L.njobs), dim3(1024), 0, s, L.jobs, capacities ? 1 : 0);
+    if (L.exact_jobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scan, dim3(L.njobs), dim3(1024), 0, s, L.jobs);
     return hipGetLastError();
 }
 

@@ -23,7 +23,7 @@ struct Launch {
     uint32_t vs_blocks, tri_blocks;
     const uint16_t *table[ST_COUNT];  // device block -> job tables (null when njobs == 1)
     uint32_t grid[ST_COUNT];          // blocks of each stage over all jobs (0: stage skipped)
-    uint32_t cap_jobs;                // jobs binning in a single pass (capacity scan)
+    uint32_t cap_jobs;                // jobs binning in a single pass (capacities placed by the vertex launch)
     uint32_t exact_jobs;              // jobs binning in two passes (count scan)
     int model_hidden;                 // hidden width of the learned jobs' MLP (16, 32 or 64; 0: none)
     uint32_t shade_modes;             // sampling modes of the shading jobs: bit 0 point, bit 1 linear
@@ -35,9 +35,10 @@ hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s);
 hipError_t launch_interleave_maps(const uint32_t *d, const uint32_t *n, const uint32_t *sp, const uint32_t *g,
                                   uint4 *out, size_t count, hipStream_t s);
 hipError_t launch_frame_init(const Launch &L, hipStream_t s);
-hipError_t launch_vertex(const Launch &L, hipStream_t s);
-hipError_t launch_capacity(const Launch &L, hipStream_t s);  // rast_mlp.hip
-hipError_t launch_scan(const Launch &L, bool capacities, hipStream_t s);
+// with_capacity 0: the vertex pass; 1: the vertex pass with the single-pass
+// jobs' capacity workgroups beside it; 2: the capacity workgroups alone.
+hipError_t launch_vertex(const Launch &L, int with_capacity, hipStream_t s);
+hipError_t launch_scan(const Launch &L, hipStream_t s);  // exact counts
 hipError_t launch_setup(const Launch &L, hipStream_t s);
 hipError_t launch_fill(const Launch &L, hipStream_t s);
 // One tile-size group (ST_RES8 / ST_RES16 / ST_RES32); the groups are independent.

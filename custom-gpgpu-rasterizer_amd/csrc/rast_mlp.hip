@@ -4,8 +4,8 @@
 // predicts log(1 + queue length) of each screen tile from features available
 // before binning: the previous frame's exact queue lengths (the tile and its
 // 3x3 and 5x5 neighbourhoods), the previous frame's min depth and the tile
-// position: all known before this frame's vertex pass, beside which the
-// model runs.  Capacities
+// position: all known before this frame's vertex pass, in whose launch the
+// frame-time inference runs (rast_capacity.h).  Capacities
 // ceil(expm1(pred) * margin) + slack let binning run in one pass; entries
 // beyond a tile's capacity are re-binned exactly (rast_kernels.hip), so the
 // prediction never changes the frame.  The reference has no counterpart: it
@@ -24,6 +24,7 @@
 #include <new>
 #include <vector>
 
+#include "rast_capacity.h"
 #include "rast_launch.h"
 #include "rast_model.h"
 #include "rast_state.h"
@@ -36,65 +37,9 @@ constexpr int F = kNumFeatures;
 
 __host__ __device__ inline size_t num_params(int H) { return (size_t)H * F + H + (size_t)H * H + H + H + 1; }
 
-// A tile's single-pass queue capacity from a predicted length c (>= 0, or
-// NaN): c + slack, at most `limit` (the frame's triangle count: a tile's
-// queue never holds a triangle twice).  Formed in 64 bits, so neither the
-// float conversion nor the slack addition can wrap.
-__device__ __forceinline__ uint32_t clamp_capacity(float c, uint32_t slack, uint32_t limit)
-{
-    const uint64_t base = c > 0.f ? (c < (float)limit ? (uint64_t)c : (uint64_t)limit) : 0ull;
-    const uint64_t v = base + slack;
-    return v < limit ? (uint32_t)v : limit;
-}
-
 // ---------------------------------------------------------------- features
 
-// Per-tile features (inputs of the capacity model), shared by inference and
-// training-set recording, all from the previous frame (the model runs before
-// this frame's vertex pass is done): the tile's exact queue length, the mean
-// and max of its 3x3 and of its 5x5 neighbourhood (the wider ring follows
-// camera motion), its min depth, and the tile position.
-__device__ __forceinline__ void tile_features(const FrameParams &fp, uint32_t t, const uint32_t *__restrict__ prev,
-                                              const float *__restrict__ prev_depth, float *o)
-{
-    const int tx = (int)(t % fp.tiles_x), ty = (int)(t / fp.tiles_x);
-    // all loads first (tiles outside the grid count as absent), so they are
-    // in flight together; then the sums in neighbourhood order
-    uint32_t pc[25];
-    bool ok[25];
-#pragma unroll
-    for (int k = 0; k < 25; ++k) {
-        const int x = tx + k % 5 - 2, y = ty + k / 5 - 2;
-        ok[k] = x >= 0 && y >= 0 && x < (int)fp.tiles_x && y < (int)fp.tiles_y;
-        const uint32_t q = ok[k] ? (uint32_t)y * fp.tiles_x + (uint32_t)x : t;
-        pc[k] = prev[q];
-    }
-    float s3 = 0.f, m3 = 0.f, s5 = 0.f, m5 = 0.f;
-    int n3 = 0, n5 = 0;
-#pragma unroll
-    for (int k = 0; k < 25; ++k) {
-        if (!ok[k]) continue;
-        const float c = (float)pc[k];
-        const bool inner = k % 5 >= 1 && k % 5 <= 3 && k / 5 >= 1 && k / 5 <= 3;
-        if (inner) {
-            s3 += c;
-            m3 = c > m3 ? c : m3;
-            ++n3;
-        }
-        s5 += c;
-        m5 = c > m5 ? c : m5;
-        ++n5;
-    }
-    o[0] = log1pf((float)pc[12]);
-    o[1] = log1pf(s3 / (float)n3);
-    o[2] = log1pf(m3);
-    o[3] = log1pf(s5 / (float)n5);
-    o[4] = log1pf(m5);
-    o[5] = prev_depth[t];
-    o[6] = ((float)ty + 0.5f) / (float)fp.tiles_y;
-    o[7] = ((float)tx + 0.5f) / (float)fp.tiles_x;
-}
-
+// Training-set features of a rendered frame (tile_features: rast_capacity.h).
 __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t *__restrict__ prev,
                                                   const float *__restrict__ prev_depth, float *__restrict__ out)
 {
@@ -104,133 +49,6 @@ __global__ __launch_bounds__(256) void k_features(FrameParams fp, const uint32_t
     float f[F];
     tile_features(fp, t, prev, prev_depth, f);
     for (int q = 0; q < F; ++q) out[(size_t)t * F + q] = f[q];
-}
-
-// Frame-time capacities of the single-pass jobs of a launch.  Learned: the
-// two hidden layers run on the matrix cores (v_mfma_f32_16x16x4f32, fp32 in
-// and out), each wave serving 16 tiles: A holds 16 tiles x 4 inputs (lane l:
-// tile l % 16, input l / 16), B 4 inputs x 16 units (lane l: input l / 16,
-// unit l % 16), and the 16 x 16 result sits in lane u + 16 (t / 4),
-// register t % 4 (tile t, unit u).  The parameters are staged in LDS once per
-// block (rows padded against bank conflicts); the first layer's activations
-// pass through LDS as the second layer's A operand; the output layer sums
-// over units within each 16-lane group.  Otherwise the previous frame's
-// count: ceil(prev * margin) + slack.  Either capacity is clamped to the
-// frame's triangle count.
-constexpr int kCapTiles = kCapacityTilesPerBlock;
-constexpr int kCapWaves = kCapTiles / 16;  // waves per block
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <int H>
-__global__ __launch_bounds__(64 * kCapWaves) void k_capacity(const JobDesc *__restrict__ jobs,
-                                                             const uint16_t *__restrict__ table)
-{
-    static_assert(H % 16 == 0 && F % 4 == 0, "unit and input counts must suit 16 x 16 x 4 tiles");
-    // weight rows padded by one float (H + 1, F + 1) and activations stored
-    // 17 wide, so the lanes of an operand read land in different LDS banks
-    constexpr int F1 = F + 1, H1 = H + 1;
-    __shared__ float w1s[H * F1], w2s[H * H1], b1s[H], b2s[H], w3s[H];
-    __shared__ float xs[kCapWaves][F][16];
-    __shared__ float h1s[kCapWaves][H][17];
-    uint32_t local;
-    const JobDesc &J = jobs[job_of(table, jobs, ST_CAP, local)];
-    const FrameParams &fp = J.fp;
-    const uint32_t *__restrict__ prev = J.prev_count;
-    const float margin = J.margin;
-    const uint32_t slack = J.slack;
-    uint32_t *__restrict__ cap = J.tile_cap;
-    const uint32_t n = fp.tiles_x * fp.tiles_y;
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    if (!J.learned) {
-        const uint32_t t = local * kCapTiles + threadIdx.x;
-        if (threadIdx.x < (uint32_t)kCapTiles && t < n) {
-            cap[t] = clamp_capacity(ceilf((float)prev[t] * margin), slack, fp.num_tris);
-        }
-        return;
-    }
-    // parameter layout: W1 [H][F], b1 [H], W2 [H][H], b2 [H], W3 [H], b3
-    const float *__restrict__ p = J.model;
-    const float *gW1 = p, *gb1 = gW1 + H * F, *gW2 = gb1 + H, *gb2 = gW2 + H * H, *gW3 = gb2 + H;
-    const float b3 = p[H * F + H + H * H + H + H];
-    constexpr int NT = 64 * kCapWaves;
-    // 16-byte loads (W1 and W2 start at multiples of 16 bytes: offsets 0 and
-    // 9 H floats of a 256-byte aligned buffer), four padded LDS stores each
-    for (int i4 = (int)threadIdx.x; i4 < H * F / 4; i4 += NT) {
-        const float4 v = reinterpret_cast<const float4 *>(gW1)[i4];
-        float *d = w1s + (4 * i4 / F) * F1 + 4 * i4 % F;
-        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-    }
-#pragma unroll 4
-    for (int i4 = (int)threadIdx.x; i4 < H * H / 4; i4 += NT) {
-        const float4 v = reinterpret_cast<const float4 *>(gW2)[i4];
-        float *d = w2s + (4 * i4 / H) * H1 + 4 * i4 % H;
-        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-    }
-    for (int i = (int)threadIdx.x; i < H; i += NT) {
-        b1s[i] = gb1[i];
-        b2s[i] = gb2[i];
-        w3s[i] = gW3[i];
-    }
-    const uint32_t t0 = local * kCapTiles + w * 16u;  // this wave's first tile
-    if (lane < 16) {
-        float x[F];
-        if (t0 + lane < n) tile_features(fp, t0 + lane, prev, J.prev_depth, x);
-        else for (int f = 0; f < F; ++f) x[f] = 0.f;
-#pragma unroll
-        for (int f = 0; f < F; ++f) xs[w][f][lane] = x[f];
-    }
-    __syncthreads();
-    const uint32_t r16 = lane & 15u, g = lane >> 4;  // operand row / column, 4-lane-group index
-    // layer 1: h1 = relu(x W1^T + b1), K = F
-#pragma unroll
-    for (int jc = 0; jc < H / 16; ++jc) {
-        v4f acc = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-        for (int ks = 0; ks < F / 4; ++ks) {
-            const float a = xs[w][ks * 4 + g][r16];
-            const float b = w1s[(jc * 16 + r16) * F1 + ks * 4 + g];
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-        }
-        const int u = jc * 16 + (int)r16;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const float hv = acc[r] + b1s[u];
-            h1s[w][u][4 * g + r] = hv > 0.f ? hv : 0.f;
-        }
-    }
-    __syncthreads();
-    // layer 2: h2 = relu(h1 W2^T + b2), K = H; layer 3 folded into the epilogue
-    v4f acc[H / 16];
-#pragma unroll
-    for (int jc = 0; jc < H / 16; ++jc) acc[jc] = v4f{ 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-    for (int ks = 0; ks < H / 4; ++ks) {
-        const float a = h1s[w][ks * 4 + g][r16];
-#pragma unroll
-        for (int jc = 0; jc < H / 16; ++jc) {
-            const float b = w2s[(jc * 16 + r16) * H1 + ks * 4 + g];
-            acc[jc] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[jc], 0, 0, 0);
-        }
-    }
-    float o[4] = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-    for (int jc = 0; jc < H / 16; ++jc) {
-        const int u = jc * 16 + (int)r16;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const float hv = acc[jc][r] + b2s[u];
-            o[r] += w3s[u] * (hv > 0.f ? hv : 0.f);
-        }
-    }
-    // sum over the 16 units held by each 16-lane group; tile 4 g + r
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-        for (int m = 8; m > 0; m >>= 1) o[r] += __shfl_xor(o[r], m);
-    if (r16 < 4) {
-        const uint32_t t = t0 + 4u * g + r16;
-        const float out = b3 + (r16 == 0 ? o[0] : r16 == 1 ? o[1] : r16 == 2 ? o[2] : o[3]);
-        if (t < n) cap[t] = clamp_capacity(ceilf(expm1f(out < 20.f ? out : 20.f) * margin), slack, fp.num_tris);
-    }
 }
 
 __global__ __launch_bounds__(256) void k_targets(uint32_t n, const uint32_t *__restrict__ count,
@@ -517,19 +335,6 @@ hipError_t launch_tile_features(const FrameParams &fp, const uint32_t *prev_coun
 }
 }  // namespace rast
 
-namespace rast {
-hipError_t launch_capacity(const Launch &L, hipStream_t s)
-{
-    if (!L.grid[ST_CAP]) return hipSuccess;
-    const dim3 g(L.grid[ST_CAP]), b(64 * kCapWaves);
-    switch (L.model_hidden) {
-    case 32: hipLaunchKernelGGL(k_capacity<32>, g, b, 0, s, L.jobs, L.table[ST_CAP]); break;
-    case 64: hipLaunchKernelGGL(k_capacity<64>, g, b, 0, s, L.jobs, L.table[ST_CAP]); break;
-    default: hipLaunchKernelGGL(k_capacity<16>, g, b, 0, s, L.jobs, L.table[ST_CAP]); break;
-    }
-    return hipGetLastError();
-}
-}  // namespace rast
 
 void rast_model_detach_renderer(rast_binmodel *m, rast_renderer *r)
 {

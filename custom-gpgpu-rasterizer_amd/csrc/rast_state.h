@@ -67,8 +67,6 @@ struct LaunchPlan {
     // side streams running the resolve passes of different tile sizes concurrently
     hipStream_t side[2] = { nullptr, nullptr };
     hipEvent_t fork = nullptr, joined[2] = { nullptr, nullptr };
-    // the capacity model and the capacity scan run beside the vertex pass
-    hipEvent_t cap_fork = nullptr, cap_join = nullptr;
 };
 void plan_free(LaunchPlan &p);
 // Waits until no queued GPU work uses r's buffers (its stream and, when its

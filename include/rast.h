@@ -116,7 +116,7 @@ typedef struct {
     uint32_t rerenders;          /* frames re-run after a queue-storage overflow (storage grown) */
     uint32_t overflow_entries;   /* single-pass policies: queue entries placed by the exact fallback */
     float ms_vertex, ms_setup, ms_count, ms_scan, ms_fill, ms_fallback, ms_raster, ms_shade, ms_total;
-    float ms_model;              /* capacity-model inference */
+    float ms_model;              /* single pass: tile capacities (model or heuristic) and queue regions */
 } rast_stats;
 
 /* Device query: CCR/CUDARenderer.cu:59-175 (DisplayGPUSpecs). */
