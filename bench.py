@@ -57,8 +57,9 @@ def parse():
                     help="only run device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--train-frames", type=int, default=200)
     ap.add_argument("--epochs", type=int, default=8)
-    ap.add_argument("--margin", type=float, default=1.25)
-    ap.add_argument("--slack", type=int, default=4)
+    ap.add_argument("--margin", type=float, default=None,
+                    help="learned queue depth margin (default: the tuning table's for the workload's bucket)")
+    ap.add_argument("--slack", type=int, default=None, help="learned queue depth slack (default: as --margin)")
     ap.add_argument("--cpu-frames", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--policies", default="exact,learned,prevframe")
@@ -106,9 +107,19 @@ def stage_profile(R, r, cams, opts, policy, warmup, steps):
     return {k: v / steps for k, v in acc.items()}, extra
 
 
+def queue_of(autotune, args, width, height, tris, table=None):
+    """Queue depth (margin, slack) per single-pass policy: the tuning
+    table's for the frame's bucket; --margin / --slack replace the learned
+    policy's."""
+    q = {p: autotune.queue_depth(width, height, tris, p, table) for p in ("learned", "prevframe")}
+    m, s = q["learned"]
+    q["learned"] = (m if args.margin is None else args.margin, s if args.slack is None else args.slack)
+    return q
+
+
 def train_model(R, r, w, train_path, args):
     model = R.BinModel(r, hidden=64, seed=7)
-    model.attach(r, args.margin, args.slack)
+    model.attach(r, *args.queue["learned"])
     for i, cam in enumerate(train_path):
         r.render(cam, w.opts, R.BIN_EXACT, R.FLAG_NO_SHADE)
         if i > 0:
@@ -137,11 +148,10 @@ def run_policies(R, r, w, train_path, test_path, policies, args):
     code = {"exact": R.BIN_EXACT, "learned": R.BIN_LEARNED, "prevframe": R.BIN_PREVFRAME}
 
     def select(p):
-        if model is not None:
-            if p == "learned":
-                model.attach(r, args.margin, args.slack)
-            else:
-                model.detach(r)
+        if model is not None and p == "learned":
+            model.attach(r, *args.queue["learned"])
+        else:  # detaches the model
+            r.set_queue_depth(*args.queue["prevframe"])
 
     rounds = {p: [] for p in policies}
     for _ in range(ROUNDS):
@@ -221,8 +231,8 @@ def flythrough_workload(R, S, autotune, args):
 
     learned, exact = views_for(), views_for()
     models = [R.BinModel(v, hidden=64, seed=31 + k) for k, v in enumerate(learned)]
-    for m, v in zip(models, learned):
-        m.attach(v, args.margin, args.slack)
+    for m, v, (_, W, H) in zip(models, learned, fly.views(0)):
+        m.attach(v, *queue_of(autotune, args, W, H, ntri, table)["learned"])
     train = S.Flythrough(fly.scene, S.camera_path(S.SEED_BASE + 0x505, args.fly_train_frames), fly.main, fly.face)
     t0 = time.perf_counter()
     for f in range(len(train.path)):
@@ -392,6 +402,7 @@ def main():
     tuned = autotune.choose(w.width, w.height, ntri)
     tuned = autotune.Params(tile=args.tile or tuned.tile, micro=tuned.micro if args.micro is None else args.micro,
                             small=tuned.small if args.small is None else args.small, big=args.big or tuned.big)
+    args.queue = queue_of(autotune, args, w.width, w.height, ntri)
     r = R.Renderer(w.width, w.height)
     tuned.apply(r)
     r.add_scene(w.scene)
@@ -450,7 +461,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural terrain, textures and camera path)",
         "config": {"workload": w.name, "triangles": ntri, "width": w.width, "height": w.height,
-                   "binning": head, "params": vars(tuned), "camera_path_frames": 100,
+                   "binning": head, "params": vars(tuned),
+                   "queue_depth": {k: {"margin": m, "slack": s} for k, (m, s) in args.queue.items()},
+                   "camera_path_frames": 100,
                    "train_frames": args.train_frames, "graph_replay": not args.no_graph,
                    "timing": f"median of {ROUNDS} rounds of {args.steps} back-to-back frames, policies interleaved"},
         "gpix_per_s": round(P / (ms * 1e-3) / 1e9, 3),

@@ -14,14 +14,20 @@ given class of frames fastest on the device at hand:
   small       tile-clipped pixel count from which the
               resolve pass rasterizes a triangle per pixel    rast_set_tuning
 
-and, for single-pass binning, the per-tile queue depth, capacity =
-ceil(predicted * margin) + slack (rast_renderer_set_binmodel).
+and, for the single-pass policies, the per-tile queue depth, capacity =
+ceil(predicted * margin) + slack (rast_renderer_set_binmodel), swept over
+margin x slack for the learned model and for the previous-frame heuristic
+once the parameters above are fixed.  Queue depth matters only when the
+frame differs from the one the capacities were predicted from, so it is
+timed over consecutive frames of a camera path: the workload's own path
+where it has one, else a short sideways pan from each job's camera.
 
 Jobs are bucketed by log2 of their pixel count and triangle count; every
 candidate is timed on all jobs of a bucket rendered as one batch (captured
 graph, so host launch costs are excluded), and the fastest wins.  The
-result is a JSON table that `choose()` reads back.  Output never depends on
-these parameters (the GPU tests check tile, micro and small invariance).
+result is a JSON table that `choose()` and `queue_depth()` read back.
+Output never depends on these parameters (the GPU tests check tile, micro
+and small invariance, and single-pass frames against exact ones).
 """
 from __future__ import annotations
 
@@ -42,6 +48,9 @@ TILES = (8, 16, 32)
 MICROS = (0, 16, 64, 128, 256)
 SMALLS = (16, 64, 192)
 BIGS = (64, 256, 1024)
+QUEUE_MARGINS = (1.0, 1.1, 1.25, 1.5)
+QUEUE_SLACKS = (0, 4, 16)
+DEFAULT_QUEUE = {"learned": (1.25, 4), "prevframe": (1.0, 0)}
 
 
 @dataclass
@@ -91,6 +100,28 @@ def choose(width: int, height: int, tris: int, table: Optional[Dict] = None) -> 
     return Params(**buckets[min(buckets, key=dist)]["params"])
 
 
+def queue_depth(width: int, height: int, tris: int, policy: str = "learned",
+                table: Optional[Dict] = None) -> tuple:
+    """(margin, slack) of a single-pass policy ("learned" or "prevframe")
+    for the bucket of a width x height frame of `tris` triangles: the tuned
+    pair of that bucket, else of the nearest bucket that has one, else the
+    default."""
+    table = load_table() if table is None else table
+    tuned = {k: b["queue"][policy] for k, b in table.get("buckets", {}).items()
+             if policy in b.get("queue", {})}
+    if not tuned:
+        return DEFAULT_QUEUE[policy]
+    key = bucket_of(width * height, tris)
+
+    def dist(k: str) -> int:
+        p, t = k[1:].split("_t")
+        q, u = key[1:].split("_t")
+        return abs(int(p) - int(q)) + abs(int(t) - int(u))
+
+    q = tuned[key] if key in tuned else tuned[min(tuned, key=dist)]
+    return float(q["margin"]), int(q["slack"])
+
+
 def candidates(tiles=TILES, micros=MICROS, smalls=SMALLS, bigs=BIGS) -> List[Params]:
     return [Params(t, m, s, b) for t, m, s, b in itertools.product(tiles, micros, smalls, bigs)]
 
@@ -125,8 +156,97 @@ def tune_bucket(R, jobs, cands: Sequence[Params], reps: int = 5, log=None):
     return timings[0][1], timings[0][0], timings
 
 
+def device_label(R) -> Dict:
+    info = R.device_info(0)
+    # the marketing name is empty on some ROCm releases; the ISA name is not
+    return {"device": info["name"] or info["arch"], "arch": info["arch"], "compute_units": info["compute_units"]}
+
+
+def pan_path(cam, frames: int = 6, step: float = 0.01) -> list:
+    """A sideways pan from `cam`: each frame moves the camera by `step`
+    times its distance from the origin, perpendicular to its view."""
+    import numpy as np
+    from dataclasses import replace
+
+    p = np.array(cam.position, np.float64)
+    f = np.array(cam.forward, np.float64)
+    right = np.cross(np.array([0.0, 1.0, 0.0]), f)
+    if np.linalg.norm(right) < 1e-6:
+        right = np.array([1.0, 0.0, 0.0])
+    right /= np.linalg.norm(right)
+    d = step * max(np.linalg.norm(p), 1e-3)
+    return [replace(cam, position=tuple(float(v) for v in p + k * d * right)) for k in range(frames)]
+
+
+def time_path(R, jobs, paths, policy: int, frames: int, reps: int) -> float:
+    """Milliseconds per batch over consecutive frames of each job's path
+    (every job in one batched launch per frame, frames back to back)."""
+    def frame(k):
+        return [(r, path[k % len(path)], opts, policy, 0) for (r, _, opts, *_), path in zip(jobs, paths)]
+
+    R.render_batch(frame(0))  # capture
+    best = float("inf")
+    for _ in range(reps):
+        R.render_batch(frame(1))  # the previous frame of the first timed one
+        for r, *_ in jobs:
+            r.sync()
+        t0 = time.perf_counter()
+        for k in range(frames):
+            R.render_batch(frame(2 + k))
+        for r, *_ in jobs:
+            r.sync()
+        best = min(best, (time.perf_counter() - t0) / frames)
+    return best * 1e3
+
+
+def train_bucket_model(R, jobs, paths, frames: int = 24, log=None):
+    """One capacity model for a bucket's jobs, trained on their exact
+    counts along their paths (a model serves every renderer it is attached
+    to)."""
+    m = R.BinModel(jobs[0][0], hidden=64, seed=11)
+    for r, *_ in jobs:
+        m.attach(r, *DEFAULT_QUEUE["learned"])
+    for k in range(frames):
+        R.render_batch([(r, path[k % len(path)], opts, R.BIN_EXACT, R.FLAG_NO_SHADE)
+                        for (r, _, opts, *_), path in zip(jobs, paths)])
+        if k > 0:
+            for r, *_ in jobs:
+                m.record(r)
+    loss = m.train(epochs=4, batch=8192, lr=2e-3, tau=0.95)
+    if log:
+        log(f"  model trained, loss {loss:.4f}")
+    return m
+
+
+def tune_queue(R, jobs, paths, frames: int = 16, reps: int = 5, log=None) -> Dict:
+    """Fastest (margin, slack) per single-pass policy for one bucket;
+    exact two-pass binning timed alongside on the same frames."""
+    out = {"exact_ms_per_batch": round(time_path(R, jobs, paths, R.BIN_EXACT, frames, reps), 4)}
+    model = train_bucket_model(R, jobs, paths, log=log)
+    try:
+        for name, code in (("learned", R.BIN_LEARNED), ("prevframe", R.BIN_PREVFRAME)):
+            timings = []
+            for margin, slack in itertools.product(QUEUE_MARGINS, QUEUE_SLACKS):
+                for r, *_ in jobs:
+                    if name == "learned":
+                        model.attach(r, margin, slack)
+                    else:
+                        r.set_queue_depth(margin, slack)
+                ms = time_path(R, jobs, paths, code, frames, reps)
+                timings.append((ms, margin, slack))
+                if log:
+                    log(f"  {name} margin {margin} slack {slack}: {ms:.3f} ms")
+            ms, margin, slack = min(timings)
+            out[name] = {"margin": margin, "slack": slack, "ms_per_batch": round(ms, 4)}
+    finally:
+        for r, *_ in jobs:
+            model.detach(r)
+        model.close()
+    return out
+
+
 def tune(R, jobs_by_bucket: Dict[str, list], cands: Sequence[Params], reps: int = 5, log=None) -> Dict:
-    out = {"device": R.device_info(0)["name"], "buckets": {}}
+    out = {**device_label(R), "buckets": {}}
     for key, jobs in sorted(jobs_by_bucket.items()):
         if log:
             log(f"bucket {key}: {len(jobs)} jobs")
@@ -147,41 +267,63 @@ def _main():
     ap.add_argument("--out", default=TABLE_PATH)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="a reduced candidate set")
+    ap.add_argument("--queue-only", action="store_true",
+                    help="keep the table's binning parameters, sweep only the queue depth")
+    ap.add_argument("--no-queue", action="store_true", help="skip the queue-depth sweep")
     args = ap.parse_args()
     cands = candidates(micros=(0, 64, 128), smalls=(16, 192), bigs=(256,)) if args.quick else candidates()
     log = lambda s: print(s, flush=True)  # noqa: E731
     by_bucket: Dict[str, list] = {}
+    paths: Dict[str, list] = {}  # per bucket, each job's camera path (queue-depth sweep)
     keep = []
+
+    def add(key, job, path):
+        by_bucket.setdefault(key, []).append(job)
+        paths.setdefault(key, []).append(path)
+
     names = args.workloads.split(",")
     if "terrain4k" in names:
         w = S.config3()
         r = R.Renderer(w.width, w.height)
         r.add_scene(w.scene)
         keep.append(r)
-        _, path = S.train_test_paths(1, 4)
-        by_bucket.setdefault(bucket_of(w.width * w.height, w.scene.num_triangles), []).append(
-            (r, path[0], w.opts, R.BIN_EXACT, 0))
+        _, path = S.train_test_paths(1, 24)
+        add(bucket_of(w.width * w.height, w.scene.num_triangles), (r, path[0], w.opts, R.BIN_EXACT, 0), path)
     if "batch256" in names:
         for job in S.config4():
             r = R.Renderer(job.width, job.height)
             r.add_scene(job.scene)
             keep.append(r)
-            by_bucket.setdefault(bucket_of(job.pixels, job.scene.num_triangles), []).append(
-                (r, job.camera, job.opts, R.BIN_EXACT, 0))
+            add(bucket_of(job.pixels, job.scene.num_triangles), (r, job.camera, job.opts, R.BIN_EXACT, 0),
+                pan_path(job.camera, 24))
     if "flythrough" in names:
         # the main view and the six cube faces of the combined workload (views of one scene)
-        fly = S.config5(frames=1)
+        fly = S.config5(frames=24)
         owner = R.Renderer(*fly.main)
         owner.add_scene(fly.scene)
         keep.append(owner)
         views = [owner] + [owner.view(fly.face, fly.face) for _ in range(6)]
         keep[len(keep):len(keep)] = views[1:]
-        for v, (cam, W, H) in zip(views, fly.views(0)):
-            by_bucket.setdefault(bucket_of(W * H, fly.scene.num_triangles), []).append(
-                (v, cam, fly.opts, R.BIN_EXACT, 0))
+        per_frame = [fly.views(f) for f in range(len(fly.path))]
+        for k, (v, (cam, W, H)) in enumerate(zip(views, per_frame[0])):
+            add(bucket_of(W * H, fly.scene.num_triangles), (v, cam, fly.opts, R.BIN_EXACT, 0),
+                [pf[k][0] for pf in per_frame])
     t0 = time.time()
-    table = tune(R, by_bucket, cands, args.reps, log)
-    table["candidates"] = len(cands)
+    if args.queue_only:
+        table = load_table(args.out)
+        table.update(device_label(R))
+    else:
+        table = tune(R, by_bucket, cands, args.reps, log)
+        table["candidates"] = len(cands)
+    if not args.no_queue:
+        for key, jobs in sorted(by_bucket.items()):
+            p = Params(**table["buckets"][key]["params"]) if key in table["buckets"] else Params()
+            for r, *_ in jobs:
+                p.apply(r)
+            log(f"bucket {key}: queue depth, {len(jobs)} jobs, {asdict(p)}")
+            q = tune_queue(R, jobs, paths[key], log=log)
+            table["buckets"].setdefault(key, {"params": asdict(p)})["queue"] = q
+        table["queue_candidates"] = {"margin": list(QUEUE_MARGINS), "slack": list(QUEUE_SLACKS)}
     table["seconds"] = round(time.time() - t0, 1)
     for r in reversed(keep):  # views before the renderer whose meshes they draw
         r.close()

@@ -205,6 +205,11 @@ class Renderer:
     def set_tuning(self, micro_limit: int = 16, small_limit: int = 16) -> None:
         check(lib().rast_set_tuning(self.h, micro_limit, small_limit), "rast_set_tuning")
 
+    def set_queue_depth(self, margin: float = 1.0, slack: int = 0) -> None:
+        """Queue depth of the previous-frame heuristic, ceil(prev * margin) +
+        slack per tile (detaches any capacity model)."""
+        check(lib().rast_renderer_set_binmodel(self.h, None, margin, slack), "rast_renderer_set_binmodel")
+
     def texture(self, texels: np.ndarray) -> C.c_void_p:
         key = id(texels)
         if key not in self.textures:
