@@ -1,5 +1,6 @@
 """Asset formats either side of the rasterizer (librast_io.so over ctypes):
-Wavefront OBJ meshes in (the reference's ObjParser, CCR/ObjParser.cpp:23-584)
+Wavefront OBJ meshes in (the reference's ObjParser, CCR/ObjParser.cpp:23-584),
+PNG textures in (IMG_Load in CUDATexture::Create, CCR/CUDATexture.cpp:24-45)
 and 32-bit BMP framebuffers out (SaveBackbufferToImage,
 CCR/ERenderer.cpp:174-177)."""
 from __future__ import annotations
@@ -37,7 +38,11 @@ def lib():
         L.rio_free.restype = None
         L.rio_write_bmp.argtypes = [C.c_char_p, vp, C.c_uint32, C.c_uint32]
         L.rio_last_error.argtypes = [C.c_char_p, C.c_size_t]
-        for f in ("rio_obj_parse", "rio_obj_load", "rio_write_bmp", "rio_last_error"):
+        L.rio_png_decode.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(vp), u32p, u32p]
+        L.rio_png_load.argtypes = [C.c_char_p, C.POINTER(vp), u32p, u32p]
+        L.rio_png_encode.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(vp), C.POINTER(C.c_size_t)]
+        for f in ("rio_obj_parse", "rio_obj_load", "rio_write_bmp", "rio_last_error", "rio_png_decode",
+                  "rio_png_load", "rio_png_encode"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -95,3 +100,36 @@ def read_bmp(path: str) -> np.ndarray:
         px = px[::-1]
     rgba = px[..., [2, 1, 0, 3]].astype(np.uint32)
     return rgba[..., 0] | rgba[..., 1] << 8 | rgba[..., 2] << 16 | rgba[..., 3] << 24
+
+
+def _texels(call, what) -> np.ndarray:
+    t = C.c_void_p()
+    w, h = C.c_uint32(0), C.c_uint32(0)
+    if call(C.byref(t), C.byref(w), C.byref(h)) != 0:
+        raise _error(what)
+    try:
+        return np.ctypeslib.as_array(C.cast(t, C.POINTER(C.c_uint32)), (h.value, w.value)).copy()
+    finally:
+        lib().rio_free(t)
+
+
+def decode_png(data: bytes) -> np.ndarray:
+    """RGBA8 texels (memory order R,G,B,A; rows top to bottom), shape (h, w)."""
+    return _texels(lambda *a: lib().rio_png_decode(data, len(data), *a), "rio_png_decode")
+
+
+def load_png(path: str) -> np.ndarray:
+    return _texels(lambda *a: lib().rio_png_load(path.encode(), *a), "rio_png_load")
+
+
+def encode_png(rgba: np.ndarray, filter: int = -1) -> bytes:
+    """An 8-bit RGBA PNG of RGBA8 texels; filter 0..4 for every row, -1 for
+    row y filtered with type y % 5."""
+    a = np.ascontiguousarray(rgba, np.uint32)
+    p, n = C.c_void_p(), C.c_size_t(0)
+    if lib().rio_png_encode(a.ctypes.data, a.shape[1], a.shape[0], filter, C.byref(p), C.byref(n)) != 0:
+        raise _error("rio_png_encode")
+    try:
+        return C.string_at(p, n.value)
+    finally:
+        lib().rio_free(p)

@@ -2,7 +2,7 @@
 
   librast.so         HIP kernels for gfx950 + the C ABI of include/rast.h
   librast_scenes.so  host-only deterministic scene generators (include/rast_scene.h)
-  librast_io.so      host-only OBJ reader and BMP writer (include/rast_io.h)
+  librast_io.so      host-only OBJ reader, PNG codec (system zlib) and BMP writer (include/rast_io.h)
 
 Every call recompiles.  Float code is built with -ffp-contract=off (no FMA
 contraction) and without fast-math, so device arithmetic matches the CPU
@@ -57,7 +57,7 @@ def build_io() -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
     out = os.path.join(LIB_DIR, "librast_io.so")
     tmp = out + ".tmp"
-    _run([CXX, *HOST_FLAGS, "-I", INCLUDE, "-o", tmp, os.path.join(CSRC, "rast_io.cpp")])
+    _run([CXX, *HOST_FLAGS, "-I", INCLUDE, "-o", tmp, os.path.join(CSRC, "rast_io.cpp"), "-lz"])
     os.replace(tmp, out)
     return out
 

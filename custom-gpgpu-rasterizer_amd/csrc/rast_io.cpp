@@ -22,6 +22,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <zlib.h>
+
 namespace {
 
 thread_local std::string g_err;
@@ -369,4 +371,267 @@ extern "C" int rio_write_bmp(const char *path, const uint32_t *rgba8, uint32_t w
     }
     if (std::fclose(f) != 0) ok = false;
     return ok ? RAST_OK : fail("write to %s failed", path);
+}
+
+// ------------------------------------------------------------------- PNG
+//
+// The reference loads its textures with SDL_image's IMG_Load
+// (CCR/CUDATexture.cpp:24-45), which for its PNG files yields 32-bit RGBA
+// texels (bytes R, G, B, A in memory: the texel format used here).  This
+// decoder is written from the PNG specification (ISO/IEC 15948): signature,
+// chunks with their CRC-32, IHDR, PLTE / tRNS, the concatenated IDAT stream
+// inflated with zlib, and the five per-row filters (None, Sub, Up, Average,
+// Paeth) undone over the bytes of a row.  Bit depth 8, colour types 0
+// (grey), 2 (RGB), 3 (palette), 4 (grey + alpha) and 6 (RGBA), no
+// interlace; anything else is rejected with an error.
+
+namespace {
+
+uint32_t be32(const uint8_t *p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+void put_be32(std::vector<uint8_t> &o, uint32_t v)
+{
+    o.push_back((uint8_t)(v >> 24));
+    o.push_back((uint8_t)(v >> 16));
+    o.push_back((uint8_t)(v >> 8));
+    o.push_back((uint8_t)v);
+}
+
+const uint8_t kPngSig[8] = { 137, 80, 78, 71, 13, 10, 26, 10 };
+
+// Paeth predictor (ISO/IEC 15948, 9.4): the neighbour nearest to a + b - c.
+inline uint8_t paeth(int a, int b, int c)
+{
+    const int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+    if (pa <= pb && pa <= pc) return (uint8_t)a;
+    return (uint8_t)(pb <= pc ? b : c);
+}
+
+// Undoes filter `type` of one row in place; prev is the previous row after
+// unfiltering (zeros above the first row), bpp the bytes per pixel.
+bool unfilter_row(int type, uint8_t *row, const uint8_t *prev, size_t n, size_t bpp)
+{
+    switch (type) {
+    case 0: return true;
+    case 1:
+        for (size_t i = bpp; i < n; ++i) row[i] = (uint8_t)(row[i] + row[i - bpp]);
+        return true;
+    case 2:
+        for (size_t i = 0; i < n; ++i) row[i] = (uint8_t)(row[i] + prev[i]);
+        return true;
+    case 3:
+        for (size_t i = 0; i < n; ++i) {
+            const int a = i >= bpp ? row[i - bpp] : 0;
+            row[i] = (uint8_t)(row[i] + ((a + prev[i]) >> 1));
+        }
+        return true;
+    case 4:
+        for (size_t i = 0; i < n; ++i) {
+            const int a = i >= bpp ? row[i - bpp] : 0, c = i >= bpp ? prev[i - bpp] : 0;
+            row[i] = (uint8_t)(row[i] + paeth(a, prev[i], c));
+        }
+        return true;
+    default: return false;
+    }
+}
+
+// Applies filter `type` to one row (the inverse of unfilter_row): out[i] =
+// raw[i] - predictor, predictors from the unfiltered bytes.
+void filter_row(int type, const uint8_t *raw, const uint8_t *prev, size_t n, size_t bpp, uint8_t *out)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const int a = i >= bpp ? raw[i - bpp] : 0, b = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+        int pred = 0;
+        switch (type) {
+        case 1: pred = a; break;
+        case 2: pred = b; break;
+        case 3: pred = (a + b) >> 1; break;
+        case 4: pred = paeth(a, b, c); break;
+        default: pred = 0; break;
+        }
+        out[i] = (uint8_t)(raw[i] - pred);
+    }
+}
+
+}  // namespace
+
+extern "C" int rio_png_decode(const uint8_t *data, size_t len, uint32_t **texels, uint32_t *width, uint32_t *height)
+{
+    if (!data || !texels || !width || !height) return fail("rio_png_decode: null argument");
+    *texels = nullptr;
+    *width = *height = 0;
+    if (len < 8 || std::memcmp(data, kPngSig, 8) != 0) return fail("not a PNG file (signature)");
+    size_t at = 8;
+    uint32_t w = 0, h = 0;
+    int ctype = -1, seen_ihdr = 0, seen_end = 0;
+    uint8_t pal[256][4];
+    uint32_t npal = 0;
+    for (int i = 0; i < 256; ++i) pal[i][0] = pal[i][1] = pal[i][2] = 0, pal[i][3] = 255;
+    int trns_grey = -1, trns_rgb[3] = { -1, -1, -1 };
+    std::vector<uint8_t> z;
+    while (!seen_end) {
+        if (len - at < 12) return fail("truncated chunk at byte %zu", at);
+        const uint32_t n = be32(data + at);
+        const uint8_t *type = data + at + 4, *body = data + at + 8;
+        if (n > 0x7FFFFFFFu || len - at - 12 < n) return fail("chunk length %u past the end of the file", n);
+        const uint32_t crc = be32(body + n);
+        if ((uint32_t)crc32(crc32(0L, Z_NULL, 0), type, n + 4) != crc)
+            return fail("CRC mismatch in chunk %.4s", (const char *)type);
+        const bool critical = !(type[0] & 0x20);
+        if (std::memcmp(type, "IHDR", 4) == 0) {
+            if (seen_ihdr || n != 13) return fail("bad IHDR");
+            w = be32(body);
+            h = be32(body + 4);
+            const int depth = body[8];
+            ctype = body[9];
+            if (w == 0 || h == 0 || w > 65535 || h > 65535) return fail("image size %ux%u not supported", w, h);
+            if (depth != 8) return fail("bit depth %d not supported (8 only)", depth);
+            if (ctype != 0 && ctype != 2 && ctype != 3 && ctype != 4 && ctype != 6) return fail("colour type %d", ctype);
+            if (body[10] != 0 || body[11] != 0) return fail("unknown compression or filter method");
+            if (body[12] != 0) return fail("interlaced PNG not supported");
+            seen_ihdr = 1;
+        } else if (!seen_ihdr) {
+            return fail("first chunk is not IHDR");
+        } else if (std::memcmp(type, "PLTE", 4) == 0) {
+            if (n % 3 || n / 3 > 256 || n == 0) return fail("bad PLTE");
+            npal = n / 3;
+            for (uint32_t i = 0; i < npal; ++i)
+                for (int c = 0; c < 3; ++c) pal[i][c] = body[3 * i + c];
+        } else if (std::memcmp(type, "tRNS", 4) == 0) {
+            if (ctype == 3) {
+                if (n > npal) return fail("tRNS longer than the palette");
+                for (uint32_t i = 0; i < n; ++i) pal[i][3] = body[i];
+            } else if (ctype == 0 && n == 2) {
+                trns_grey = (int)(body[0] << 8 | body[1]);
+            } else if (ctype == 2 && n == 6) {
+                for (int c = 0; c < 3; ++c) trns_rgb[c] = (int)(body[2 * c] << 8 | body[2 * c + 1]);
+            } else {
+                return fail("bad tRNS");
+            }
+        } else if (std::memcmp(type, "IDAT", 4) == 0) {
+            z.insert(z.end(), body, body + n);
+        } else if (std::memcmp(type, "IEND", 4) == 0) {
+            seen_end = 1;
+        } else if (critical) {
+            return fail("unknown critical chunk %.4s", (const char *)type);
+        }
+        at += 12 + (size_t)n;
+    }
+    if (z.empty()) return fail("no IDAT");
+    if (ctype == 3 && npal == 0) return fail("palette image without PLTE");
+    const size_t bpp = ctype == 0 || ctype == 3 ? 1 : ctype == 4 ? 2 : ctype == 2 ? 3 : 4;
+    const size_t stride = (size_t)w * bpp, raw_len = (stride + 1) * h;
+    std::vector<uint8_t> raw(raw_len + 1);
+    uLongf got = (uLongf)raw.size();
+    const int zr = uncompress(raw.data(), &got, z.data(), (uLong)z.size());
+    if (zr != Z_OK) return fail("zlib inflate failed (%d)", zr);
+    if (got != raw_len) return fail("image data is %lu bytes, expected %zu", (unsigned long)got, raw_len);
+    uint32_t *out = (uint32_t *)std::malloc((size_t)w * h * 4);
+    if (!out) {
+        g_err = "out of host memory";
+        return RAST_E_OUT_OF_MEMORY;
+    }
+    std::vector<uint8_t> zero(stride, 0);
+    const uint8_t *prev = zero.data();
+    for (uint32_t y = 0; y < h; ++y) {
+        uint8_t *row = raw.data() + (size_t)y * (stride + 1);
+        if (!unfilter_row(row[0], row + 1, prev, stride, bpp)) {
+            std::free(out);
+            return fail("row %u: unknown filter type %d", y, row[0]);
+        }
+        const uint8_t *px = row + 1;
+        uint32_t *dst = out + (size_t)y * w;
+        for (uint32_t x = 0; x < w; ++x) {
+            uint32_t r, g, b, a = 255;
+            switch (ctype) {
+            case 0:
+                r = g = b = px[x];
+                if (trns_grey == (int)px[x]) a = 0;
+                break;
+            case 2:
+                r = px[3 * x], g = px[3 * x + 1], b = px[3 * x + 2];
+                if (trns_rgb[0] == (int)r && trns_rgb[1] == (int)g && trns_rgb[2] == (int)b) a = 0;
+                break;
+            case 3:
+                if (px[x] >= npal) {
+                    std::free(out);
+                    return fail("palette index %u out of range", px[x]);
+                }
+                r = pal[px[x]][0], g = pal[px[x]][1], b = pal[px[x]][2], a = pal[px[x]][3];
+                break;
+            case 4: r = g = b = px[2 * x], a = px[2 * x + 1]; break;
+            default: r = px[4 * x], g = px[4 * x + 1], b = px[4 * x + 2], a = px[4 * x + 3]; break;
+            }
+            dst[x] = r | g << 8 | b << 16 | a << 24;
+        }
+        prev = row + 1;
+    }
+    *texels = out;
+    *width = w;
+    *height = h;
+    return RAST_OK;
+}
+
+extern "C" int rio_png_load(const char *path, uint32_t **texels, uint32_t *width, uint32_t *height)
+{
+    if (!path) return fail("rio_png_load: null path");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail("cannot open %s", path);
+    std::vector<uint8_t> bytes;
+    uint8_t buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) bytes.insert(bytes.end(), buf, buf + got);
+    std::fclose(f);
+    return rio_png_decode(bytes.data(), bytes.size(), texels, width, height);
+}
+
+extern "C" int rio_png_encode(const uint32_t *rgba8, uint32_t width, uint32_t height, int filter, uint8_t **out,
+                              size_t *out_len)
+{
+    if (!rgba8 || !out || !out_len || !width || !height || width > 65535 || height > 65535 || filter < -1 ||
+        filter > 4)
+        return fail("rio_png_encode: bad arguments");
+    *out = nullptr;
+    *out_len = 0;
+    const size_t stride = (size_t)width * 4;
+    std::vector<uint8_t> raw((stride + 1) * height), cur(stride), prev(stride, 0);
+    for (uint32_t y = 0; y < height; ++y) {
+        for (uint32_t x = 0; x < width; ++x) {
+            const uint32_t t = rgba8[(size_t)y * width + x];
+            for (int c = 0; c < 4; ++c) cur[4 * (size_t)x + c] = (uint8_t)(t >> (8 * c));
+        }
+        const int type = filter >= 0 ? filter : (int)(y % 5);  // -1: every filter in turn
+        uint8_t *row = raw.data() + (size_t)y * (stride + 1);
+        row[0] = (uint8_t)type;
+        filter_row(type, cur.data(), prev.data(), stride, 4, row + 1);
+        prev.swap(cur);
+    }
+    uLongf zlen = compressBound((uLong)raw.size());
+    std::vector<uint8_t> z(zlen);
+    if (compress2(z.data(), &zlen, raw.data(), (uLong)raw.size(), 6) != Z_OK) return fail("zlib deflate failed");
+    std::vector<uint8_t> o(kPngSig, kPngSig + 8);
+    auto chunk = [&](const char *type, const uint8_t *body, size_t n) {
+        put_be32(o, (uint32_t)n);
+        const size_t start = o.size();
+        o.insert(o.end(), type, type + 4);
+        o.insert(o.end(), body, body + n);
+        put_be32(o, (uint32_t)crc32(crc32(0L, Z_NULL, 0), o.data() + start, (uInt)(n + 4)));
+    };
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, width);
+    put_be32(ihdr, height);
+    const uint8_t tail[5] = { 8, 6, 0, 0, 0 };  // 8-bit RGBA, deflate, adaptive filtering, no interlace
+    ihdr.insert(ihdr.end(), tail, tail + 5);
+    chunk("IHDR", ihdr.data(), ihdr.size());
+    chunk("IDAT", z.data(), zlen);
+    chunk("IEND", nullptr, 0);
+    uint8_t *p = (uint8_t *)std::malloc(o.size());
+    if (!p) {
+        g_err = "out of host memory";
+        return RAST_E_OUT_OF_MEMORY;
+    }
+    std::memcpy(p, o.data(), o.size());
+    *out = p;
+    *out_len = o.size();
+    return RAST_OK;
 }

@@ -1,11 +1,13 @@
 /*
  * rast_io.h -- host-side asset formats on either side of the rasterizer:
- * Wavefront OBJ meshes in, framebuffers out (librast_io.so, no GPU code).
+ * Wavefront OBJ meshes and PNG textures in, framebuffers out
+ * (librast_io.so, no GPU code; PNG over the system zlib).
  *
  * Reference (kdedeurw/Custom-GPGPU-Rasterizer, CCR = CustomCUDARasterizer/):
  *   ObjParser::OpenFile / ParseData / StorePosition / StoreFace / StoreUV /
  *   StoreNormal / AssignVertices  CCR/ObjParser.cpp:23-584
  *   Elite::Renderer::SaveBackbufferToImage (SDL_SaveBMP)  CCR/ERenderer.cpp:174-177
+ *   CUDATexture::Create (IMG_Load + pitched upload)  CCR/CUDATexture.cpp:24-45
  *
  * OBJ parsing follows the reference line by line, including its format
  * assumptions: a position line is "v" followed by two spaces (the 3ds Max
@@ -51,6 +53,20 @@ void rio_free(void *p);
  * bytes), as SDL_SaveBMP writes a 32-bit surface; rgba8 texels in memory
  * order R,G,B,A, rows of width texels. */
 int rio_write_bmp(const char *path, const uint32_t *rgba8, uint32_t width, uint32_t height);
+
+/* IMG_Load for the reference's textures (CCR/CUDATexture.cpp:24-45): a PNG
+ * file's bytes decoded to rgba8 texels (memory order R,G,B,A, rows top to
+ * bottom, the layout rast_texture_create takes).  Bit depth 8; grey, RGB,
+ * palette, grey + alpha and RGBA; not interlaced; every chunk's CRC is
+ * checked.  *texels is released with rio_free. */
+int rio_png_decode(const uint8_t *data, size_t len, uint32_t **texels, uint32_t *width, uint32_t *height);
+/* The same from a file. */
+int rio_png_load(const char *path, uint32_t **texels, uint32_t *width, uint32_t *height);
+/* rgba8 texels to an 8-bit RGBA PNG (no counterpart in the reference: test
+ * data and frame output).  filter 0..4 filters every row with that type,
+ * -1 row y with type y % 5.  *out is released with rio_free. */
+int rio_png_encode(const uint32_t *rgba8, uint32_t width, uint32_t height, int filter, uint8_t **out,
+                   size_t *out_len);
 
 /* Message of the calling thread's last failure. */
 int rio_last_error(char *buf, size_t n);
