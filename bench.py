@@ -63,6 +63,10 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--policies", default="exact,learned,prevframe")
+    ap.add_argument("--resolve-mode", type=int, default=0,
+                    help="resolve launch form: 0 one workgroup per tile, 1 persistent, 2 persistent with shading")
+    ap.add_argument("--no-resolve-ab", dest="resolve_ab", action="store_false",
+                    help="skip timing the three resolve launch forms")
     return ap.parse_args()
 
 
@@ -159,6 +163,22 @@ def run_policies(R, r, w, train_path, test_path, policies, args):
             select(p)
             rounds[p].append(time_frames(R, r, test_path, w.opts, code[p], args.warmup, args.steps,
                                          graph=not args.no_graph))
+    resolve_ab = None
+    if getattr(args, "resolve_ab", False) and policies:
+        # the resolve stage's launch forms on the headline policy, interleaved rounds
+        p = "learned" if "learned" in policies else policies[0]
+        select(p)
+        names = {R.RESOLVE_PER_TILE: "per_tile", R.RESOLVE_PERSISTENT: "persistent",
+                 R.RESOLVE_PERSISTENT_SHADE: "persistent_shade"}
+        ab = {m: [] for m in names}
+        for _ in range(ROUNDS):
+            for m in names:
+                r.set_resolve_mode(m)
+                ab[m].append(time_frames(R, r, test_path, w.opts, code[p], args.warmup, args.steps,
+                                         graph=not args.no_graph))
+        r.set_resolve_mode(args.resolve_mode)
+        resolve_ab = {"policy": p, **{names[m]: round(sorted(v)[len(v) // 2] * 1e3 / args.steps, 4)
+                                      for m, v in ab.items()}}
     for p in policies:
         select(p)
         stages, extra = stage_profile(R, r, test_path, w.opts, code[p], args.warmup, args.steps)
@@ -168,6 +188,8 @@ def run_policies(R, r, w, train_path, test_path, policies, args):
                       "stages_ms": {k: round(v, 4) for k, v in stages.items()}, **extra}
     if model is not None:
         model.close()
+    if resolve_ab:
+        results["_resolve_modes"] = resolve_ab
     return results, train_s, {p: rounds[p][-1] for p in policies}
 
 
@@ -184,17 +206,31 @@ def batch_workload(R, S, autotune, args):
         r.add_scene(j.scene)
         rs.append(r)
         spec.append((r, j.camera, j.opts, R.BIN_EXACT, 0))
-    for _ in range(max(args.warmup, 2)):
-        R.render_batch(spec)
-    for r in rs:
-        r.sync()
     steps = max(args.steps, 10)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        R.render_batch(spec)
-    for r in rs:
-        r.sync()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
+
+    def time_batch():
+        for _ in range(max(args.warmup, 2)):
+            R.render_batch(spec)
+        for r in rs:
+            r.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            R.render_batch(spec)
+        for r in rs:
+            r.sync()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    # the resolve stage's launch forms (a batch takes its first job's), interleaved rounds
+    forms = {R.RESOLVE_PER_TILE: "per_tile", R.RESOLVE_PERSISTENT: "persistent",
+             R.RESOLVE_PERSISTENT_SHADE: "persistent_shade"}
+    times = {m: [] for m in forms}
+    for _ in range(ROUNDS):
+        for m in forms:
+            rs[0].set_resolve_mode(m)
+            times[m].append(time_batch())
+    med = {m: sorted(v)[len(v) // 2] for m, v in times.items()}
+    rs[0].set_resolve_mode(args.resolve_mode)
+    ms = med[args.resolve_mode]
     for r in rs:
         r.close()
     tris = sum(j.scene.num_triangles for j in jobs)
@@ -202,6 +238,8 @@ def batch_workload(R, S, autotune, args):
     return {"workload": "256 jobs, 64..1024 px per axis (log-uniform), icospheres and terrains of 1k..200k tris",
             "jobs": len(jobs), "triangles": tris, "pixels": px, "ms_per_batch": round(ms, 4),
             "mtris_per_s": round(tris / (ms * 1e-3) / 1e6, 2), "gpix_per_s": round(px / (ms * 1e-3) / 1e9, 3),
+            "resolve_form": forms[args.resolve_mode],
+            "resolve_forms_ms_per_batch": {forms[m]: round(v, 4) for m, v in med.items()},
             "tuning": "per job-size bucket from tuning.json" if table else "defaults"}
 
 
@@ -405,9 +443,11 @@ def main():
     args.queue = queue_of(autotune, args, w.width, w.height, ntri)
     r = R.Renderer(w.width, w.height)
     tuned.apply(r)
+    r.set_resolve_mode(args.resolve_mode)
     r.add_scene(w.scene)
     policies = [p for p in args.policies.split(",") if p]
     results, train_s, _ = run_policies(R, r, w, train_path, test_path, policies, args)
+    resolve_modes = results.pop("_resolve_modes", None)
     r.close()
 
     plain = None
@@ -417,7 +457,8 @@ def main():
         r2 = R.Renderer(w.width, w.height)
         pp.apply(r2)
         r2.add_scene(w.scene)
-        res2, _, _ = run_policies(R, r2, w, train_path, test_path, ["exact", "learned"], args)
+        res2, _, _ = run_policies(R, r2, w, train_path, test_path, ["exact", "learned"],
+                                  argparse.Namespace(**{**vars(args), "resolve_ab": False}))
         r2.close()
         plain = {"params": vars(pp), "exact_ms_per_frame": round(res2["exact"]["ms_per_frame"], 4),
                  "learned_ms_per_frame": round(res2["learned"]["ms_per_frame"], 4),
@@ -469,6 +510,7 @@ def main():
         "gpix_per_s": round(P / (ms * 1e-3) / 1e9, 3),
         "policies": {k: {"mtris_per_s": round(ntri / (v["ms_per_frame"] * 1e-3) / 1e6, 2), **v}
                      for k, v in results.items()},
+        "resolve_modes_ms_per_frame": resolve_modes,
         "plain_binner": plain,
         "batched": batch,
         "flythrough": fly,

@@ -376,6 +376,15 @@ extern "C" int rast_set_tuning(rast_renderer *r, uint32_t micro_limit, uint32_t 
     return RAST_OK;
 }
 
+extern "C" int rast_set_resolve_mode(rast_renderer *r, int mode)
+{
+    if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_set_resolve_mode: null renderer");
+    if (mode < RAST_RESOLVE_PER_TILE || mode > RAST_RESOLVE_PERSISTENT_SHADE)
+        return rast_set_error(RAST_E_INVALID_ARG, "resolve mode %d not in [0, 2]", mode);
+    r->resolve_mode = mode;
+    return RAST_OK;
+}
+
 extern "C" int rast_resize(rast_renderer *r, uint32_t width, uint32_t height)
 {
     if (!r) return rast_set_error(RAST_E_INVALID_ARG, "rast_resize: null renderer");
@@ -723,6 +732,7 @@ static uint64_t plan_key(rast_renderer *const *rs, uint32_t n, int hidden)
     auto mix = [&h](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
     mix(n);
     mix((uint64_t)hidden);
+    mix((uint64_t)(n ? rs[0]->resolve_mode : 0));
     // which entries share a source mesh (vertex groups): only views share meshes
     bool views = false;
     for (uint32_t j = 0; j < n; ++j) views = views || rs[j]->scene_src != nullptr;
@@ -925,8 +935,15 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
             off += tables[st].size();
         }
     }
-    // staging block: JobDesc[n] then DevMesh[meshes]
-    const size_t bytes = (size_t)n * sizeof(JobDesc) + (size_t)P.nmeshes * sizeof(DevMesh);
+    // resolve form: the persistent kernels are sized to what the device holds
+    L.resolve_mode = n ? rs[0]->resolve_mode : 0;
+    L.fused_shade = 0;
+    if (L.resolve_mode == RAST_RESOLVE_PERSISTENT_SHADE && (L.shade_modes == 1u || L.shade_modes == 2u))
+        L.fused_shade = (int)L.shade_modes;
+    for (int st : { ST_RES8, ST_RES16, ST_RES32 })
+        L.resident[st] = L.resolve_mode && L.grid[st] ? (uint32_t)resolve_resident_blocks(st, L.fused_shade) : 0u;
+    // staging block: the work counters (zeros), JobDesc[n], then DevMesh[meshes]
+    const size_t bytes = kStageCounterBytes + (size_t)n * sizeof(JobDesc) + (size_t)P.nmeshes * sizeof(DevMesh);
     if (bytes > P.stage_bytes || !P.h_stage[0] || !P.h_stage[1]) {
         for (int i = 0; i < 2; ++i) {
             if (P.h_stage[i]) (void)hipHostFree(P.h_stage[i]);
@@ -936,8 +953,9 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     }
     RAST_TRY(dev_alloc(P.d_stage, bytes));
     P.stage_bytes = bytes;
-    L.jobs = (const JobDesc *)P.d_stage.p;
-    L.meshes = (const DevMesh *)((char *)P.d_stage.p + (size_t)n * sizeof(JobDesc));
+    L.work = (uint32_t *)P.d_stage.p;
+    L.jobs = (const JobDesc *)((char *)P.d_stage.p + kStageCounterBytes);
+    L.meshes = (const DevMesh *)((char *)P.d_stage.p + kStageCounterBytes + (size_t)n * sizeof(JobDesc));
     P.key = key;
     P.built = true;
     return RAST_OK;
@@ -950,8 +968,9 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
 {
     RAST_TRY(plan_wait_block(P, P.parity));
     void *h = P.h_stage[P.parity];
-    JobDesc *jd = (JobDesc *)h;
-    DevMesh *dm = (DevMesh *)((char *)h + (size_t)n * sizeof(JobDesc));
+    std::memset(h, 0, kStageCounterBytes);  // the launch's work counters start at zero
+    JobDesc *jd = (JobDesc *)((char *)h + kStageCounterBytes);
+    DevMesh *dm = (DevMesh *)((char *)h + kStageCounterBytes + (size_t)n * sizeof(JobDesc));
     uint32_t me = 0;
     for (uint32_t j = 0; j < n; ++j) {
         rast_renderer *r = rs[j];
@@ -1022,8 +1041,8 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
         J.vis = (uint32_t *)r->vis.p;
         J.depth = (float *)r->depth.p;
         J.rgba = (uint32_t *)r->rgba.p;
-        const uint32_t *dflt = (const uint32_t *)((const char *)P.d_stage.p + (size_t)j * sizeof(JobDesc) +
-                                                  offsetof(JobDesc, defaults));
+        const uint32_t *dflt = (const uint32_t *)((const char *)P.d_stage.p + kStageCounterBytes +
+                                                  (size_t)j * sizeof(JobDesc) + offsetof(JobDesc, defaults));
         rast_frame_consts fc;
         camera_frame(r->last_cam, r->W, r->H, fc);
         uint32_t vb = 0;

@@ -1163,12 +1163,18 @@ __device__ __forceinline__ void bitonic_sort(uint64_t *v, uint32_t P, uint32_t t
 // pixels of the tile one pixel per lane, the others as one compacted item
 // list (prefix sum over their clipped pixel ranges, LDS atomic min).  The few
 // screen-list triangles are then tested by every lane on its own pixels, with
-// a conservative per-pixel depth reject before the exact test.
-template <int T, int NT, int STAGE>
-__global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table)
+// a conservative per-pixel depth reject before the exact test.  SHADE != 0
+// (1 point, 2 linear sampling): the tile's pixels are shaded right after,
+// from the visibility the lanes hold (jobs with that sampling mode and a
+// shading stage).
+template <bool kLinear>
+__device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
+                                            uint32_t x, uint32_t y, uint32_t id);
+
+template <int T, int NT, int STAGE, int SHADE>
+__device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, const DevMesh *__restrict__ meshes,
+                                             const float *unit)
 {
-    uint32_t local;
-    const JobDesc &J = jobs[job_of(table, jobs, STAGE, local)];
     const FrameParams &fp = J.fp;
     const uint32_t *__restrict__ tile_count = J.tile_count;
     const uint32_t *__restrict__ tile_off = J.tile_off;
@@ -1203,11 +1209,7 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
     __shared__ uint32_t ovf_base, ovf_fill;
 
     const uint32_t ntiles = fp.tiles_x * fp.tiles_y;
-    // Tiles in launch order: the hardware spreads consecutive workgroups over
-    // the 8 XCDs, so neighbouring tiles run on different XCDs at once.  Giving
-    // each XCD a contiguous band instead measured 30% slower here.
-    const uint32_t tile = local;
-    if (tile >= ntiles) return;
+    if (tile >= ntiles) return;  // the same for the whole block
     const uint32_t txi = tile % fp.tiles_x, tyi = tile / fp.tiles_x;
     const int X0 = (int)(txi * T), Y0 = (int)(tyi * T);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -1557,6 +1559,26 @@ __global__ __launch_bounds__(NT) void k_resolve(const JobDesc *__restrict__ jobs
             stg(tile_min_depth + tile, m);
         }
     }
+    if constexpr (SHADE != 0) {
+        if (J.nblocks[ST_SHADE] && (fp.sample_state != 0) == (SHADE == 2)) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q)
+                if (inside[q]) shade_pixel<SHADE == 2>(meshes, J, unit, (uint32_t)pxs[q], (uint32_t)pys[q],
+                                                       ~(uint32_t)best[q] - 1u);
+        }
+    }
+}
+
+// One workgroup per tile, in launch order: the hardware spreads consecutive
+// workgroups over the 8 XCDs, so neighbouring tiles run on different XCDs at
+// once (giving each XCD a contiguous band instead measured 30% slower).
+template <int T, int NT, int STAGE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_resolve(
+    const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table)
+{
+    uint32_t local;
+    const JobDesc &J = jobs[job_of(table, jobs, STAGE, local)];
+    resolve_tile<T, NT, STAGE, 0>(J, local, nullptr, nullptr);
 }
 
 // ------------------------------------------------------------- shading
@@ -1987,6 +2009,48 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
         if (y0 + 4u * k < fp.height) shade_pixel<kLinear>(meshes, J, unit, x, y0 + 4u * k, ids[k]);
 }
 
+// The persistent form of the resolve stage (the reference's TODO at
+// CCR/CUDARenderer.cu:284-285): a grid sized to the workgroups the GPU holds
+// at once, each taking the next (job, tile) of the stage from one counter
+// over all jobs of the launch until none is left; with SHADE, each tile is
+// shaded by the workgroup that resolved it (no visibility re-read, no
+// shading launch).  The counter is zeroed by the launch's staging upload.
+// Every workgroup leaves once the counter has passed the item count, which
+// each fetch increments.
+template <int T, int NT, int STAGE, int SHADE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADE ? 3 : 4))) void k_resolve_persistent(
+                                                           const DevMesh *__restrict__ meshes,
+                                                           const JobDesc *__restrict__ jobs,
+                                                           const uint16_t *__restrict__ table,
+                                                           uint32_t *__restrict__ work, uint32_t items)
+{
+    __shared__ uint32_t item_s;
+    __shared__ float unit[SHADE ? 256 : 1];
+    if constexpr (SHADE != 0) {
+        for (int c = (int)threadIdx.x; c < 256; c += NT) unit[c] = byte_unit((uint32_t)c);
+    }
+    for (;;) {
+        // The barrier first: every lane has finished the previous tile (and
+        // read item_s) before lane 0 fetches.  It also keeps the compiler
+        // from merging the fetch with the lane-0 store that ends the previous
+        // tile: merged, the loop is structurized so that the other lanes run
+        // ahead into the barrier below while lane 0 waits for its fetch, and
+        // they read the old item again, without end.
+        __syncthreads();
+        if (threadIdx.x == 0) item_s = atomicAdd(work, 1u);
+        __syncthreads();
+        // the same for every lane, and readfirstlane says so: a scalar exit
+        const uint32_t item = __builtin_amdgcn_readfirstlane(item_s);
+        if (item >= items) break;
+        uint32_t j = 0, local = item;
+        if (table) {
+            j = __builtin_amdgcn_readfirstlane((uint32_t)table[item]);
+            local = item - jobs[j].block0[STAGE];
+        }
+        resolve_tile<T, NT, STAGE, SHADE>(jobs[j], local, meshes, unit);
+    }
+}
+
 // DrawTextureKernel / DrawTextureGlobalKernel (CCR/CUDARenderer.cu:1565-1617):
 // one texture over the whole frame, one pixel per thread (vis none, depth 1).
 __global__ __launch_bounds__(256) void k_draw_texture(uint32_t *__restrict__ vis, float *__restrict__ depth,
@@ -2044,11 +2108,11 @@ __global__ __launch_bounds__(256) void k_draw_texture(uint32_t *__restrict__ vis
 // Copies the counters of every job that is not shaded to its renderer's
 // pinned host block (one workgroup per job), replacing a device-to-host copy
 // per job; the shading kernel publishes the others.
-__global__ __launch_bounds__(256) void k_publish(const JobDesc *__restrict__ jobs)
+__global__ __launch_bounds__(256) void k_publish(const JobDesc *__restrict__ jobs, int all)
 {
     __shared__ uint32_t red[4];
     const JobDesc &J = jobs[blockIdx.x];
-    if (J.nblocks[ST_SHADE]) return;
+    if (J.nblocks[ST_SHADE] && !all) return;
     publish_counters<4>(J, red);
 }
 
@@ -2139,10 +2203,65 @@ hipError_t launch_fallback(const Launch &L, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int T, int NT, int STAGE>
+static hipError_t launch_resolve_persistent(const Launch &L, hipStream_t s)
+{
+    const uint32_t items = L.grid[STAGE];
+    const uint32_t g = items < L.resident[STAGE] ? items : L.resident[STAGE];
+    uint32_t *work = L.work + STAGE;
+    switch (L.fused_shade) {
+    case 1:
+        hipLaunchKernelGGL((k_resolve_persistent<T, NT, STAGE, 1>), dim3(g), dim3(NT), 0, s, L.meshes, L.jobs,
+                           L.table[STAGE], work, items);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_resolve_persistent<T, NT, STAGE, 2>), dim3(g), dim3(NT), 0, s, L.meshes, L.jobs,
+                           L.table[STAGE], work, items);
+        break;
+    default:
+        hipLaunchKernelGGL((k_resolve_persistent<T, NT, STAGE, 0>), dim3(g), dim3(NT), 0, s, L.meshes, L.jobs,
+                           L.table[STAGE], work, items);
+        break;
+    }
+    return hipGetLastError();
+}
+
+int resolve_resident_blocks(int stage, int fused_shade)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    hipError_t e = hipErrorInvalidValue;
+#define RAST_OCC(T, NT, ST)                                                                                      \
+    switch (fused_shade) {                                                                                     \
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resolve_persistent<T, NT, ST, 1>, NT, 0); break; \
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resolve_persistent<T, NT, ST, 2>, NT, 0); break; \
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resolve_persistent<T, NT, ST, 0>, NT, 0); break; \
+    }
+    switch (stage) {
+    case ST_RES8: RAST_OCC(8, 64, ST_RES8); break;
+    case ST_RES16: RAST_OCC(16, 256, ST_RES16); break;
+    case ST_RES32: RAST_OCC(32, 256, ST_RES32); break;
+    default: return 0;
+    }
+#undef RAST_OCC
+    if (e != hipSuccess || per_cu <= 0) return 0;
+    return per_cu * cus;
+}
+
 hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s)
 {
     const uint32_t g = L.grid[stage];
     if (!g) return hipSuccess;
+    if (L.resolve_mode != 0 && L.resident[stage]) {
+        switch (stage) {
+        case ST_RES8: return launch_resolve_persistent<8, 64, ST_RES8>(L, s);
+        case ST_RES16: return launch_resolve_persistent<16, 256, ST_RES16>(L, s);
+        case ST_RES32: return launch_resolve_persistent<32, 256, ST_RES32>(L, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (stage) {
     case ST_RES8: hipLaunchKernelGGL((k_resolve<8, 64, ST_RES8>), dim3(g), dim3(64), 0, s, L.jobs, L.table[ST_RES8]); break;
     case ST_RES16:
@@ -2158,7 +2277,7 @@ hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s)
 
 hipError_t launch_shade(const Launch &L, hipStream_t s)
 {
-    if (!L.grid[ST_SHADE]) return hipSuccess;
+    if (!L.grid[ST_SHADE] || L.fused_shade) return hipSuccess;
     // one instance per sampling mode present: a mode's code alone keeps the
     // register count (and so the waves per SIMD) of that instance
     if (L.shade_modes & 1u)
@@ -2179,8 +2298,10 @@ hipError_t launch_draw_texture(uint32_t *vis, float *depth, uint32_t *rgba, uint
 
 hipError_t launch_publish(const Launch &L, hipStream_t s)
 {
-    if (L.unshaded_jobs == 0) return hipSuccess;  // every job's shading kernel publishes
-    hipLaunchKernelGGL(k_publish, dim3(L.njobs), dim3(256), 0, s, L.jobs);
+    // shading in the resolve pass: every job's counters go out here
+    const int all = L.fused_shade != 0;
+    if (L.unshaded_jobs == 0 && !all) return hipSuccess;  // every job's shading kernel publishes
+    hipLaunchKernelGGL(k_publish, dim3(L.njobs), dim3(256), 0, s, L.jobs, all);
     return hipGetLastError();
 }
 

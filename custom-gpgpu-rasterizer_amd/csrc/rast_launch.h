@@ -28,7 +28,16 @@ struct Launch {
     int model_hidden;                 // hidden width of the learned jobs' MLP (16, 32 or 64; 0: none)
     uint32_t shade_modes;             // sampling modes of the shading jobs: bit 0 point, bit 1 linear
     uint32_t unshaded_jobs;           // jobs without a shading stage (their counters go out with k_publish)
+    // resolve stage: 0 one workgroup per tile; 1 persistent workgroups taking
+    // tiles from a counter; 2 the same, shading each tile as it is resolved
+    int resolve_mode;
+    int fused_shade;                  // 0, or the sampling mode shaded in the resolve pass (1 point, 2 linear)
+    uint32_t resident[ST_COUNT];      // persistent resolve: workgroups the GPU holds at once, per stage
+    uint32_t *work;                   // device: per-stage work counters, zeroed by the staging upload
 };
+
+// Staging bytes in front of the job table: the launch's work counters.
+constexpr size_t kStageCounterBytes = 64;
 
 // Resets a key buffer to depth 1 / no triangle (before a renderer's first frame).
 hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s);
@@ -43,6 +52,9 @@ hipError_t launch_setup(const Launch &L, hipStream_t s);
 hipError_t launch_fill(const Launch &L, hipStream_t s);
 // One tile-size group (ST_RES8 / ST_RES16 / ST_RES32); the groups are independent.
 hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s);
+// Workgroups of the persistent resolve kernel of a stage the current device
+// holds at once (0 on failure).
+int resolve_resident_blocks(int stage, int fused_shade);
 hipError_t launch_shade(const Launch &L, hipStream_t s);
 hipError_t launch_publish(const Launch &L, hipStream_t s);
 // Placement of the previous-frame policy's entries beyond capacity.

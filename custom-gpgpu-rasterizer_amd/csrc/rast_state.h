@@ -105,6 +105,7 @@ struct rast_renderer {
     hipStream_t stream = nullptr;
     uint32_t W = 0, H = 0;
     uint32_t tile = 16, big_threshold = 256, small_limit = 16;
+    int resolve_mode = 0;                  // RAST_RESOLVE_*
     std::vector<rast_mesh *> meshes;
     std::vector<rast_texture *> textures;
     std::vector<TexQuad> tex_quads;        // interleaved map sets of this renderer's textures

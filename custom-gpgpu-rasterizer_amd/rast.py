@@ -25,11 +25,12 @@ VIS_NONE = 0xFFFFFFFF
 BIN_EXACT, BIN_LEARNED, BIN_PREVFRAME = 0, 1, 2
 DRAW_TEXTURE, DRAW_TEXTURE_GLOBAL = 0, 1
 FLAG_TIMING, FLAG_NO_SHADE, FLAG_GRAPH = 1, 2, 4
+RESOLVE_PER_TILE, RESOLVE_PERSISTENT, RESOLVE_PERSISTENT_SHADE = 0, 1, 2
 DEBUG_VERTEX_POS, DEBUG_VERTEX_ATTR, DEBUG_TRI_RECORDS, DEBUG_TRI_ALIVE, DEBUG_TILE_COUNTS, DEBUG_BIG_LIST, DEBUG_TILE_CAPS = range(7)
 
 EXPORTS = [
     "rast_last_error", "rast_device_query", "rast_renderer_create", "rast_renderer_destroy", "rast_setup_bins",
-    "rast_resize", "rast_set_tuning", "rast_mesh_create", "rast_mesh_destroy", "rast_mesh_set_world", "rast_mesh_set_textures",
+    "rast_resize", "rast_set_tuning", "rast_set_resolve_mode", "rast_mesh_create", "rast_mesh_destroy", "rast_mesh_set_world", "rast_mesh_set_textures",
     "rast_mesh_num_triangles", "rast_mesh_triangle_base", "rast_texture_create", "rast_texture_destroy",
     "rast_camera_frame", "rast_render", "rast_sync", "rast_readback", "rast_get_stats", "rast_frame_buffers",
     "rast_debug_read", "rast_binmodel_create", "rast_binmodel_destroy", "rast_binmodel_num_features",
@@ -99,6 +100,7 @@ def lib():
         L.rast_setup_bins.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.rast_resize.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.rast_set_tuning.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.rast_set_resolve_mode.argtypes = [vp, C.c_int]
         L.rast_mesh_create.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.c_int, vp, C.POINTER(vp)]
         L.rast_mesh_destroy.argtypes = [vp]
         L.rast_mesh_set_world.argtypes = [vp, vp]
@@ -204,6 +206,10 @@ class Renderer:
 
     def set_tuning(self, micro_limit: int = 16, small_limit: int = 16) -> None:
         check(lib().rast_set_tuning(self.h, micro_limit, small_limit), "rast_set_tuning")
+
+    def set_resolve_mode(self, mode: int) -> None:
+        """RESOLVE_PER_TILE, RESOLVE_PERSISTENT or RESOLVE_PERSISTENT_SHADE."""
+        check(lib().rast_set_resolve_mode(self.h, mode), "rast_set_resolve_mode")
 
     def set_queue_depth(self, margin: float = 1.0, slack: int = 0) -> None:
         """Queue depth of the previous-frame heuristic, ceil(prev * margin) +

@@ -175,6 +175,16 @@ int rast_setup_bins(rast_renderer *r, uint32_t tile_size, uint32_t big_threshold
  * per lane, larger ones one pixel per lane.  Defaults 16 / 16.  The output
  * does not depend on either value. */
 int rast_set_tuning(rast_renderer *r, uint32_t micro_limit, uint32_t small_limit);
+/* Resolve-stage launch form (the persistent kernel of the reference's TODO,
+ * CCR/CUDARenderer.cu:284-285): RAST_RESOLVE_PER_TILE (default) one
+ * workgroup per tile; RAST_RESOLVE_PERSISTENT workgroups as many as the GPU
+ * holds, each taking the next tile of any job from one counter;
+ * RAST_RESOLVE_PERSISTENT_SHADE the same, each tile shaded by the workgroup
+ * that resolved it (when the launch's shaded frames share one sampling mode;
+ * otherwise shading stays a pass of its own).  A batch takes its first
+ * job's form.  The output does not depend on it. */
+enum { RAST_RESOLVE_PER_TILE = 0, RAST_RESOLVE_PERSISTENT = 1, RAST_RESOLVE_PERSISTENT_SHADE = 2 };
+int rast_set_resolve_mode(rast_renderer *r, int mode);
 /* Resize the framebuffer (a new Resolution in CCR/Resolution.cpp). */
 int rast_resize(rast_renderer *r, uint32_t width, uint32_t height);
 

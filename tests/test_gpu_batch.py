@@ -200,3 +200,52 @@ def test_back_to_back_launches_keep_their_own_staging():
     finally:
         for r in A + B:
             r.close()
+
+
+@pytest.mark.parametrize("mode", [R.RESOLVE_PERSISTENT, R.RESOLVE_PERSISTENT_SHADE])
+def test_persistent_resolve_gives_the_per_tile_frames(mode):
+    """The persistent resolve kernels (one counter over every job's tiles;
+    with shading in the resolve pass) render the frames of the one-workgroup-
+    per-tile launch, bit for bit: a batch of uneven jobs of all three tile
+    sizes (both sampling modes: shading stays a pass of its own) and of one
+    sampling mode (shaded in the resolve pass), then single frames of the
+    terrain with every binning policy over a moving camera."""
+    for mixed in (True, False):
+        rs, jobs = batch_jobs(1, 9)
+        try:
+            if not mixed:
+                jobs = [(r, c, S.Opts(cull_mode=o.cull_mode, sample_state=S.SAMPLE_LINEAR), p, f)
+                        for r, c, o, p, f in jobs]
+            R.render_batch(jobs)
+            want = [r.readback() for r, *_ in jobs]
+            jobs[0][0].set_resolve_mode(mode)  # a batch takes its first job's form
+            for k in range(2):
+                R.render_batch(jobs)
+                for (r, *_), w in zip(jobs, want):
+                    same_frames(r.readback(), w, f"mixed={mixed} batch {k}, {r.width}x{r.height}")
+        finally:
+            for r, *_ in jobs:
+                r.close()
+            rs[0].close()
+    w = small_terrain()
+    _, path = S.train_test_paths(2, 4)
+    for tile in (8, 16, 32):
+        with R.Renderer(w.width, w.height, tile=tile) as a, R.Renderer(w.width, w.height, tile=tile) as b:
+            a.add_scene(w.scene)
+            b.add_scene(w.scene)
+            b.set_resolve_mode(mode)
+            models = []
+            for r in (a, b):
+                m = R.BinModel(r, hidden=16, seed=5)
+                m.attach(r, 1.1, 2)
+                models.append(m)
+            for policy in (R.BIN_EXACT, R.BIN_LEARNED, R.BIN_PREVFRAME):
+                for i, cam in enumerate(path):
+                    a.render(cam, w.opts, policy)
+                    b.render(cam, w.opts, policy, R.FLAG_GRAPH if i % 2 else 0)
+                    same_frames(a.readback(), b.readback(), f"tile {tile} policy {policy} frame {i}")
+                    sa, sb = a.stats(), b.stats()
+                    for key in ("num_visible", "num_bin_pairs", "overflow_tiles"):
+                        assert sa[key] == sb[key], (tile, policy, i, key)
+            for m in models:
+                m.close()
