@@ -13,8 +13,9 @@
 // getters the reference calls (GetMeshes, GetVertexBuffer, GetWorldConst,
 // GetPosition, GetForward, ...), so a reference caller keeps its own types.
 // Differences: errors throw rast::Error (the reference printed and went on,
-// CCR/PCH.h:46-56); textures are created from RGBA8 texels, not image files
-// (no image decoder is part of this library); Present() copies the frame into
+// CCR/PCH.h:46-56); textures are created from a PNG file (decoded by
+// librast_io, include/rast_io.h: link -lrast_io when using the path forms)
+// or from RGBA8 texels; Present() copies the frame into
 // WindowHelper::pBackBufferPixels instead of an SDL surface.
 //
 // The classes live in namespace rast; define RAST_REFERENCE_NAMES before
@@ -32,6 +33,7 @@
 #include <vector>
 
 #include "rast.h"
+#include "rast_io.h"
 
 namespace rast {
 
@@ -88,6 +90,8 @@ class CUDATexture {
 public:
     CUDATexture() = default;
     CUDATexture(rast_renderer *r, const uint32_t *rgba8, uint32_t width, uint32_t height) { Create(r, rgba8, width, height); }
+    // CUDATexture(const std::string& path) (CCR/CUDATexture.cpp:13-17)
+    CUDATexture(rast_renderer *r, const std::string &path) { Create(r, path.c_str()); }
     virtual ~CUDATexture() noexcept { Destroy(); }
     CUDATexture(const CUDATexture &) = delete;
     CUDATexture &operator=(const CUDATexture &) = delete;
@@ -99,6 +103,24 @@ public:
         check(rast_texture_create(r, rgba8, width, height, &tex_), "rast_texture_create");
         width_ = width;
         height_ = height;
+    }
+    // Create(const char* path) (CCR/CUDATexture.cpp:24-45): IMG_Load of a
+    // PNG, then the upload; false (nothing allocated) if the file cannot be
+    // read or decoded, as the reference returns false when IMG_Load fails.
+    bool Create(rast_renderer *r, const char *path)
+    {
+        uint32_t *texels = nullptr, w = 0, h = 0;
+        if (rio_png_load(path, &texels, &w, &h) != RAST_OK) return false;
+        Destroy();
+        const int rc = rast_texture_create(r, texels, w, h, &tex_);
+        rio_free(texels);
+        if (rc != RAST_OK) {
+            tex_ = nullptr;
+            return false;
+        }
+        width_ = w;
+        height_ = h;
+        return true;
     }
     virtual void Destroy()
     {
@@ -279,6 +301,14 @@ public:
     {
         return new CUDATexture(r_, rgba8, width, height);
     }
+    // A texture from a PNG file (the reference's CUDATexture(path)); null if
+    // the file cannot be read or decoded.
+    CUDATexture *CreateTexture(const std::string &path)
+    {
+        std::unique_ptr<CUDATexture> t(new CUDATexture());
+        if (!t->Create(r_, path.c_str())) return nullptr;
+        return t.release();
+    }
 
     // SetupBins(numBins, binDim, binQueueMaxSize) (CCR/CUDARenderer.cu:200-205):
     // square tiles of binDim (8, 16 or 32 pixels); the queue depth is sized
@@ -364,9 +394,9 @@ public:
     CUDABenchMarker &GetBenchMarker() { return bench_; }
 
     // Test hooks DrawTexture / DrawTextureGlobal (CCR/CUDARenderer.cu:1969-2012):
-    // the frame becomes the texture, then Present().  The reference loads the
-    // texture from an image path; here it is an existing CUDATexture (the
-    // path overloads need an image decoder, see rast_io.h).
+    // the frame becomes the texture, then Present().  The reference takes an
+    // image path (the char* overloads below: the PNG is loaded, drawn and
+    // released); the CUDATexture overloads draw a texture already made.
     void DrawTexture(const CUDATexture &tex)
     {
         check(rast_draw_texture(r_, tex.GetHandle(), RAST_DRAW_TEXTURE, 1, RAST_SAMPLE_POINT), "rast_draw_texture");
@@ -379,6 +409,12 @@ public:
                                 enum_value(sampleState)),
               "rast_draw_texture");
         Present();
+    }
+    void DrawTexture(const char *tp) { DrawTexture(*LoadForDraw(tp)); }
+    template <class SampleStateT = int>
+    void DrawTextureGlobal(const char *tp, bool isStretchedToWindow = true, SampleStateT sampleState = SampleStateT(0))
+    {
+        DrawTextureGlobal(*LoadForDraw(tp), isStretchedToWindow, sampleState);
     }
 
     rast_renderer *GetHandle() const { return r_; }
@@ -422,6 +458,16 @@ public:
     }
 
 private:
+    // the char* draw hooks' texture: the reference drew whatever Create left
+    // (nothing, for a missing file); here a file that does not load throws
+    std::unique_ptr<CUDATexture> LoadForDraw(const char *tp)
+    {
+        std::unique_ptr<CUDATexture> t(new CUDATexture());
+        if (!tp || !t->Create(r_, tp))
+            throw Error(RAST_E_INVALID_ARG, std::string("cannot load texture ") + (tp ? tp : "(null)"));
+        return t;
+    }
+
     rast_renderer *r_ = nullptr;
     uint32_t width_ = 0, height_ = 0;
     uint32_t *back_buffer_ = nullptr;

@@ -3,7 +3,9 @@
 // with app-side types shaped like the reference's (Mesh, SceneGraph,
 // SceneManager, Camera, WindowHelper), on the 640x480 UV-sphere frame.
 // The frame is checked against the CPU oracle: visibility and depth bit for
-// bit, RGBA8 within 1/255 per channel.  Exit status 0 = pass.
+// bit, RGBA8 within 1/255 per channel; a texture loaded from the PNG fixture
+// tests/golden/uv_grid.png (argv[1]: the repository root) is drawn and
+// checked texel by texel.  Exit status 0 = pass.
 #define RAST_REFERENCE_NAMES
 #include "rast.hpp"
 
@@ -92,8 +94,9 @@ static uint32_t rgba(int r, int g, int b) { return (uint32_t)r | (uint32_t)g << 
         }                                        \
     } while (0)
 
-int main()
+int main(int argc, char **argv)
 {
+    const std::string root = argc > 1 ? argv[1] : ".";  // the repository, for the PNG fixture
     const uint32_t W = 640, H = 480;
     uint32_t nv = 0, ni = 0;
     rsc_uv_sphere(64, 32, 1.f, nullptr, &nv, nullptr, &ni);
@@ -152,11 +155,54 @@ int main()
         // DrawTextureGlobal: the checker stretched over the window (point sampling)
         renderer.DrawTextureGlobal(*tm.GetCUDATexture(0), true, app::SampleState::Point);
         drawn = back;
+        // a texture from a PNG file, as the reference's CUDATexture(path)
+        // loads its maps: the uv grid fixture is the 32-pixel checker
+        const std::string grid = root + "/tests/golden/uv_grid.png";
+        EXPECT(tm.AddCUDATexture(renderer.CreateTexture(grid)) == 1, "png texture id");
+        const CUDATexture *png = tm.GetCUDATexture(1);
+        EXPECT(png && png->GetWidth() == 512 && png->GetHeight() == 512, "png texture size");
+        EXPECT(renderer.CreateTexture(root + "/tests/golden/no_such.png") == nullptr,
+               "a missing file gives no texture");
+        renderer.DrawTextureGlobal(*png, false, app::SampleState::Point);
+        // the grid's texels by its rule (white where the 32-pixel cells'
+        // indices sum to an even number, black elsewhere), sampled by the
+        // oracle at uv = (x, y) / 512 (not stretched; magenta past u = 1)
+        std::vector<uint32_t> rule(512 * 512);
+        for (uint32_t y = 0; y < 512; ++y)
+            for (uint32_t x = 0; x < 512; ++x)
+                rule[(size_t)y * 512 + x] = ((x / 32 + y / 32) % 2) == 0 ? 0xFFFFFFFFu : 0xFF000000u;
+        const orc_texture grid_tex{ rule.data(), 512, 512 };
+        for (uint32_t y = 0; y < H; y += 9)
+            for (uint32_t x = 0; x < W; x += 7) {
+                const float uv[2] = { (float)x / 512.f, (float)y / 512.f };
+                float c[3];
+                orc_sample(&grid_tex, uv, ORC_SAMPLE_POINT, c);
+                const uint32_t want = orc_pack_rgba(c), got = back[(size_t)y * W + x];
+                EXPECT(got == want, "png grid at (%u, %u): %08x, want %08x", x, y, got, want);
+            }
+        // the reference's path form of the hook gives the same frame
+        const std::vector<uint32_t> from_texture = back;
+        renderer.DrawTextureGlobal(grid.c_str(), false, app::SampleState::Point);
+        EXPECT(back == from_texture, "DrawTextureGlobal(path) differs from DrawTextureGlobal(texture)");
+        renderer.DrawTexture(*png);
+        const std::vector<uint32_t> point_fetch = back;
+        renderer.DrawTexture(grid.c_str());
+        EXPECT(back == point_fetch, "DrawTexture(path) differs from DrawTexture(texture)");
+        EXPECT(point_fetch[0] == 0xFFFFFFFFu && point_fetch[50] == 0xFF000000u, "DrawTexture: %08x %08x",
+               point_fetch[0], point_fetch[50]);
+        threw = false;
+        try {
+            renderer.DrawTexture((root + "/tests/golden/no_such.png").c_str());
+        } catch (const rast::Error &e) {
+            threw = e.code() == RAST_E_INVALID_ARG;
+        }
+        EXPECT(threw, "DrawTexture of a missing file throws");
         delete pRenderer;
     }
-    // the texture handle outlived its renderer: destroying it now is safe
+    // the texture handles outlived their renderer: destroying them now is safe
     tm.RemoveCUDATexture(0);
     EXPECT(tm.GetCUDATexture(0) == nullptr, "texture removed");
+    tm.RemoveCUDATexture(1);
     for (uint32_t y = 0; y < H; y += 7)
         for (uint32_t x = 0; x < W; x += 5) {
             const orc_texture ot{ checker.data(), 256, 256 };
