@@ -287,6 +287,7 @@ __device__ __forceinline__ int clamp_trunc(float v, float hi)
 
 constexpr uint64_t kClearKey = ((uint64_t)0x3F800000u << 32) | 0xFFFFFFFFull;  // depth 1.0, no triangle
 constexpr int kMaskChunks = 64;  // 64-item chunks per start-mask window (one mask word per lane)
+constexpr uint32_t kFragRing = 128;  // setup's per-wave queue of covered micro-triangle pixels
 
 // Depth-test key: min z first, then max ID (low word = ~(id + 1)).
 __device__ __forceinline__ uint64_t make_key(float z, uint32_t id)
@@ -591,6 +592,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     __shared__ MicroLds mstage[256];
     __shared__ uint32_t moff[256];
     __shared__ unsigned long long mmask[4 * kMaskChunks];
+    __shared__ uint32_t mfrag[4 * kFragRing];
     // workgroups in launch order (consecutive blocks spread over the XCDs;
     // XCD-contiguous ranges measured slower)
     const uint32_t blk = blockIdx.x;
@@ -707,6 +709,10 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     // 64-bit atomic min.  The micro triangles are compacted in LDS; for every
     // 64-item chunk a 64-bit mask marks the items where a triangle starts, so
     // an item's triangle is the count of starts up to it (one popcount).
+    // Only about one item in ten is covered, so the edge test runs per item
+    // and the covered ones are queued in a per-wave ring in LDS (triangle,
+    // offset in its range); the depth (three divisions) and the key minimum
+    // run on 64 queued fragments at a time, with every lane busy.
     {
         const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
         const uint64_t mm = __ballot(micro);
@@ -735,6 +741,29 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                 wo[rank] = start;
             }
             uint32_t before = 0;  // triangles starting before the current chunk (wave-uniform)
+            uint32_t *ring = mfrag + wave * kFragRing;
+            uint32_t head = 0, queued = 0;  // wave-uniform
+            // depth test of ring entries [head, head + n): lane l takes entry head + l
+            auto drain = [&](uint32_t n) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane < n) {
+                    const uint32_t e = ring[(head + lane) & (kFragRing - 1u)];
+                    const MicroLds &mt = ws[e & 63u];
+                    const int px = (int)(mt.xr & 0xFFFFu) + (int)((e >> 6) & 0x3FFu);
+                    const int py = (int)(mt.yr & 0xFFFFu) + (int)(e >> 16);
+                    float w0, w1, w2, z;
+                    edges(mt, (float)px, (float)py, w0, w1, w2);
+                    if (depth_of(mt, w0, w1, w2, z))
+                        gatomic_min(keys + (size_t)py * fp.width + px, (unsigned long long)make_key(z, mt.id));
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                head += n;
+                queued -= n;
+            };
             // windows of kMaskChunks chunks (one pass unless micro_limit is large)
             for (uint32_t wbase = 0; wbase < total; wbase += 64u * kMaskChunks) {
                 const uint32_t nch = min((total - wbase + 63u) >> 6, (uint32_t)kMaskChunks);
@@ -751,17 +780,27 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                     const uint64_t m = wm[c];
                     const uint32_t kk = wbase + c * 64u + lane;
                     const uint32_t j = before + (uint32_t)__popcll(m & ((2ull << lane) - 1ull)) - 1u;
+                    bool covered = false;
+                    uint32_t entry = 0;
                     if (kk < total) {
                         const MicroLds &mt = ws[j];
                         int px, py;
                         item_pixel(mt, kk - wo[j], px, py);
-                        float z;
-                        if (fragment(mt, (float)px, (float)py, z))
-                            gatomic_min(keys + (size_t)py * fp.width + px, (unsigned long long)make_key(z, mt.id));
+                        float w0, w1, w2;
+                        edges(mt, (float)px, (float)py, w0, w1, w2);
+                        covered = w0 >= 0.f && w1 >= 0.f && w2 >= 0.f;
+                        // offsets in the range: below micro_limit <= 1024 each
+                        entry = j | ((uint32_t)(px - (int)(mt.xr & 0xFFFFu)) << 6) |
+                                ((uint32_t)(py - (int)(mt.yr & 0xFFFFu)) << 16);
                     }
+                    const uint64_t cm = __ballot(covered);
+                    if (covered) ring[(head + queued + popc_below(cm)) & (kFragRing - 1u)] = entry;
+                    queued += (uint32_t)__popcll(cm);
+                    if (queued >= 64u) drain(64u);
                     before += (uint32_t)__popcll(m);
                 }
             }
+            if (queued) drain(queued);
         }
     }
 
@@ -1198,12 +1237,14 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
     constexpr int PPT = NPIX / NT;
     constexpr int NW = NT / 64;
     static_assert(NPIX % NT == 0, "tile pixels must be a multiple of the workgroup size");
+    static_assert(T <= 256 && NT <= 256, "ring entries hold the staged triangle and the pixel in 8 bits each");
     __shared__ uint64_t key[NPIX];
     __shared__ TriLds stage[NT];
     __shared__ float4 bplane[NT];  // screen-list triangles: conservative depth plane over the tile
     __shared__ uint2 bbox[NT];     // and their pixel ranges
     __shared__ uint32_t off[NT];
     __shared__ uint32_t wlist[NT];
+    __shared__ uint32_t rring[NW * kFragRing];  // per wave: covered items waiting for their depth
     __shared__ uint32_t wred_u[NW], wwhole[NW];
     __shared__ float wdmax[NW];
     __shared__ uint32_t ovf_base, ovf_fill;
@@ -1353,17 +1394,54 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
             if (whole) wlist[pbefore + popc_below(wm)] = tid;
             __syncthreads();
             // compacted items; slots past this chunk's triangles start at
-            // nitems, so the owner search stops at the last staged one
+            // nitems, so the owner search stops at the last staged one.  The
+            // edge test runs per item; the covered items (about one in six)
+            // queue in the wave's ring (staged triangle, pixel in the tile),
+            // and the depth (three divisions) and key minimum run on 64 of
+            // them at a time, with every lane busy.
             const uint32_t n_here = n_tile - base < (uint32_t)NT ? n_tile - base : (uint32_t)NT;
-            for (uint32_t kk = tid; kk < nitems; kk += NT) {
-                const uint32_t j = find_item_owner(off, n_here, kk);
-                const TriLds &mt = stage[j];
-                int px, py;
-                item_pixel(mt, kk - off[j], px, py);
-                float z;
-                if (fragment(mt, (float)px, (float)py, z))
-                    atomicMin((unsigned long long *)&key[(py - Y0) * T + (px - X0)],
-                              (unsigned long long)make_key(z, mt.id));
+            {
+                uint32_t *ring = rring + wave * kFragRing;
+                uint32_t head = 0, queued = 0;  // wave-uniform
+                auto drain = [&](uint32_t n) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (lane < n) {
+                        const uint32_t e = ring[(head + lane) & (kFragRing - 1u)];
+                        const TriLds &mt = stage[e & 0xFFu];
+                        const int lx = (int)((e >> 8) & 0xFFu), ly = (int)(e >> 16);
+                        float w0, w1, w2, z;
+                        edges(mt, (float)(X0 + lx), (float)(Y0 + ly), w0, w1, w2);
+                        if (depth_of(mt, w0, w1, w2, z))
+                            atomicMin((unsigned long long *)&key[ly * T + lx], (unsigned long long)make_key(z, mt.id));
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    head += n;
+                    queued -= n;
+                };
+                for (uint32_t kb = wave * 64u; kb < nitems; kb += NT) {
+                    const uint32_t kk = kb + lane;
+                    bool covered = false;
+                    uint32_t entry = 0;
+                    if (kk < nitems) {
+                        const uint32_t j = find_item_owner(off, n_here, kk);
+                        const TriLds &mt = stage[j];
+                        int px, py;
+                        item_pixel(mt, kk - off[j], px, py);
+                        float w0, w1, w2;
+                        edges(mt, (float)px, (float)py, w0, w1, w2);
+                        covered = w0 >= 0.f && w1 >= 0.f && w2 >= 0.f;
+                        entry = j | ((uint32_t)(px - X0) << 8) | ((uint32_t)(py - Y0) << 16);
+                    }
+                    const uint64_t cm = __ballot(covered);
+                    if (covered) ring[(head + queued + popc_below(cm)) & (kFragRing - 1u)] = entry;
+                    queued += (uint32_t)__popcll(cm);
+                    if (queued >= 64u) drain(64u);
+                }
+                if (queued) drain(queued);
             }
             if (nwhole) {
                 __syncthreads();
