@@ -287,7 +287,8 @@ __device__ __forceinline__ int clamp_trunc(float v, float hi)
 
 constexpr uint64_t kClearKey = ((uint64_t)0x3F800000u << 32) | 0xFFFFFFFFull;  // depth 1.0, no triangle
 constexpr int kMaskChunks = 64;  // 64-item chunks per start-mask window (one mask word per lane)
-constexpr uint32_t kFragRing = 128;  // setup's per-wave queue of covered micro-triangle pixels
+constexpr uint32_t kFragRing = 128;  // per-wave queue of covered items (setup's micro triangles, resolve's queue)
+constexpr uint32_t kOwnerWords = 64;  // resolve: 64-item mask words per item window
 
 // Depth-test key: min z first, then max ID (low word = ~(id + 1)).
 __device__ __forceinline__ uint64_t make_key(float z, uint32_t id)
@@ -334,30 +335,20 @@ __device__ __forceinline__ void edge_vectors(const MicroLds &t, float &ax, float
     cx = t.sx0 - t.sx2; cy = t.sy0 - t.sy2;
 }
 
-// Item q of a triangle's pixel range [x0,x1) x [y0,y1), row-major.
+// Item q of a triangle's pixel range [x0,x1) x [y0,y1), row-major, for
+// q < 1024 and a row width w = x1 - x0 <= 1024 (setup's micro triangles: q <
+// micro_limit; resolve's items: q < small_limit; both limits are at most
+// 1024) with t.inv = fl(1 / w).  (q + 0.5) / w lies at least 0.5 / w >= 2^-11
+// from an integer, and fl(fl(q + 0.5) t.inv) is within 2^-23 (q + 0.5) / w <
+// 1.3e-4 of it (two roundings; q + 0.5 is exact), so its truncation is the
+// row.
 template <class Tri>
 __device__ __forceinline__ void item_pixel(const Tri &t, uint32_t q, int &px, int &py)
 {
-    const int x0 = (int)(t.xr & 0xFFFFu), x1 = (int)(t.xr >> 16), y0 = (int)(t.yr & 0xFFFFu);
-    const int w = x1 - x0;
-    int r = (int)((float)q * t.inv);  // may be off by one; corrected below
-    int c = (int)q - r * w;
-    if (c < 0) { --r; c += w; }
-    if (c >= w) { ++r; c -= w; }
-    px = x0 + c;
-    py = y0 + r;
-}
-
-// Largest j in [0, n) with off[j] <= k (off non-decreasing, off[0] = 0).
-__device__ __forceinline__ uint32_t find_item_owner(const uint32_t *off, uint32_t n, uint32_t k)
-{
-    uint32_t lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= k) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+    const uint32_t x0 = t.xr & 0xFFFFu, w = (t.xr >> 16) - x0, y0 = t.yr & 0xFFFFu;
+    const uint32_t r = (uint32_t)(((float)q + 0.5f) * t.inv);
+    px = (int)(x0 + q - __umul24(r, w));
+    py = (int)(y0 + r);
 }
 
 // The three edge functions of CCR/CUDAROPs.cu:120-145 at pixel (px, py).
@@ -726,7 +717,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
             }
             const uint32_t total = __shfl(incl, 63);
             const uint32_t start = incl - area;
-            MicroLds *ws = mstage + wave * 64;
+            const uint32_t wsb = wave * 64u;  // the wave's slots in mstage (32-bit LDS index)
             uint32_t *wo = moff + wave * 64;
             unsigned long long *wm = mmask + wave * kMaskChunks;
             if (micro) {
@@ -737,7 +728,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                 m.yr = (uint32_t)ymin | ((uint32_t)ymax << 16);
                 m.id = gid;
                 m.inv = 1.0f / (float)(xmax - xmin);
-                ws[rank] = m;
+                mstage[wsb + rank] = m;
                 wo[rank] = start;
             }
             uint32_t before = 0;  // triangles starting before the current chunk (wave-uniform)
@@ -750,7 +741,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 if (lane < n) {
                     const uint32_t e = ring[(head + lane) & (kFragRing - 1u)];
-                    const MicroLds &mt = ws[e & 63u];
+                    const MicroLds &mt = mstage[(uint16_t)(wsb + (e & 63u))];
                     const int px = (int)(mt.xr & 0xFFFFu) + (int)((e >> 6) & 0x3FFu);
                     const int py = (int)(mt.yr & 0xFFFFu) + (int)(e >> 16);
                     float w0, w1, w2, z;
@@ -783,7 +774,7 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
                     bool covered = false;
                     uint32_t entry = 0;
                     if (kk < total) {
-                        const MicroLds &mt = ws[j];
+                        const MicroLds &mt = mstage[(uint16_t)(wsb + j)];
                         int px, py;
                         item_pixel(mt, kk - wo[j], px, py);
                         float w0, w1, w2;
@@ -1245,7 +1236,10 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
     __shared__ uint32_t off[NT];
     __shared__ uint32_t wlist[NT];
     __shared__ uint32_t rring[NW * kFragRing];  // per wave: covered items waiting for their depth
-    __shared__ uint32_t wred_u[NW], wwhole[NW];
+    __shared__ unsigned long long omask[kOwnerWords];  // item window: bit k = a staged triangle's first item
+    __shared__ uint32_t ocum[kOwnerWords];             // first items before each mask word
+    __shared__ uint8_t olist[NT];                      // staged triangles with items, in item order
+    __shared__ uint32_t wred_u[NW], wwhole[NW], wown[NW], otot;
     __shared__ float wdmax[NW];
     __shared__ uint32_t ovf_base, ovf_fill;
 
@@ -1381,25 +1375,30 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                 if (lane >= (uint32_t)o) incl += y;
             }
             if (lane == 63) wred_u[wave] = incl;
-            const uint64_t wm = __ballot(whole);
-            if (lane == 0) wwhole[wave] = (uint32_t)__popcll(wm);
+            const uint64_t wm = __ballot(whole), hm = __ballot(area != 0);
+            if (lane == 0) {
+                wwhole[wave] = (uint32_t)__popcll(wm);
+                wown[wave] = (uint32_t)__popcll(hm);
+            }
             __syncthreads();
-            uint32_t before = 0, nitems = 0, pbefore = 0, nwhole = 0;
+            uint32_t before = 0, nitems = 0, pbefore = 0, nwhole = 0, obefore = 0;
             for (int w = 0; w < NW; ++w) {
-                if (w < (int)wave) { before += wred_u[w]; pbefore += wwhole[w]; }
+                if (w < (int)wave) { before += wred_u[w]; pbefore += wwhole[w]; obefore += wown[w]; }
                 nitems += wred_u[w];
                 nwhole += wwhole[w];
             }
-            off[tid] = before + incl - area;
+            const uint32_t first = before + incl - area;  // this triangle's first item
+            off[tid] = first;
             if (whole) wlist[pbefore + popc_below(wm)] = tid;
+            if (area) olist[obefore + popc_below(hm)] = (uint8_t)tid;
             __syncthreads();
-            // compacted items; slots past this chunk's triangles start at
-            // nitems, so the owner search stops at the last staged one.  The
-            // edge test runs per item; the covered items (about one in six)
-            // queue in the wave's ring (staged triangle, pixel in the tile),
-            // and the depth (three divisions) and key minimum run on 64 of
-            // them at a time, with every lane busy.
-            const uint32_t n_here = n_tile - base < (uint32_t)NT ? n_tile - base : (uint32_t)NT;
+            // compacted items, in windows of 64 kOwnerWords items: a mask bit
+            // marks each triangle's first item, so an item's triangle is the
+            // count of first items up to it (a popcount and the word's prefix
+            // count).  The edge test runs per item; the covered items (about
+            // one in six) queue in the wave's ring (staged triangle, pixel in
+            // the tile), and the depth (three divisions) and key minimum run
+            // on 64 of them at a time, with every lane busy.
             {
                 uint32_t *ring = rring + wave * kFragRing;
                 uint32_t head = 0, queued = 0;  // wave-uniform
@@ -1409,7 +1408,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     if (lane < n) {
                         const uint32_t e = ring[(head + lane) & (kFragRing - 1u)];
-                        const TriLds &mt = stage[e & 0xFFu];
+                        const TriLds &mt = stage[(uint16_t)(e & 0xFFu)];
                         const int lx = (int)((e >> 8) & 0xFFu), ly = (int)(e >> 16);
                         float w0, w1, w2, z;
                         edges(mt, (float)(X0 + lx), (float)(Y0 + ly), w0, w1, w2);
@@ -1422,24 +1421,50 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     head += n;
                     queued -= n;
                 };
-                for (uint32_t kb = wave * 64u; kb < nitems; kb += NT) {
-                    const uint32_t kk = kb + lane;
-                    bool covered = false;
-                    uint32_t entry = 0;
-                    if (kk < nitems) {
-                        const uint32_t j = find_item_owner(off, n_here, kk);
-                        const TriLds &mt = stage[j];
-                        int px, py;
-                        item_pixel(mt, kk - off[j], px, py);
-                        float w0, w1, w2;
-                        edges(mt, (float)px, (float)py, w0, w1, w2);
-                        covered = w0 >= 0.f && w1 >= 0.f && w2 >= 0.f;
-                        entry = j | ((uint32_t)(px - X0) << 8) | ((uint32_t)(py - Y0) << 16);
+                constexpr uint32_t kWindow = 64u * kOwnerWords;
+                static_assert(kWindow % NT == 0, "a window holds whole rounds of the block's item chunks");
+                uint32_t rbase = 0;  // triangles whose first item precedes the window
+                for (uint32_t wb = 0; wb < nitems; wb += kWindow) {
+                    if (wb) __syncthreads();  // the previous window's masks are read
+                    if (tid < kOwnerWords) omask[tid] = 0ull;
+                    __syncthreads();
+                    if (area && first >= wb && first - wb < kWindow)
+                        atomicOr(&omask[(first - wb) >> 6], 1ull << (first & 63u));
+                    __syncthreads();
+                    if (wave == 0) {
+                        const uint32_t c = lane < kOwnerWords ? (uint32_t)__popcll(omask[lane]) : 0u;
+                        uint32_t ci = c;
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const uint32_t y = __shfl_up(ci, o);
+                            if (lane >= (uint32_t)o) ci += y;
+                        }
+                        if (lane < kOwnerWords) ocum[lane] = rbase + ci - c;
+                        if (lane == 63) otot = ci;
                     }
-                    const uint64_t cm = __ballot(covered);
-                    if (covered) ring[(head + queued + popc_below(cm)) & (kFragRing - 1u)] = entry;
-                    queued += (uint32_t)__popcll(cm);
-                    if (queued >= 64u) drain(64u);
+                    __syncthreads();
+                    rbase += otot;
+                    const uint32_t wend = nitems - wb < kWindow ? nitems : wb + kWindow;
+                    for (uint32_t kb = wb + wave * 64u; kb < wend; kb += NT) {
+                        const uint32_t kk = kb + lane;
+                        const uint32_t wi = (kb - wb) >> 6;  // wave-uniform
+                        const uint64_t m = omask[wi];
+                        bool covered = false;
+                        uint32_t entry = 0;
+                        if (kk < wend) {
+                            const uint32_t j = olist[ocum[wi] + (uint32_t)__popcll(m & ((2ull << lane) - 1ull)) - 1u];
+                            const TriLds &mt = stage[(uint16_t)j];
+                            int px, py;
+                            item_pixel(mt, kk - off[j], px, py);
+                            float w0, w1, w2;
+                            edges(mt, (float)px, (float)py, w0, w1, w2);
+                            covered = w0 >= 0.f && w1 >= 0.f && w2 >= 0.f;
+                            entry = j | ((uint32_t)(px - X0) << 8) | ((uint32_t)(py - Y0) << 16);
+                        }
+                        const uint64_t cm = __ballot(covered);
+                        if (covered) ring[(head + queued + popc_below(cm)) & (kFragRing - 1u)] = entry;
+                        queued += (uint32_t)__popcll(cm);
+                        if (queued >= 64u) drain(64u);
+                    }
                 }
                 if (queued) drain(queued);
             }
