@@ -45,8 +45,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE_PATH = os.path.join(HERE, "tuning.json")
 
 TILES = (8, 16, 32)
-MICROS = (0, 16, 64, 128, 256)
-SMALLS = (16, 64, 192)
+MICROS = (0, 16, 32, 64, 128, 256)
+SMALLS = (16, 64, 192, 512, 1024)
 BIGS = (64, 256, 1024)
 QUEUE_MARGINS = (1.0, 1.1, 1.25, 1.5)
 QUEUE_SLACKS = (0, 4, 16)
