@@ -343,7 +343,7 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
     DBuf *bufs[] = { &r->pos4, &r->attr, &r->trirec,
                      &r->alive, &r->survivors, &r->tile_count[0], &r->tile_count[1], &r->tile_off, &r->tile_cap,
                      &r->tile_pool, &r->tile_cursor, &r->tile_min_depth[0], &r->tile_min_depth[1], &r->keys,
-                     &r->queue, &r->pool, &r->big, &r->ovf, &r->counters, &r->vis, &r->depth,
+                     &r->queue, &r->pool, &r->big, &r->ovf, &r->vis, &r->depth,
                      &r->rgba };
     for (DBuf *b : bufs) dfree(*b);
     if (r->h_counters) (void)hipHostFree(r->h_counters);
@@ -630,7 +630,6 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->tile_cap, tb, &synced));
     HIP_TRY(grow(r, r->tile_pool, tb, &synced));
     HIP_TRY(grow(r, r->tile_cursor, tb, &synced));
-    HIP_TRY(grow(r, r->counters, sizeof(FrameCounters) + 64, &synced));
     if (r->queue_cap == 0) r->queue_cap = std::max<uint32_t>(1u << 20, std::min<uint64_t>(4ull * nt, 0x7FFFFFFFull));
     if (r->pool_cap == 0) r->pool_cap = 1u << 20;
     if (r->big_cap == 0) r->big_cap = 1u << 16;
@@ -942,8 +941,9 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         L.fused_shade = (int)L.shade_modes;
     for (int st : { ST_RES8, ST_RES16, ST_RES32 })
         L.resident[st] = L.resolve_mode && L.grid[st] ? (uint32_t)resolve_resident_blocks(st, L.fused_shade) : 0u;
-    // staging block: the work counters (zeros), JobDesc[n], then DevMesh[meshes]
-    const size_t bytes = kStageCounterBytes + (size_t)n * sizeof(JobDesc) + (size_t)P.nmeshes * sizeof(DevMesh);
+    // staging block: the work counters (zeros), JobDesc[n], FrameCounters[n]
+    // (zeros), then DevMesh[meshes]
+    const size_t bytes = stage_meshes_offset(n) + (size_t)P.nmeshes * sizeof(DevMesh);
     if (bytes > P.stage_bytes || !P.h_stage[0] || !P.h_stage[1]) {
         for (int i = 0; i < 2; ++i) {
             if (P.h_stage[i]) (void)hipHostFree(P.h_stage[i]);
@@ -955,7 +955,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     P.stage_bytes = bytes;
     L.work = (uint32_t *)P.d_stage.p;
     L.jobs = (const JobDesc *)((char *)P.d_stage.p + kStageCounterBytes);
-    L.meshes = (const DevMesh *)((char *)P.d_stage.p + kStageCounterBytes + (size_t)n * sizeof(JobDesc));
+    L.meshes = (const DevMesh *)((char *)P.d_stage.p + stage_meshes_offset(n));
     P.key = key;
     P.built = true;
     return RAST_OK;
@@ -969,8 +969,10 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
     RAST_TRY(plan_wait_block(P, P.parity));
     void *h = P.h_stage[P.parity];
     std::memset(h, 0, kStageCounterBytes);  // the launch's work counters start at zero
+    // and so do the jobs' frame counters
+    std::memset((char *)h + stage_counters_offset(n), 0, stage_meshes_offset(n) - stage_counters_offset(n));
     JobDesc *jd = (JobDesc *)((char *)h + kStageCounterBytes);
-    DevMesh *dm = (DevMesh *)((char *)h + kStageCounterBytes + (size_t)n * sizeof(JobDesc));
+    DevMesh *dm = (DevMesh *)((char *)h + stage_meshes_offset(n));
     uint32_t me = 0;
     for (uint32_t j = 0; j < n; ++j) {
         rast_renderer *r = rs[j];
@@ -1027,7 +1029,7 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
         J.ovf = (uint2 *)r->ovf.p;
         J.survivors = (uint32_t *)r->survivors.p;
         J.big = (uint32_t *)r->big.p;
-        J.counters = (FrameCounters *)r->counters.p;
+        J.counters = (FrameCounters *)((char *)P.d_stage.p + stage_counters_offset(n) + (size_t)j * sizeof(FrameCounters));
         J.host_counters = r->d_counters_host;
         J.alive_part = (uint32_t *)P.d_alive_part.p;
         J.alive_part0 = 0;
@@ -1102,7 +1104,6 @@ static int enqueue_launch(LaunchPlan &P, hipStream_t s, rast_renderer *timed)
     const Launch &L = P.L;
     HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage[P.parity], P.stage_bytes, hipMemcpyHostToDevice, s));
     RAST_TRY(mark(timed, EV_START, s));
-    HIP_TRY(launch_frame_init(L, s));
     if (!timed) {
         // the capacities read only the previous frame's counts and depths:
         // their workgroups run in the vertex launch, beside the transform

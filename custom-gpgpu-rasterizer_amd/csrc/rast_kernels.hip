@@ -15,7 +15,9 @@
 //
 // Pipeline (one launch per stage for all jobs and meshes of a launch):
 //   k_vertex   per vertex: clip/NDC position + shading attributes; beside
-//              it, single-pass jobs' tile capacities and queue regions
+//              it, single-pass jobs' tile capacities and queue regions, and
+//              the reset of every job's per-tile counts (the frame counters
+//              start at zero with the launch's staging upload)
 //   k_setup    per triangle: assembly, cull, screen setup record; triangles
 //              whose bounding box has at most micro_limit pixels are
 //              rasterized on the spot with 64-bit atomic min into the key
@@ -431,11 +433,13 @@ __device__ __forceinline__ void load_tri(const uint32_t *__restrict__ rec, uint3
 
 // ---------------------------------------------------------------- kernels
 
-__global__ __launch_bounds__(256) void k_frame_init(const JobDesc *__restrict__ jobs,
-                                                    const uint16_t *__restrict__ table)
+// Block blk of the init stage: resets 256 tiles of its job's per-tile
+// counts, min depths and (previous-frame capacities) fallback offsets.
+__device__ __forceinline__ void frame_init_block(const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table,
+                                                 uint32_t blk)
 {
     uint32_t local;
-    const JobDesc &J = jobs[job_of(table, jobs, ST_INIT, local)];
+    const JobDesc &J = jobs[job_of(table, jobs, ST_INIT, local, blk)];
     const uint32_t n = J.fp.tiles_x * J.fp.tiles_y;
     const uint32_t i = local * 256u + threadIdx.x;
     if (i < n) {
@@ -443,7 +447,6 @@ __global__ __launch_bounds__(256) void k_frame_init(const JobDesc *__restrict__ 
         J.tile_min_depth[i] = 1.0f;
         if (J.single && !J.learned) J.tile_pool[i] = 0xFFFFFFFFu;
     }
-    if (i < sizeof(FrameCounters) / 4) reinterpret_cast<uint32_t *>(J.counters)[i] = 0;
 }
 
 // Texel i of the interleaved copy holds texel i of the diffuse, normal,
@@ -558,7 +561,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                                                 const uint2 *__restrict__ groups,
                                                 const uint32_t *__restrict__ entries,
                                                 const JobDesc *__restrict__ jobs,
-                                                const uint16_t *__restrict__ cap_table, uint32_t cap_blocks)
+                                                const uint16_t *__restrict__ cap_table, uint32_t cap_blocks,
+                                                const uint16_t *__restrict__ init_table, uint32_t init_blocks)
 {
     constexpr int kVertexF4 = 256 * 4, kCapF4 = (capacity_lds_floats<H>() + 3) / 4;
     __shared__ float4 lds[kVertexF4 > kCapF4 ? kVertexF4 : kCapF4];
@@ -568,7 +572,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             return;
         }
     }
-    vertex_block(meshes, block_group, groups, entries, jobs, blockIdx.x - cap_blocks, lds);
+    // the capacity workgroups read the previous frame's counts and depths,
+    // the init workgroups reset this frame's (the other parity)
+    const uint32_t blk = blockIdx.x - cap_blocks;
+    if (blk < init_blocks) {
+        frame_init_block(jobs, init_table, blk);
+        return;
+    }
+    vertex_block(meshes, block_group, groups, entries, jobs, blk - init_blocks, lds);
 }
 
 // TA_Kernel + TB_Kernel (CCR/CUDARenderer.cu:702-796,903-988) on the CPU
@@ -2223,12 +2234,6 @@ inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 }  // namespace
 
-hipError_t launch_frame_init(const Launch &L, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_frame_init, dim3(L.grid[ST_INIT]), dim3(256), 0, s, L.jobs, L.table[ST_INIT]);
-    return hipGetLastError();
-}
-
 hipError_t launch_interleave_maps(const uint32_t *d, const uint32_t *n, const uint32_t *sp, const uint32_t *g,
                                   uint4 *out, size_t count, hipStream_t s)
 {
@@ -2249,27 +2254,29 @@ hipError_t launch_vertex(const Launch &L, int with_capacity, hipStream_t s)
 {
     const uint32_t cap_blocks = L.grid[ST_CAP];
     const uint32_t vs_blocks = with_capacity == 2 ? 0u : L.vs_blocks;  // 2: the capacity stage alone
+    const uint32_t init_blocks = with_capacity == 2 ? 0u : L.grid[ST_INIT];
+    const uint16_t *init_table = L.table[ST_INIT];
     if (!with_capacity || !cap_blocks) {
-        if (vs_blocks == 0) return hipSuccess;
-        hipLaunchKernelGGL(k_vertex<0>, dim3(vs_blocks), dim3(256), 0, s, L.meshes, L.vs_block_group, L.vs_groups,
-                           L.vs_entries, L.jobs, (const uint16_t *)nullptr, 0u);
+        if (vs_blocks + init_blocks == 0) return hipSuccess;
+        hipLaunchKernelGGL(k_vertex<0>, dim3(init_blocks + vs_blocks), dim3(256), 0, s, L.meshes, L.vs_block_group,
+                           L.vs_groups, L.vs_entries, L.jobs, (const uint16_t *)nullptr, 0u, init_table, init_blocks);
         return hipGetLastError();
     }
-    const dim3 g(cap_blocks + vs_blocks), b(256);
+    const dim3 g(cap_blocks + init_blocks + vs_blocks), b(256);
     // one instance per hidden width (a launch's learned jobs share one);
     // heuristic-only launches take the smallest
     switch (L.model_hidden) {
     case 32:
         hipLaunchKernelGGL(k_vertex<32>, g, b, 0, s, L.meshes, L.vs_block_group, L.vs_groups, L.vs_entries, L.jobs,
-                           L.table[ST_CAP], cap_blocks);
+                           L.table[ST_CAP], cap_blocks, init_table, init_blocks);
         break;
     case 64:
         hipLaunchKernelGGL(k_vertex<64>, g, b, 0, s, L.meshes, L.vs_block_group, L.vs_groups, L.vs_entries, L.jobs,
-                           L.table[ST_CAP], cap_blocks);
+                           L.table[ST_CAP], cap_blocks, init_table, init_blocks);
         break;
     default:
         hipLaunchKernelGGL(k_vertex<16>, g, b, 0, s, L.meshes, L.vs_block_group, L.vs_groups, L.vs_entries, L.jobs,
-                           L.table[ST_CAP], cap_blocks);
+                           L.table[ST_CAP], cap_blocks, init_table, init_blocks);
         break;
     }
     return hipGetLastError();

@@ -39,13 +39,23 @@ struct Launch {
 // Staging bytes in front of the job table: the launch's work counters.
 constexpr size_t kStageCounterBytes = 64;
 
+// The staging block of a launch of n jobs: work counters, JobDesc[n], each
+// job's FrameCounters (zeros, like the work counters: the upload starts every
+// frame's counters at zero), then the mesh table.
+inline size_t stage_counters_offset(uint32_t n) { return kStageCounterBytes + (size_t)n * sizeof(JobDesc); }
+inline size_t stage_meshes_offset(uint32_t n)
+{
+    return (stage_counters_offset(n) + (size_t)n * sizeof(FrameCounters) + 15) & ~(size_t)15;
+}
+
 // Resets a key buffer to depth 1 / no triangle (before a renderer's first frame).
 hipError_t launch_clear_keys(unsigned long long *keys, size_t n, hipStream_t s);
 hipError_t launch_interleave_maps(const uint32_t *d, const uint32_t *n, const uint32_t *sp, const uint32_t *g,
                                   uint4 *out, size_t count, hipStream_t s);
-hipError_t launch_frame_init(const Launch &L, hipStream_t s);
 // with_capacity 0: the vertex pass; 1: the vertex pass with the single-pass
 // jobs' capacity workgroups beside it; 2: the capacity workgroups alone.
+// Modes 0 and 1 also reset the jobs' per-tile counts and depths (the frame
+// counters start at zero with the staging upload).
 hipError_t launch_vertex(const Launch &L, int with_capacity, hipStream_t s);
 hipError_t launch_scan(const Launch &L, hipStream_t s);  // exact counts
 hipError_t launch_setup(const Launch &L, hipStream_t s);
@@ -69,15 +79,21 @@ hipError_t launch_draw_texture(uint32_t *vis, float *depth, uint32_t *rgba, uint
 // read with a vector load (there is no 16-bit scalar load); it is the same
 // for the whole block, and readfirstlane says so, which keeps the job index
 // and every job field read through it in scalar registers.
-__device__ __forceinline__ uint32_t job_of(const uint16_t *table, const JobDesc *jobs, int stage, uint32_t &local)
+__device__ __forceinline__ uint32_t job_of(const uint16_t *table, const JobDesc *jobs, int stage, uint32_t &local,
+                                           uint32_t blk)
 {
     if (!table) {
-        local = blockIdx.x;
+        local = blk;
         return 0;
     }
-    const uint32_t j = __builtin_amdgcn_readfirstlane((uint32_t)table[blockIdx.x]);
-    local = blockIdx.x - jobs[j].block0[stage];
+    const uint32_t j = __builtin_amdgcn_readfirstlane((uint32_t)table[blk]);
+    local = blk - jobs[j].block0[stage];
     return j;
+}
+
+__device__ __forceinline__ uint32_t job_of(const uint16_t *table, const JobDesc *jobs, int stage, uint32_t &local)
+{
+    return job_of(table, jobs, stage, local, blockIdx.x);
 }
 
 // Training-set features of a rendered frame (rast_mlp.hip).
