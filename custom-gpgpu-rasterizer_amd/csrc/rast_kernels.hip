@@ -1606,7 +1606,22 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
             }
             __syncthreads();
             // the plane and pixel range of the next triangle are read ahead;
-            // the whole triangle only where some lane passes them
+            // the whole triangle only where some lane passes them.  The
+            // wave's largest depth (pixels of the frame) skips a triangle for
+            // the whole wave where its plane lies above it over the entire
+            // tile: the plane's float value at its lowest corner, less 8 u S
+            // (S its term magnitudes; its float evaluation is within 3 u S of
+            // the exact affine plane, whose minimum over the tile is at that
+            // corner), bounds the triangle's depth at every pixel from below.
+            auto wave_max_depth = [&]() {
+                float d = 0.f;
+#pragma unroll
+                for (int q = 0; q < PPT; ++q)
+                    if (inside[q]) d = fmaxf(d, __uint_as_float((uint32_t)(best[q] >> 32)));
+                for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o));
+                return d;
+            };
+            float wmax = wave_max_depth();
             float4 pl_next = bplane[0];
             uint2 bb_next = bbox[0];
             for (uint32_t j = 0; j < nb; ++j) {
@@ -1616,6 +1631,9 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     pl_next = bplane[j + 1];
                     bb_next = bbox[j + 1];
                 }
+                const float lmin = pl.x + fminf(pl.y, 0.f) * (float)(T - 1) + fminf(pl.z, 0.f) * (float)(T - 1);
+                const float mag = fabsf(pl.x) + (fabsf(pl.y) + fabsf(pl.z)) * (float)(T - 1);
+                if (lmin - mag * (8.0f / 16777216.0f) > wmax) continue;  // the same for the whole wave
                 const int x0 = (int)(bb.x & 0xFFFFu), y0 = (int)(bb.x >> 16);
                 const int x1 = (int)(bb.y & 0xFFFFu), y1 = (int)(bb.y >> 16);
                 bool pass[PPT];
@@ -1641,6 +1659,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                         best[q] = kk < best[q] ? kk : best[q];
                     }
                 }
+                wmax = wave_max_depth();
             }
             __syncthreads();
         }
