@@ -698,8 +698,9 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
     const bool micro = nonempty && npix <= fp.micro_limit;
     uint32_t tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0, span = 0;
     if (nonempty && !micro) {
-        tx0 = (uint32_t)xmin / fp.tile; tx1 = (uint32_t)(xmax - 1) / fp.tile;
-        ty0 = (uint32_t)ymin / fp.tile; ty1 = (uint32_t)(ymax - 1) / fp.tile;
+        const uint32_t sh = (uint32_t)__builtin_ctz(fp.tile);  // tiles are 8, 16 or 32 pixels
+        tx0 = (uint32_t)xmin >> sh; tx1 = (uint32_t)(xmax - 1) >> sh;
+        ty0 = (uint32_t)ymin >> sh; ty1 = (uint32_t)(ymax - 1) >> sh;
         span = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
     }
     const bool is_big = nonempty && !micro && span > fp.big_threshold;
@@ -926,8 +927,9 @@ __global__ __launch_bounds__(256) void k_fill(const JobDesc *__restrict__ jobs, 
         const uint32_t gid = survivors[i];
         const uint2 bb = *reinterpret_cast<const uint2 *>(trirec + (size_t)gid * kTriRecWords + 10);
         const uint32_t xmin = bb.x & 0xFFFFu, ymin = bb.x >> 16, xmax = bb.y & 0xFFFFu, ymax = bb.y >> 16;
-        const uint32_t tx0 = xmin / fp.tile, tx1 = (xmax - 1) / fp.tile;
-        const uint32_t ty0 = ymin / fp.tile, ty1 = (ymax - 1) / fp.tile;
+        const uint32_t sh = (uint32_t)__builtin_ctz(fp.tile);  // tiles are 8, 16 or 32 pixels
+        const uint32_t tx0 = xmin >> sh, tx1 = (xmax - 1) >> sh;
+        const uint32_t ty0 = ymin >> sh, ty1 = (ymax - 1) >> sh;
         for (uint32_t ty = ty0; ty <= ty1; ++ty)
             for (uint32_t tx = tx0; tx <= tx1; ++tx) {
                 const uint32_t t = ty * fp.tiles_x + tx;
