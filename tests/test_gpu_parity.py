@@ -177,6 +177,26 @@ def test_small_limit_invariance(small):
     assert_bitwise("depth", depth, odepth)
 
 
+@pytest.mark.parametrize("tile,micro,small", [(32, 0, 1024), (16, 0, 1024), (32, 1024, 1024), (8, 0, 64)])
+def test_crowded_tiles_and_largest_limits(tile, micro, small):
+    """600 overlapping random triangles at 160x96: tens of thousands of queued
+    items per tile (several 4096-item owner windows and several 256-triangle
+    staging chunks), items and micro-triangle ranges up to the 1024-pixel
+    limits; exact and previous-frame binning against the oracle."""
+    scene = random_triangle_scene(23, 600)
+    W, H = 160, 96
+    ovis, odepth, _, _ = O.render(scene, screen_camera(), S.Opts(), W, H, threads=4, want_rgba=False)
+    for policy in (R.BIN_EXACT, R.BIN_PREVFRAME):
+        with R.Renderer(W, H, tile=tile) as r:
+            r.set_tuning(micro, small)
+            r.add_scene(scene)
+            for _ in range(2):  # the heuristic sizes its queues from the frame before
+                r.render(screen_camera(), S.Opts(), policy)
+            vis, depth, _ = r.readback()
+        assert_bitwise(f"vis tile={tile} micro={micro} small={small} policy={policy}", vis, ovis)
+        assert_bitwise(f"depth tile={tile} micro={micro} small={small} policy={policy}", depth, odepth)
+
+
 def test_config2_icospheres_full_size():
     w = S.config2()
     st = check_frame(w.scene, w.camera, w.opts, w.width, w.height, threads=16)
