@@ -877,7 +877,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
                 group_of.emplace(m, g);
                 members.emplace_back();
                 group_block0.push_back((uint32_t)vs_group.size());
-                vs_group.insert(vs_group.end(), ceil_div(m->nv, 256), g);
+                vs_group.insert(vs_group.end(), ceil_div(m->nv, kVertexChunk), g);
             } else {
                 g = it->second;
             }

@@ -39,6 +39,7 @@ constexpr int kNumFeatures = 8;
 // Tiles per capacity workgroup (one per lane of its four waves).
 constexpr int kCapacityTilesPerBlock = 256;
 constexpr int kShadeRows = 8;  // pixel rows per shading block (64 wide; a multiple of 4)
+constexpr uint32_t kVertexChunk = 128;  // vertices per vertex workgroup (256 threads; at most 256)
 
 struct DevTexture {
     const uint32_t *texels;

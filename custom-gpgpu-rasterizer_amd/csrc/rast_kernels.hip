@@ -483,12 +483,13 @@ __device__ __forceinline__ void vertex_block(const DevMesh *__restrict__ meshes,
     // scene): the vertex is read once and transformed for each of them
     const uint2 gr = groups[block_group[blk]];
     const DevMesh &M0 = meshes[entries[gr.x]];
-    const uint32_t vbase = (blk - M0.vs_block0) * 256u;
-    const uint32_t nv = M0.num_verts - vbase < 256u ? M0.num_verts - vbase : 256u;  // vertices of this block
+    static_assert(kVertexChunk <= 256 && 256 % kVertexChunk == 0, "a chunk's quads are whole rounds of the block");
+    const uint32_t vbase = (blk - M0.vs_block0) * kVertexChunk;
+    const uint32_t nv = M0.num_verts - vbase < kVertexChunk ? M0.num_verts - vbase : kVertexChunk;  // vertices of this block
     const uint32_t tid = threadIdx.x;
     const float4 *src = reinterpret_cast<const float4 *>(M0.ivert) + (size_t)vbase * 4;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < (int)(kVertexChunk * 4 / 256); ++k) {
         const uint32_t w = (uint32_t)k * 256u + tid;  // quad w of the block: vertex w / 4, part w % 4
         if (w < nv * 4u) rows[at(w >> 2, w & 3u)] = ldg(src + w);
     }
@@ -540,7 +541,7 @@ __device__ __forceinline__ void vertex_block(const DevMesh *__restrict__ meshes,
         __syncthreads();
         float4 *dst = attr + (size_t)(M.vert_base + vbase) * 4;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < (int)(kVertexChunk * 4 / 256); ++k) {
             const uint32_t w = (uint32_t)k * 256u + tid;
             if (w < nv * 4u) stg(dst + w, rows[at(w >> 2, w & 3u)]);
         }
