@@ -778,6 +778,7 @@ void plan_free(LaunchPlan &p)
     dfree(p.d_alive_part);
     dfree(p.d_tables);
     dfree(p.d_stage);
+    dfree(p.d_sattr);
     p.built = false;
     p.key = 0;
 }
@@ -859,8 +860,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         const int res = r->tile == 8 ? ST_RES8 : (r->tile == 16 ? ST_RES16 : ST_RES32);
         nb[res] = ntiles;
         nb[ST_SHADE] = job_shades(r) ? ceil_div(r->W, 64) * ceil_div(r->H, (uint32_t)kShadeRows) : 0;
-        if (nb[ST_SHADE]) L.shade_modes |= 1u << r->last_opts.sample_state;
-        else ++L.unshaded_jobs;
+        if (!nb[ST_SHADE]) ++L.unshaded_jobs;
         for (int st = 0; st < ST_COUNT; ++st) {
             P.block0[(size_t)j * ST_COUNT + st] = (uint32_t)grid[st];
             P.nblocks[(size_t)j * ST_COUNT + st] = nb[st];
@@ -934,6 +934,30 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
             off += tables[st].size();
         }
     }
+    // jobs drawing one scene (views of one owner): that scene's
+    // view-independent attributes are written once, into the plan's buffer
+    {
+        std::unordered_map<const rast_renderer *, uint32_t> owner_jobs;
+        for (uint32_t j = 0; j < n; ++j) ++owner_jobs[scene_owner(rs[j])];
+        std::unordered_map<const rast_renderer *, size_t> owner_off;
+        P.sattr_off.assign(n, SIZE_MAX);
+        size_t sbytes = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            const rast_renderer *o = scene_owner(rs[j]);
+            if (owner_jobs[o] < 2) continue;
+            auto it = owner_off.find(o);
+            if (it == owner_off.end()) {
+                it = owner_off.emplace(o, sbytes).first;
+                sbytes += ((size_t)rs[j]->total_verts + 1) * 48;
+            }
+            P.sattr_off[j] = it->second;
+        }
+        if (sbytes) RAST_TRY(dev_alloc(P.d_sattr, sbytes));
+    }
+    // shading instances: bit (sampling mode + 2 for the shared-scene layout)
+    for (uint32_t j = 0; j < n; ++j)
+        if (P.nblocks[(size_t)j * ST_COUNT + ST_SHADE])
+            L.shade_modes |= 1u << (rs[j]->last_opts.sample_state + (P.sattr_off[j] != SIZE_MAX ? 2 : 0));
     // resolve form: the persistent kernels are sized to what the device holds
     L.resolve_mode = n ? rs[0]->resolve_mode : 0;
     L.fused_shade = 0;
@@ -1029,6 +1053,8 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
         J.ovf = (uint2 *)r->ovf.p;
         J.survivors = (uint32_t *)r->survivors.p;
         J.big = (uint32_t *)r->big.p;
+        J.sattr = P.sattr_off[j] != SIZE_MAX ? (float4 *)((char *)P.d_sattr.p + P.sattr_off[j]) : nullptr;
+        r->last_shared_attr = J.sattr != nullptr;
         J.counters = (FrameCounters *)((char *)P.d_stage.p + stage_counters_offset(n) + (size_t)j * sizeof(FrameCounters));
         J.host_counters = r->d_counters_host;
         J.alive_part = (uint32_t *)P.d_alive_part.p;
@@ -1359,7 +1385,13 @@ extern "C" int rast_debug_read(rast_renderer *r, int which, void *dst, size_t by
     std::vector<uint32_t> tmp;
     switch (which) {
     case RAST_DEBUG_VERTEX_POS: size = (size_t)r->total_verts * 16; src = r->pos4.p; break;
-    case RAST_DEBUG_VERTEX_ATTR: size = (size_t)r->total_verts * 64; src = r->attr.p; break;
+    case RAST_DEBUG_VERTEX_ATTR:
+        if (r->last_shared_attr)
+            return rast_set_error(RAST_E_STATE, "vertex attributes of views rendered in one launch are split "
+                                                "(view-independent part shared); render the view alone to read them");
+        size = (size_t)r->total_verts * 64;
+        src = r->attr.p;
+        break;
     case RAST_DEBUG_TRI_RECORDS: size = (size_t)r->total_tris * kTriRecWords * 4; src = r->trirec.p; break;
     case RAST_DEBUG_TRI_ALIVE: size = r->total_tris; src = r->alive.p; break;
     case RAST_DEBUG_TILE_COUNTS: size = (size_t)ntiles * 4; src = r->tile_count[last].p; break;

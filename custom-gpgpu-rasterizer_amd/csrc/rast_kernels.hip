@@ -502,6 +502,54 @@ __device__ __forceinline__ void vertex_block(const DevMesh *__restrict__ meshes,
     const float nx = q1.y, ny = q1.z, nz = q1.w;
     const float tx = q2.x, ty = q2.y, tz = q2.z;
     const float cr = q2.w, cg = q3.x, cb = q3.y;
+    if (jobs[M0.job].sattr) {
+        // views of one scene (a launch's jobs drawing this mesh share sattr):
+        // the view-independent attributes once, then per view the position
+        // and one float4 (1/w, view direction times 1/w)
+        const JobDesc &J0 = jobs[M0.job];
+        const float *Wd = M0.world;
+        const float n0 = m4(Wd, 0, 0) * nx + m4(Wd, 0, 1) * ny + m4(Wd, 0, 2) * nz;
+        const float n1 = m4(Wd, 1, 0) * nx + m4(Wd, 1, 1) * ny + m4(Wd, 1, 2) * nz;
+        const float n2 = m4(Wd, 2, 0) * nx + m4(Wd, 2, 1) * ny + m4(Wd, 2, 2) * nz;
+        const float t0 = m4(Wd, 0, 0) * tx + m4(Wd, 0, 1) * ty + m4(Wd, 0, 2) * tz;
+        const float t1 = m4(Wd, 1, 0) * tx + m4(Wd, 1, 1) * ty + m4(Wd, 1, 2) * tz;
+        const float t2 = m4(Wd, 2, 0) * tx + m4(Wd, 2, 1) * ty + m4(Wd, 2, 2) * tz;
+        const float w0 = m4(Wd, 0, 0) * px + m4(Wd, 0, 1) * py + m4(Wd, 0, 2) * pz + m4(Wd, 0, 3);
+        const float w1 = m4(Wd, 1, 0) * px + m4(Wd, 1, 1) * py + m4(Wd, 1, 2) * pz + m4(Wd, 1, 3);
+        const float w2 = m4(Wd, 2, 0) * px + m4(Wd, 2, 1) * py + m4(Wd, 2, 2) * pz + m4(Wd, 2, 3);
+        __syncthreads();  // the input rows are consumed
+        rows[at(tid, 0)] = make_float4(u, vv, n0, n1);
+        rows[at(tid, 1)] = make_float4(n2, t0, t1, t2);
+        rows[at(tid, 2)] = make_float4(cr, cg, cb, 0.f);
+        __syncthreads();
+        float4 *sdst = J0.sattr + (size_t)(M0.vert_base + vbase) * 3;
+#pragma unroll
+        for (int k = 0; k < (int)((kVertexChunk * 3 + 255) / 256); ++k) {
+            const uint32_t w = (uint32_t)k * 256u + tid;
+            if (w < nv * 3u) stg(sdst + w, rows[at(w / 3u, w % 3u)]);
+        }
+        for (uint32_t e = 0; e < gr.y; ++e) {
+            const DevMesh &M = meshes[entries[gr.x + e]];
+            const JobDesc &J = jobs[M.job];
+            const float *W = M.wvp;
+            float o0 = m4(W, 0, 0) * px + m4(W, 0, 1) * py + m4(W, 0, 2) * pz + m4(W, 0, 3);
+            float o1 = m4(W, 1, 0) * px + m4(W, 1, 1) * py + m4(W, 1, 2) * pz + m4(W, 1, 3);
+            float o2 = m4(W, 2, 0) * px + m4(W, 2, 1) * py + m4(W, 2, 2) * pz + m4(W, 2, 3);
+            const float o3 = m4(W, 3, 0) * px + m4(W, 3, 1) * py + m4(W, 3, 2) * pz + m4(W, 3, 3);
+            o0 /= o3;
+            o1 /= o3;
+            o2 /= o3;
+            float vx = w0 - M.cam_pos[0], vy = w1 - M.cam_pos[1], vz = w2 - M.cam_pos[2];
+            normalize3(vx, vy, vz);
+            const float iw = 1.f / o3;
+            if (live) {
+                const uint32_t g = M.vert_base + v;
+                stg(J.pos4 + g, make_float4(o0, o1, o2, o3));
+                stg(J.attr + g, make_float4(iw, vx * iw, vy * iw, vz * iw));
+            }
+        }
+        return;
+    }
     for (uint32_t e = 0; e < gr.y; ++e) {
         const DevMesh &M = meshes[entries[gr.x + e]];
         const JobDesc &J = jobs[M.job];
@@ -1211,7 +1259,7 @@ __device__ __forceinline__ void bitonic_sort(uint64_t *v, uint32_t P, uint32_t t
 // (1 point, 2 linear sampling): the tile's pixels are shaded right after,
 // from the visibility the lanes hold (jobs with that sampling mode and a
 // shading stage).
-template <bool kLinear>
+template <bool kLinear, bool kSplit>
 __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
                                             uint32_t x, uint32_t y, uint32_t id);
 
@@ -1699,7 +1747,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
         if (J.nblocks[ST_SHADE] && (fp.sample_state != 0) == (SHADE == 2)) {
 #pragma unroll
             for (int q = 0; q < PPT; ++q)
-                if (inside[q]) shade_pixel<SHADE == 2>(meshes, J, unit, (uint32_t)pxs[q], (uint32_t)pys[q],
+                if (inside[q]) shade_pixel<SHADE == 2, false>(meshes, J, unit, (uint32_t)pxs[q], (uint32_t)pys[q],
                                                        ~(uint32_t)best[q] - 1u);
         }
     }
@@ -1994,7 +2042,7 @@ __device__ __forceinline__ uint32_t light_pixel_exact(const MapColours &c, float
 // texture table is read with scalar loads, and when its four maps have one
 // size the texel indices are computed once and the texels come from the
 // maps' interleaved copy (one 16-byte load per tap).
-template <bool kLinear, bool kExact>
+template <bool kLinear, bool kExact, bool kSplit>
 __device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
                                            uint32_t x, uint32_t y, const uint4 q0, const uint4 q1, const uint4 q3)
 {
@@ -2008,10 +2056,32 @@ __device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, c
     const float sx1 = __uint_as_float(q0.z), sy1 = __uint_as_float(q0.w);
     const float sx2 = __uint_as_float(q1.x), sy2 = __uint_as_float(q1.y);
     const float area = __uint_as_float(q1.w);
-    const float4 *A0 = attr + (size_t)q3.x * 4, *A1 = attr + (size_t)q3.y * 4, *A2 = attr + (size_t)q3.z * 4;
-    const float4 a00 = ldg(A0), a10 = ldg(A1), a20 = ldg(A2);
-    const float4 a01 = ldg(A0 + 1), a11 = ldg(A1 + 1), a21 = ldg(A2 + 1);
-    const float4 a02 = ldg(A0 + 2), a12 = ldg(A1 + 2), a22 = ldg(A2 + 2);
+    // the three vertices' attributes times 1/w: read as stored, or (views
+    // sharing a scene) formed here from the view-independent part and the
+    // view's 1/w, the same binary32 products the vertex pass forms otherwise
+    float4 a00, a10, a20, a01, a11, a21, a02, a12, a22;
+    const float4 *__restrict__ sattr = J.sattr;
+    if constexpr (kSplit) {
+        const float4 *S0 = sattr + (size_t)q3.x * 3, *S1 = sattr + (size_t)q3.y * 3, *S2 = sattr + (size_t)q3.z * 3;
+        const float4 s00 = ldg(S0), s10 = ldg(S1), s20 = ldg(S2);
+        const float4 s01 = ldg(S0 + 1), s11 = ldg(S1 + 1), s21 = ldg(S2 + 1);
+        const float4 v0 = ldg(attr + q3.x), v1 = ldg(attr + q3.y), v2 = ldg(attr + q3.z);
+        auto q0of = [](float4 a, float4 b, float4 vw) {
+            return make_float4(a.x * vw.x, a.y * vw.x, vw.x, a.z * vw.x);
+        };
+        auto q1of = [](float4 a, float4 b, float4 vw) {
+            return make_float4(a.w * vw.x, b.x * vw.x, b.y * vw.x, b.z * vw.x);
+        };
+        auto q2of = [](float4 a, float4 b, float4 vw) { return make_float4(b.w * vw.x, vw.y, vw.z, vw.w); };
+        a00 = q0of(s00, s01, v0); a10 = q0of(s10, s11, v1); a20 = q0of(s20, s21, v2);
+        a01 = q1of(s00, s01, v0); a11 = q1of(s10, s11, v1); a21 = q1of(s20, s21, v2);
+        a02 = q2of(s00, s01, v0); a12 = q2of(s10, s11, v1); a22 = q2of(s20, s21, v2);
+    } else {
+        const float4 *A0 = attr + (size_t)q3.x * 4, *A1 = attr + (size_t)q3.y * 4, *A2 = attr + (size_t)q3.z * 4;
+        a00 = ldg(A0); a10 = ldg(A1); a20 = ldg(A2);
+        a01 = ldg(A0 + 1); a11 = ldg(A1 + 1); a21 = ldg(A2 + 1);
+        a02 = ldg(A0 + 2); a12 = ldg(A1 + 2); a22 = ldg(A2 + 2);
+    }
     const uint32_t me = q3.w & 0x7FFFFFFFu;  // bit 31: the exact-shading mark
     const uint32_t mi = me - J.mesh0 < fp.num_meshes ? me : J.mesh0;  // entry in the launch's mesh table
     const uint32_t mi0 = __builtin_amdgcn_readfirstlane(mi);
@@ -2057,7 +2127,18 @@ __device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, c
         return;
     }
     if (!has_diffuse) {  // vertex colour (perspective-weighted, not x w)
-        const float4 a03 = ldg(A0 + 3), a13 = ldg(A1 + 3), a23 = ldg(A2 + 3);
+        float4 a03, a13, a23;
+        if constexpr (kSplit) {
+            const float4 c0 = ldg(sattr + (size_t)q3.x * 3 + 2), c1 = ldg(sattr + (size_t)q3.y * 3 + 2);
+            const float4 c2 = ldg(sattr + (size_t)q3.z * 3 + 2);
+            a03 = make_float4(c0.x * a00.z, c0.y * a00.z, c0.z * a00.z, 0.f);
+            a13 = make_float4(c1.x * a10.z, c1.y * a10.z, c1.z * a10.z, 0.f);
+            a23 = make_float4(c2.x * a20.z, c2.y * a20.z, c2.z * a20.z, 0.f);
+        } else {
+            a03 = ldg(attr + (size_t)q3.x * 4 + 3);
+            a13 = ldg(attr + (size_t)q3.y * 4 + 3);
+            a23 = ldg(attr + (size_t)q3.z * 4 + 3);
+        }
         stg(rgba + pid, (uint32_t)(pack_rgba(RAST_FINTERP(a03.x, a13.x, a23.x), RAST_FINTERP(a03.y, a13.y, a23.y),
                                              RAST_FINTERP(a03.z, a13.z, a23.z))));
         return;
@@ -2092,7 +2173,7 @@ __device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, c
 // triangle's setup record and runs the fused or the exact shading body as
 // setup marked the triangle (two separate code paths, so the rare exact one
 // does not raise the register count of the fused one).
-template <bool kLinear>
+template <bool kLinear, bool kSplit>
 __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, const JobDesc &J, const float *unit,
                                             uint32_t x, uint32_t y, uint32_t id)
 {
@@ -2103,15 +2184,15 @@ __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, 
     }
     const uint4 *rp = reinterpret_cast<const uint4 *>(J.trirec + (size_t)id * kTriRecWords);
     const uint4 q0 = ldg(rp), q1 = ldg(rp + 1), q3 = ldg(rp + 3);  // q2 holds z2, z1 and the bbox
-    if (q3.w >> 31) shade_body<kLinear, true>(meshes, J, unit, x, y, q0, q1, q3);
-    else shade_body<kLinear, false>(meshes, J, unit, x, y, q0, q1, q3);
+    if (q3.w >> 31) shade_body<kLinear, true, kSplit>(meshes, J, unit, x, y, q0, q1, q3);
+    else shade_body<kLinear, false, kSplit>(meshes, J, unit, x, y, q0, q1, q3);
 }
 
 
 // A block shades 64 x kShadeRows pixels: lane x of wave w takes the pixels
 // (x, w), (x, w + 4), ...  The vis reads of all of them go out first, so each
 // further pixel of a lane waits for three dependent gathers instead of four.
-template <bool kLinear>
+template <bool kLinear, bool kSplit>
 __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
                                                const uint16_t *__restrict__ table)
 {
@@ -2121,7 +2202,9 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     uint32_t local;
     const JobDesc &J = jobs[job_of(table, jobs, ST_SHADE, local)];
     const FrameParams &fp = J.fp;
-    if ((fp.sample_state != 0) != kLinear) return;  // the other sampling mode's instance shades this frame
+    // another instance shades this frame: the other sampling mode's, or the
+    // one for the other attribute layout
+    if ((fp.sample_state != 0) != kLinear || (J.sattr != nullptr) != kSplit) return;
     // the frame's counters are final once resolve has run: the job's first
     // block copies them to the renderer's pinned host block (k_publish does
     // this for jobs that are not shaded)
@@ -2142,7 +2225,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshe
     }
 #pragma unroll
     for (int k = 0; k < P; ++k)
-        if (y0 + 4u * k < fp.height) shade_pixel<kLinear>(meshes, J, unit, x, y0 + 4u * k, ids[k]);
+        if (y0 + 4u * k < fp.height) shade_pixel<kLinear, kSplit>(meshes, J, unit, x, y0 + 4u * k, ids[k]);
 }
 
 // The persistent form of the resolve stage (the reference's TODO at
@@ -2410,12 +2493,21 @@ hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s)
 hipError_t launch_shade(const Launch &L, hipStream_t s)
 {
     if (!L.grid[ST_SHADE] || L.fused_shade) return hipSuccess;
-    // one instance per sampling mode present: a mode's code alone keeps the
-    // register count (and so the waves per SIMD) of that instance
+    // one instance per sampling mode and attribute layout present: a mode's
+    // code alone keeps the register count (and so the waves per SIMD) of that
+    // instance
     if (L.shade_modes & 1u)
-        hipLaunchKernelGGL(k_shade<false>, dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs, L.table[ST_SHADE]);
+        hipLaunchKernelGGL((k_shade<false, false>), dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs,
+                           L.table[ST_SHADE]);
     if (L.shade_modes & 2u)
-        hipLaunchKernelGGL(k_shade<true>, dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs, L.table[ST_SHADE]);
+        hipLaunchKernelGGL((k_shade<true, false>), dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs,
+                           L.table[ST_SHADE]);
+    if (L.shade_modes & 4u)
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs,
+                           L.table[ST_SHADE]);
+    if (L.shade_modes & 8u)
+        hipLaunchKernelGGL((k_shade<true, true>), dim3(L.grid[ST_SHADE]), dim3(256), 0, s, L.meshes, L.jobs,
+                           L.table[ST_SHADE]);
     return hipGetLastError();
 }
 

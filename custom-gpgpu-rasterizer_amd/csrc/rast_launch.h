@@ -26,7 +26,7 @@ struct Launch {
     uint32_t cap_jobs;                // jobs binning in a single pass (capacities placed by the vertex launch)
     uint32_t exact_jobs;              // jobs binning in two passes (count scan)
     int model_hidden;                 // hidden width of the learned jobs' MLP (16, 32 or 64; 0: none)
-    uint32_t shade_modes;             // sampling modes of the shading jobs: bit 0 point, bit 1 linear
+    uint32_t shade_modes;             // shading instances: bit 0 point, 1 linear; 2, 3 the same for jobs sharing a scene
     uint32_t unshaded_jobs;           // jobs without a shading stage (their counters go out with k_publish)
     // resolve stage: 0 one workgroup per tile; 1 persistent workgroups taking
     // tiles from a counter; 2 the same, shading each tile as it is resolved

@@ -54,6 +54,10 @@ struct LaunchPlan {
     std::vector<uint32_t> vs_block0, tri_block0;  // per mesh entry
     DBuf d_block_mesh, d_tables, d_stage;
     DBuf d_alive_part;  // culling survivors per setup block
+    // view-independent attributes of the scenes two or more jobs draw (views):
+    // 48 bytes per vertex of each such scene, at sattr_off[job] (SIZE_MAX: none)
+    DBuf d_sattr;
+    std::vector<size_t> sattr_off;
     // Two pinned staging blocks (JobDesc[njobs] then DevMesh[nmeshes]) used in
     // turn, so the next launch is staged while the previous one still reads
     // its block: the host never waits for the frame before last.  One
@@ -142,6 +146,7 @@ struct rast_renderer {
     bool frame_pending = false;
     bool frame_valid = false;
     bool last_single = false;
+    bool last_shared_attr = false;  // the last frame's attributes were split (views rendered together)
     bool prev_valid_before = false;
     bool frame_had_prev = false;  // the last frame had a previous frame (features valid)
     uint32_t rerenders = 0;
