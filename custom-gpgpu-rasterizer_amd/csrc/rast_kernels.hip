@@ -2193,7 +2193,7 @@ __device__ __forceinline__ void shade_pixel(const DevMesh *__restrict__ meshes, 
 // (x, w), (x, w + 4), ...  The vis reads of all of them go out first, so each
 // further pixel of a lane waits for three dependent gathers instead of four.
 template <bool kLinear, bool kSplit>
-__global__ __launch_bounds__(256) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kSplit && !kLinear ? 8 : 1))) void k_shade(const DevMesh *__restrict__ meshes, const JobDesc *__restrict__ jobs,
                                                const uint16_t *__restrict__ table)
 {
     __shared__ float unit[256];
