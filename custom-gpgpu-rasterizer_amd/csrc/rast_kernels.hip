@@ -1656,9 +1656,9 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                 bbox[slot] = make_uint2(t.xr, t.yr);
             }
             __syncthreads();
-            // the plane and pixel range of the next triangle are read ahead;
-            // the whole triangle only where some lane passes them.  The
-            // wave's largest depth (pixels of the frame) skips a triangle for
+            // a triangle's plane and pixel range are tested first, the whole
+            // triangle only where some lane passes them.  The wave's largest
+            // depth (pixels of the frame) skips a triangle for
             // the whole wave where its plane lies above it over the entire
             // tile: the plane's float value at its lowest corner, less 8 u S
             // (S its term magnitudes; its float evaluation is within 3 u S of
@@ -1672,19 +1672,16 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                 for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o));
                 return d;
             };
-            float wmax = wave_max_depth();
-            float4 pl_next = bplane[0];
-            uint2 bb_next = bbox[0];
-            for (uint32_t j = 0; j < nb; ++j) {
-                const float4 pl = pl_next;
-                const uint2 bb = bb_next;
-                if (j + 1 < nb) {
-                    pl_next = bplane[j + 1];
-                    bb_next = bbox[j + 1];
-                }
+            // the wave's lower bound of triangle j's depth over the tile, and
+            // the exact tests of triangle j on the lanes' pixels (true when
+            // some lane passed the per-pixel bounds; wmax is then current)
+            auto wave_floor = [](const float4 &pl) {
                 const float lmin = pl.x + fminf(pl.y, 0.f) * (float)(T - 1) + fminf(pl.z, 0.f) * (float)(T - 1);
                 const float mag = fabsf(pl.x) + (fabsf(pl.y) + fabsf(pl.z)) * (float)(T - 1);
-                if (lmin - mag * (8.0f / 16777216.0f) > wmax) continue;  // the same for the whole wave
+                return lmin - mag * (8.0f / 16777216.0f);
+            };
+            float wmax = wave_max_depth();
+            auto run = [&](uint32_t j, const float4 &pl, const uint2 &bb) {
                 const int x0 = (int)(bb.x & 0xFFFFu), y0 = (int)(bb.x >> 16);
                 const int x1 = (int)(bb.y & 0xFFFFu), y1 = (int)(bb.y >> 16);
                 bool pass[PPT];
@@ -1699,7 +1696,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     pass[q] = px >= x0 && px < x1 && py >= y0 && py < y1 && !(zc > db) && !(zc + pl.w < 0.f);
                     any = any || pass[q];
                 }
-                if (__ballot(any) == 0) continue;
+                if (__ballot(any) == 0) return false;
                 const TriLds &st = stage[j];
 #pragma unroll
                 for (int q = 0; q < PPT; ++q) {
@@ -1711,6 +1708,22 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     }
                 }
                 wmax = wave_max_depth();
+                return true;
+            };
+            // 64 triangles at a time, one per lane: the wave keeps a mask of
+            // those whose floor is not above its largest depth and narrows
+            // it each time that depth falls, so a skipped triangle costs no
+            // iteration (stored order is kept)
+            for (uint32_t g = 0; g < nb; g += 64u) {
+                const uint32_t jl = g + lane;
+                float floor_l = 3.0e38f;
+                if (jl < nb) floor_l = wave_floor(bplane[jl]);
+                uint64_t live = __ballot(jl < nb && !(floor_l > wmax));
+                while (live) {
+                    const uint32_t j = g + (uint32_t)__builtin_ctzll(live);
+                    live &= live - 1u;
+                    if (run(j, bplane[j], bbox[j])) live &= __ballot(!(floor_l > wmax));
+                }
             }
             __syncthreads();
         }
