@@ -1294,7 +1294,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
     __shared__ uint64_t key[NPIX];
     __shared__ TriLds stage[NT];
     __shared__ float4 bplane[NT];  // screen-list triangles: conservative depth plane over the tile
-    __shared__ uint2 bbox[NT];     // and their pixel ranges
+    __shared__ uint32_t bid[NT];   // and their IDs
     __shared__ uint32_t off[NT];
     __shared__ uint32_t wlist[NT];
     __shared__ uint32_t rring[NW * kFragRing];  // per wave: covered items waiting for their depth
@@ -1563,22 +1563,85 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
         for (int q = 0; q < PPT; ++q) best[q] = key[(int)tid + q * NT];
     }
 
-    // Screen-list triangles.  A list longer than one chunk is first bounded
-    // over the tile (pass 1), keeping the triangles that can still take a
-    // pixel, keyed by their depth lower bound (in the key array, free once the
-    // queue is resolved).  Sorted front to back, pass 2 then stages them chunk
-    // by chunk and stops where the next lower bound lies behind every pixel of
-    // the tile.  A list of one chunk, or more survivors than fit, goes to
-    // pass 2 in its stored order.  Each chunk is loaded and bounded in
-    // parallel, compacted into LDS, and every lane tests its own pixels
-    // against it.
+    // Screen-list triangles.  A triangle's plane and pixel range are tested
+    // first, the whole triangle only where some lane passes them.  The
+    // wave's largest depth (pixels of the frame) skips a triangle for the
+    // whole wave where its plane lies above it over the entire tile: the
+    // plane's float value at its lowest corner, less 8 u S (S its term
+    // magnitudes; its float evaluation is within 3 u S of the exact affine
+    // plane, whose minimum over the tile is at that corner), bounds the
+    // triangle's depth at every pixel from below.  The wave keeps a mask of
+    // up to 64 triangles (one per lane) whose bound is not above its largest
+    // depth and narrows it each time that depth falls, so a skipped triangle
+    // costs no iteration.
+    //
+    // LDS holds a kept triangle's plane and ID; its record is read again at
+    // its (uniform) address where it is run.  A list of at most 64 is
+    // handled by every wave on its own: its lanes load and bound the whole
+    // list against the wave's largest depth, with no barrier.  A longer list is
+    // first bounded over the tile (pass 1), keeping the triangles that can
+    // still take a pixel, keyed by their depth lower bound (in the key
+    // array, free once the queue is resolved).  Sorted front to back, pass 2
+    // then stages them chunk by chunk and stops where the next lower bound
+    // lies behind every pixel of the tile.  More survivors than fit go to
+    // pass 2 in their stored order.  Each chunk is loaded and bounded in
+    // parallel and compacted into LDS.
     uint32_t n_big = ldg(&ctr->num_big);
     n_big = n_big < big_cap ? n_big : big_cap;
     if (n_big) {
-        float tmax = tile_max_depth<PPT, NW>(best, lane, wave, wdmax);
+        auto wave_max_depth = [&]() {
+            float d = 0.f;
+#pragma unroll
+            for (int q = 0; q < PPT; ++q)
+                if (inside[q]) d = fmaxf(d, __uint_as_float((uint32_t)(best[q] >> 32)));
+            for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o));
+            return d;
+        };
+        auto wave_floor = [](const float4 &pl) {
+            const float lmin = pl.x + fminf(pl.y, 0.f) * (float)(T - 1) + fminf(pl.z, 0.f) * (float)(T - 1);
+            const float mag = fabsf(pl.x) + (fabsf(pl.y) + fabsf(pl.z)) * (float)(T - 1);
+            return lmin - mag * (8.0f / 16777216.0f);
+        };
+        float wmax = wave_max_depth();
+        // the exact tests of triangle st on the lanes' pixels; true when some
+        // lane passed the per-pixel bounds (wmax is then current)
+        auto run = [&](const TriLds &st, const float4 &pl, const uint2 &bb) {
+            const int x0 = (int)(bb.x & 0xFFFFu), y0 = (int)(bb.x >> 16);
+            const int x1 = (int)(bb.y & 0xFFFFu), y1 = (int)(bb.y >> 16);
+            bool pass[PPT];
+            bool any = false;
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int px = pxs[q], py = pys[q];
+                // lower bound of the triangle's depth here: above the current
+                // depth it loses, and with the band width below 0 it is out of range
+                const float zc = pl.x + pl.y * (float)(px - X0) + pl.z * (float)(py - Y0);
+                const float db = __uint_as_float((uint32_t)(best[q] >> 32));
+                pass[q] = px >= x0 && px < x1 && py >= y0 && py < y1 && !(zc > db) && !(zc + pl.w < 0.f);
+                any = any || pass[q];
+            }
+            if (__ballot(any) == 0) return false;
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                float z;
+                if (pass[q] && fragment_closer(st, (float)pxs[q], (float)pys[q],
+                                               __uint_as_float((uint32_t)(best[q] >> 32)), z)) {
+                    const uint64_t kk = make_key(z, st.id);
+                    best[q] = kk < best[q] ? kk : best[q];
+                }
+            }
+            wmax = wave_max_depth();
+            return true;
+        };
+        // the short form: every wave bounds the whole list (at most 64) on
+        // its own against its largest depth and keeps it in its own 64
+        // entries of bplane / bid
+        const bool own = n_big <= 64u;  // the same for the whole block
+        float tmax = 1.0f;
         uint64_t *cand = key;  // NPIX entries: (ordered lower bound << 32) | list index
         uint32_t n_cand = 0;
         const bool presort = n_big > (uint32_t)NT;
+        if (!own) tmax = tile_max_depth<PPT, NW>(best, lane, wave, wdmax);
         for (uint32_t base = 0; presort && base < n_big; base += NT) {
             const uint32_t i = base + tid;
             bool keep = false;
@@ -1619,113 +1682,78 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
             bitonic_sort<NT>(cand, P, tid);
         }
         const uint32_t n_list = sorted ? n_cand : n_big;
-        for (uint32_t base = 0; base < n_list; base += NT) {
-            if (base) tmax = tile_max_depth<PPT, NW>(best, lane, wave, wdmax);
-            // sorted: every further triangle is bounded below by this one's bound
-            if (sorted && (uint32_t)(cand[base] >> 32) > ordered_bits(tmax)) break;
-            const uint32_t i = base + tid;
+        const uint32_t step = own ? 64u : (uint32_t)NT;
+        for (uint32_t base = 0; base < n_list; base += step) {
+            if (!own) {
+                if (base) tmax = tile_max_depth<PPT, NW>(best, lane, wave, wdmax);
+                // sorted: every further triangle is bounded below by this one's bound
+                if (sorted && (uint32_t)(cand[base] >> 32) > ordered_bits(tmax)) break;
+            }
+            const uint32_t i = base + (own ? lane : tid);
             bool keep = false;
-            TriLds t;
-            float4 pl;
+            uint32_t id = 0xFFFFFFFFu;
+            float4 pl = make_float4(3.0e38f, 0.f, 0.f, 0.f);
             if (i < n_list) {
                 const uint32_t li = sorted ? (uint32_t)cand[i] : i;
-                const uint32_t id = ldg(big + li);
+                id = ldg(big + li);
                 if (id < num_tris) {
+                    TriLds t;
                     load_tri(trirec, id, t);
                     const int x0 = (int)(t.xr & 0xFFFFu), y0 = (int)(t.xr >> 16);
                     const int x1 = (int)(t.yr & 0xFFFFu), y1 = (int)(t.yr >> 16);
                     if (x1 > X0 && x0 < X0 + T && y1 > Y0 && y0 < Y0 + T) {
-                        pl = tile_bound(t, X0, Y0, T, tmax);
+                        pl = tile_bound(t, X0, Y0, T, own ? wmax : tmax);
                         keep = !(pl.x > kSkipBound);
                     }
                 }
             }
-            const uint64_t km = __ballot(keep);
-            if (lane == 0) wred_u[wave] = (uint32_t)__popcll(km);
-            __syncthreads();
-            uint32_t before = 0, nb = 0;
-            for (int w = 0; w < NW; ++w) {
-                if (w < (int)wave) before += wred_u[w];
-                nb += wred_u[w];
-            }
-            if (keep) {
-                const uint32_t slot = before + popc_below(km);
-                t.inv = 1.0f / t.area;
-                stage[slot] = t;
-                bplane[slot] = pl;
-                bbox[slot] = make_uint2(t.xr, t.yr);
-            }
-            __syncthreads();
-            // a triangle's plane and pixel range are tested first, the whole
-            // triangle only where some lane passes them.  The wave's largest
-            // depth (pixels of the frame) skips a triangle for
-            // the whole wave where its plane lies above it over the entire
-            // tile: the plane's float value at its lowest corner, less 8 u S
-            // (S its term magnitudes; its float evaluation is within 3 u S of
-            // the exact affine plane, whose minimum over the tile is at that
-            // corner), bounds the triangle's depth at every pixel from below.
-            auto wave_max_depth = [&]() {
-                float d = 0.f;
-#pragma unroll
-                for (int q = 0; q < PPT; ++q)
-                    if (inside[q]) d = fmaxf(d, __uint_as_float((uint32_t)(best[q] >> 32)));
-                for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o));
-                return d;
-            };
-            // the wave's lower bound of triangle j's depth over the tile, and
-            // the exact tests of triangle j on the lanes' pixels (true when
-            // some lane passed the per-pixel bounds; wmax is then current)
-            auto wave_floor = [](const float4 &pl) {
-                const float lmin = pl.x + fminf(pl.y, 0.f) * (float)(T - 1) + fminf(pl.z, 0.f) * (float)(T - 1);
-                const float mag = fabsf(pl.x) + (fabsf(pl.y) + fabsf(pl.z)) * (float)(T - 1);
-                return lmin - mag * (8.0f / 16777216.0f);
-            };
-            float wmax = wave_max_depth();
-            auto run = [&](uint32_t j, const float4 &pl, const uint2 &bb) {
-                const int x0 = (int)(bb.x & 0xFFFFu), y0 = (int)(bb.x >> 16);
-                const int x1 = (int)(bb.y & 0xFFFFu), y1 = (int)(bb.y >> 16);
-                bool pass[PPT];
-                bool any = false;
-#pragma unroll
-                for (int q = 0; q < PPT; ++q) {
-                    const int px = pxs[q], py = pys[q];
-                    // lower bound of the triangle's depth here: above the current
-                    // depth it loses, and with the band width below 0 it is out of range
-                    const float zc = pl.x + pl.y * (float)(px - X0) + pl.z * (float)(py - Y0);
-                    const float db = __uint_as_float((uint32_t)(best[q] >> 32));
-                    pass[q] = px >= x0 && px < x1 && py >= y0 && py < y1 && !(zc > db) && !(zc + pl.w < 0.f);
-                    any = any || pass[q];
+            // the entries to run: the wave's own 64, or the block's compacted chunk
+            uint32_t j0, nb;
+            if (own) {
+                j0 = wave * 64u;
+                nb = n_list;
+                if (!keep) pl = make_float4(3.0e38f, 0.f, 0.f, 0.f);
+                bplane[j0 + lane] = pl;
+                bid[j0 + lane] = id;
+                // the wave's lanes read one another's entries below
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            } else {
+                const uint64_t km = __ballot(keep);
+                if (lane == 0) wred_u[wave] = (uint32_t)__popcll(km);
+                __syncthreads();
+                uint32_t before = 0;
+                nb = 0;
+                for (int w = 0; w < NW; ++w) {
+                    if (w < (int)wave) before += wred_u[w];
+                    nb += wred_u[w];
                 }
-                if (__ballot(any) == 0) return false;
-                const TriLds &st = stage[j];
-#pragma unroll
-                for (int q = 0; q < PPT; ++q) {
-                    float z;
-                    if (pass[q] && fragment_closer(st, (float)pxs[q], (float)pys[q],
-                                                   __uint_as_float((uint32_t)(best[q] >> 32)), z)) {
-                        const uint64_t kk = make_key(z, st.id);
-                        best[q] = kk < best[q] ? kk : best[q];
-                    }
+                if (keep) {
+                    const uint32_t slot = before + popc_below(km);
+                    bplane[slot] = pl;
+                    bid[slot] = id;
                 }
-                wmax = wave_max_depth();
-                return true;
-            };
-            // 64 triangles at a time, one per lane: the wave keeps a mask of
-            // those whose floor is not above its largest depth and narrows
-            // it each time that depth falls, so a skipped triangle costs no
-            // iteration (stored order is kept)
+                __syncthreads();
+                j0 = 0;
+            }
+            // 64 entries at a time, one per lane
             for (uint32_t g = 0; g < nb; g += 64u) {
                 const uint32_t jl = g + lane;
                 float floor_l = 3.0e38f;
-                if (jl < nb) floor_l = wave_floor(bplane[jl]);
+                if (jl < nb) floor_l = wave_floor(bplane[j0 + jl]);
                 uint64_t live = __ballot(jl < nb && !(floor_l > wmax));
                 while (live) {
-                    const uint32_t j = g + (uint32_t)__builtin_ctzll(live);
+                    const uint32_t j = j0 + g + (uint32_t)__builtin_ctzll(live);
                     live &= live - 1u;
-                    if (run(j, bplane[j], bbox[j])) live &= __ballot(!(floor_l > wmax));
+                    const float4 plj = bplane[j];
+                    TriLds st;
+                    load_tri(trirec, bid[j], st);
+                    st.inv = 1.0f / st.area;
+                    if (run(st, plj, make_uint2(st.xr, st.yr))) live &= __ballot(!(floor_l > wmax));
                 }
             }
-            __syncthreads();
+            if (!own) __syncthreads();
         }
     }
 

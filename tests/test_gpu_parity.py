@@ -135,14 +135,24 @@ def test_repeated_frames_bit_identical():
         np.testing.assert_array_equal(x.view(np.uint32), y.view(np.uint32))
 
 
-def test_big_triangles_screen_list():
-    # two screen-covering triangles plus small ones; big_threshold 1 forces the screen-wide list
-    scene = random_triangle_scene(7, 50)
+@pytest.mark.parametrize("ntri", [50, 400])
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_big_triangles_screen_list(ntri, tile):
+    """Two screen-covering triangles plus small ones; big_threshold 1 puts
+    all that span more than one tile on the screen-wide list: 12-22 with 50
+    triangles, which fits one wave (every wave bounds it on its own), and
+    107-186 with 400, longer than a wave: one staged chunk for the 16 and 32
+    tiles (256 threads), sorted front to back first for the 8 tiles (64
+    threads)."""
+    scene = random_triangle_scene(7, ntri)
     quad = mesh([(-50, -50, -20), (50, -50, -20), (50, 50, -20), (-50, 50, -20)], [(0, 1, 2), (0, 2, 3)])
     scene = S.Scene(scene.meshes + [quad])
+    ovis, odepth, _, _ = O.render(scene, screen_camera(), S.Opts(), 128, 96, threads=4, want_rgba=False)
     for bt in (1, 4, 256):
-        ovis, odepth, _, _ = O.render(scene, screen_camera(), S.Opts(), 128, 96, threads=4, want_rgba=False)
-        vis, depth, _, st = gpu_render(scene, screen_camera(), S.Opts(), 128, 96, big_threshold=bt)
+        vis, depth, _, st = gpu_render(scene, screen_camera(), S.Opts(), 128, 96, tile=tile, big_threshold=bt)
+        if bt == 1:
+            assert (st["num_big"] <= 64) == (ntri == 50), st["num_big"]
+            assert st["num_big"] <= 256, st["num_big"]
         assert_bitwise(f"vis bt={bt}", vis, ovis)
         assert_bitwise(f"depth bt={bt}", depth, odepth)
 
