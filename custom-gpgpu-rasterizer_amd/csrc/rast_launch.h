@@ -42,6 +42,8 @@ constexpr size_t kStageCounterBytes = 64;
 // The staging block of a launch of n jobs: work counters, JobDesc[n], each
 // job's FrameCounters (zeros, like the work counters: the upload starts every
 // frame's counters at zero), then the mesh table.
+static_assert(kStageCounterBytes % 16 == 0 && sizeof(JobDesc) % 16 == 0 && sizeof(DevMesh) % 16 == 0,
+              "the staging block is uploaded in 16-byte words");
 inline size_t stage_counters_offset(uint32_t n) { return kStageCounterBytes + (size_t)n * sizeof(JobDesc); }
 inline size_t stage_meshes_offset(uint32_t n)
 {
@@ -66,6 +68,8 @@ hipError_t launch_resolve(const Launch &L, int stage, hipStream_t s);
 // holds at once (0 on failure).
 int resolve_resident_blocks(int stage, int fused_shade);
 hipError_t launch_shade(const Launch &L, hipStream_t s);
+// Copies a staging block (a multiple of 16 bytes) from pinned host memory.
+hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_publish(const Launch &L, hipStream_t s);
 // Placement of the previous-frame policy's entries beyond capacity.
 hipError_t launch_fallback(const Launch &L, hipStream_t s);
