@@ -611,7 +611,7 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     bool synced = false;
     const size_t nv = (size_t)r->total_verts + 1, nt = (size_t)r->total_tris + 1;
     HIP_TRY(grow(r, r->pos4, nv * 16, &synced));
-    HIP_TRY(grow(r, r->attr, nv * 64, &synced));
+    HIP_TRY(grow(r, r->attr, nv * 48, &synced));
     HIP_TRY(grow(r, r->trirec, nt * kTriRecWords * 4, &synced));
     HIP_TRY(grow(r, r->alive, nt, &synced));
     HIP_TRY(grow(r, r->survivors, nt * 4, &synced));
@@ -953,7 +953,7 @@ static int build_plan(LaunchPlan &P, rast_renderer *const *rs, uint32_t n, int h
             auto it = owner_off.find(o);
             if (it == owner_off.end()) {
                 it = owner_off.emplace(o, sbytes).first;
-                sbytes += ((size_t)rs[j]->total_verts + 1) * 48;
+                sbytes += ((size_t)rs[j]->total_verts + 1) * 32;
             }
             P.sattr_off[j] = it->second;
         }
@@ -1394,7 +1394,7 @@ extern "C" int rast_debug_read(rast_renderer *r, int which, void *dst, size_t by
         if (r->last_shared_attr)
             return rast_set_error(RAST_E_STATE, "vertex attributes of views rendered in one launch are split "
                                                 "(view-independent part shared); render the view alone to read them");
-        size = (size_t)r->total_verts * 64;
+        size = (size_t)r->total_verts * 48;
         src = r->attr.p;
         break;
     case RAST_DEBUG_TRI_RECORDS: size = (size_t)r->total_tris * kTriRecWords * 4; src = r->trirec.p; break;

@@ -135,12 +135,14 @@ struct JobDesc {
     uint32_t defaults[4];        // 1x1 stand-in maps: [1] normal, [2] specular, [3] gloss
     const float *model;          // MLP parameters (learned capacities)
     float4 *pos4;
-    // shading attributes.  sattr null: attr holds four float4 per vertex, every
-    // attribute times the vertex's 1/w (u v 1/w n.x | n.y n.z t.x t.y | t.z vd.xyz
-    // | colour).  Several jobs of a launch drawing one scene (views): sattr holds
-    // that scene's view-independent part once, three float4 per vertex (u v n.x
-    // n.y | n.z t.xyz | colour, not times 1/w), and attr one float4 per vertex
-    // per job (1/w, vd.xyz times 1/w); shading forms the same products.
+    // shading attributes.  sattr null: attr holds three float4 per vertex, every
+    // attribute times the vertex's 1/w (u v 1/w n.x | n.y n.z t.x t.y | t.z
+    // vd.xyz).  Several jobs of a launch drawing one scene (views): sattr holds
+    // that scene's view-independent part once, two float4 per vertex (u v n.x
+    // n.y | n.z t.xyz, not times 1/w), and attr one float4 per vertex per job
+    // (1/w, vd.xyz times 1/w); shading forms the same products.  The vertex
+    // colour (read only where a mesh has no diffuse map) is taken from the
+    // mesh's input vertices at shading, times the stored 1/w.
     float4 *attr;
     float4 *sattr;
     uint32_t *trirec;

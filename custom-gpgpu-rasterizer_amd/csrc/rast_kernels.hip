@@ -495,13 +495,12 @@ __device__ __forceinline__ void vertex_block(const DevMesh *__restrict__ meshes,
     }
     __syncthreads();
     const bool live = tid < nv;
-    const float4 q0 = rows[at(tid, 0)], q1 = rows[at(tid, 1)], q2 = rows[at(tid, 2)], q3 = rows[at(tid, 3)];
+    const float4 q0 = rows[at(tid, 0)], q1 = rows[at(tid, 1)], q2 = rows[at(tid, 2)];  // q3: colour, read at shading
     const uint32_t v = vbase + tid;
     const float px = q0.x, py = q0.y, pz = q0.z;
     const float u = q0.w, vv = q1.x;
     const float nx = q1.y, ny = q1.z, nz = q1.w;
     const float tx = q2.x, ty = q2.y, tz = q2.z;
-    const float cr = q2.w, cg = q3.x, cb = q3.y;
     if (jobs[M0.job].sattr) {
         // views of one scene (a launch's jobs drawing this mesh share sattr):
         // the view-independent attributes once, then per view the position
@@ -520,13 +519,12 @@ __device__ __forceinline__ void vertex_block(const DevMesh *__restrict__ meshes,
         __syncthreads();  // the input rows are consumed
         rows[at(tid, 0)] = make_float4(u, vv, n0, n1);
         rows[at(tid, 1)] = make_float4(n2, t0, t1, t2);
-        rows[at(tid, 2)] = make_float4(cr, cg, cb, 0.f);
         __syncthreads();
-        float4 *sdst = J0.sattr + (size_t)(M0.vert_base + vbase) * 3;
+        float4 *sdst = J0.sattr + (size_t)(M0.vert_base + vbase) * 2;
 #pragma unroll
-        for (int k = 0; k < (int)((kVertexChunk * 3 + 255) / 256); ++k) {
+        for (int k = 0; k < (int)(kVertexChunk * 2 / 256); ++k) {
             const uint32_t w = (uint32_t)k * 256u + tid;
-            if (w < nv * 3u) stg(sdst + w, rows[at(w / 3u, w % 3u)]);
+            if (w < nv * 2u) stg(sdst + w, rows[at(w >> 1, w & 1u)]);
         }
         for (uint32_t e = 0; e < gr.y; ++e) {
             const DevMesh &M = meshes[entries[gr.x + e]];
@@ -579,19 +577,21 @@ __device__ __forceinline__ void vertex_block(const DevMesh *__restrict__ meshes,
         // Every attribute is stored multiplied by 1/w: the interpolation of
         // CCR/CUDAROPs.cu:347-380 forms w_k * (c_k * (1/w_k)) per pixel, and the
         // product c_k * (1/w_k) is the same binary32 value whether it is formed
-        // here once or per pixel.  PBR shading reads the first three float4s only.
+        // here once or per pixel.  48 bytes a vertex, the three float4s shading
+        // reads (the vertex colour is multiplied there; a 64-byte record with
+        // it took the 4K vertex pass 3 us longer, and planar arrays cost shading
+        // 3 us more in cache lines touched).
         const float iw = 1.f / o3;
         __syncthreads();  // the rows of the previous job (or of the input) are consumed
         rows[at(tid, 0)] = make_float4(u * iw, vv * iw, iw, n0 * iw);
         rows[at(tid, 1)] = make_float4(n1 * iw, n2 * iw, t0 * iw, t1 * iw);
         rows[at(tid, 2)] = make_float4(t2 * iw, vx * iw, vy * iw, vz * iw);
-        rows[at(tid, 3)] = make_float4(cr * iw, cg * iw, cb * iw, 0.f);
         __syncthreads();
-        float4 *dst = attr + (size_t)(M.vert_base + vbase) * 4;
+        float4 *dst = attr + (size_t)(M.vert_base + vbase) * 3;
 #pragma unroll
-        for (int k = 0; k < (int)(kVertexChunk * 4 / 256); ++k) {
+        for (int k = 0; k < (int)((kVertexChunk * 3 + 255) / 256); ++k) {
             const uint32_t w = (uint32_t)k * 256u + tid;
-            if (w < nv * 4u) stg(dst + w, rows[at(w >> 2, w & 3u)]);
+            if (w < nv * 3u) stg(dst + w, rows[at(w / 3u, w % 3u)]);
         }
     }
 }
@@ -2103,7 +2103,7 @@ __device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, c
     float4 a00, a10, a20, a01, a11, a21, a02, a12, a22;
     const float4 *__restrict__ sattr = J.sattr;
     if constexpr (kSplit) {
-        const float4 *S0 = sattr + (size_t)q3.x * 3, *S1 = sattr + (size_t)q3.y * 3, *S2 = sattr + (size_t)q3.z * 3;
+        const float4 *S0 = sattr + (size_t)q3.x * 2, *S1 = sattr + (size_t)q3.y * 2, *S2 = sattr + (size_t)q3.z * 2;
         const float4 s00 = ldg(S0), s10 = ldg(S1), s20 = ldg(S2);
         const float4 s01 = ldg(S0 + 1), s11 = ldg(S1 + 1), s21 = ldg(S2 + 1);
         const float4 v0 = ldg(attr + q3.x), v1 = ldg(attr + q3.y), v2 = ldg(attr + q3.z);
@@ -2118,7 +2118,7 @@ __device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, c
         a01 = q1of(s00, s01, v0); a11 = q1of(s10, s11, v1); a21 = q1of(s20, s21, v2);
         a02 = q2of(s00, s01, v0); a12 = q2of(s10, s11, v1); a22 = q2of(s20, s21, v2);
     } else {
-        const float4 *A0 = attr + (size_t)q3.x * 4, *A1 = attr + (size_t)q3.y * 4, *A2 = attr + (size_t)q3.z * 4;
+        const float4 *A0 = attr + (size_t)q3.x * 3, *A1 = attr + (size_t)q3.y * 3, *A2 = attr + (size_t)q3.z * 3;
         a00 = ldg(A0); a10 = ldg(A1); a20 = ldg(A2);
         a01 = ldg(A0 + 1); a11 = ldg(A1 + 1); a21 = ldg(A2 + 1);
         a02 = ldg(A0 + 2); a12 = ldg(A1 + 2); a22 = ldg(A2 + 2);
@@ -2168,18 +2168,16 @@ __device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, c
         return;
     }
     if (!has_diffuse) {  // vertex colour (perspective-weighted, not x w)
-        float4 a03, a13, a23;
-        if constexpr (kSplit) {
-            const float4 c0 = ldg(sattr + (size_t)q3.x * 3 + 2), c1 = ldg(sattr + (size_t)q3.y * 3 + 2);
-            const float4 c2 = ldg(sattr + (size_t)q3.z * 3 + 2);
-            a03 = make_float4(c0.x * a00.z, c0.y * a00.z, c0.z * a00.z, 0.f);
-            a13 = make_float4(c1.x * a10.z, c1.y * a10.z, c1.z * a10.z, 0.f);
-            a23 = make_float4(c2.x * a20.z, c2.y * a20.z, c2.z * a20.z, 0.f);
-        } else {
-            a03 = ldg(attr + (size_t)q3.x * 4 + 3);
-            a13 = ldg(attr + (size_t)q3.y * 4 + 3);
-            a23 = ldg(attr + (size_t)q3.z * 4 + 3);
-        }
+        // the mesh's input colour (floats 11-13 of the vertex) times the
+        // vertex's stored 1/w: the binary32 product the interpolation takes
+        const float *iv = ldg(&meshes[mi].ivert);
+        const uint32_t vb = ldg(&meshes[mi].vert_base);
+        auto colour = [&](uint32_t gv, float iw) {
+            const float4 *p = reinterpret_cast<const float4 *>(iv) + (size_t)(gv - vb) * 4;
+            const float4 c = ldg(p + 2), d = ldg(p + 3);
+            return make_float4(c.w * iw, d.x * iw, d.y * iw, 0.f);
+        };
+        const float4 a03 = colour(q3.x, a00.z), a13 = colour(q3.y, a10.z), a23 = colour(q3.z, a20.z);
         stg(rgba + pid, (uint32_t)(pack_rgba(RAST_FINTERP(a03.x, a13.x, a23.x), RAST_FINTERP(a03.y, a13.y, a23.y),
                                              RAST_FINTERP(a03.z, a13.z, a23.z))));
         return;

@@ -55,7 +55,7 @@ struct LaunchPlan {
     DBuf d_block_mesh, d_tables, d_stage;
     DBuf d_alive_part;  // culling survivors per setup block
     // view-independent attributes of the scenes two or more jobs draw (views):
-    // 48 bytes per vertex of each such scene, at sattr_off[job] (SIZE_MAX: none)
+    // 32 bytes per vertex of each such scene, at sattr_off[job] (SIZE_MAX: none)
     DBuf d_sattr;
     std::vector<size_t> sattr_off;
     // Two pinned staging blocks (JobDesc[njobs] then DevMesh[nmeshes]) used in

@@ -250,7 +250,8 @@ int rast_frame_buffers(rast_renderer *r, void **vis, void **depth, void **rgba);
  * which = RAST_DEBUG_*; copies at most `bytes` and stores the size. */
 enum {
     RAST_DEBUG_VERTEX_POS = 0,  /* float4 per vertex (ndc x, y, z, clip w), all meshes */
-    RAST_DEBUG_VERTEX_ATTR = 1, /* 16 floats per vertex (each attribute times 1/w); RAST_E_STATE after a
+    RAST_DEBUG_VERTEX_ATTR = 1, /* 12 floats per vertex, each attribute times 1/w: u v 1/w n.x | n.y n.z t.x t.y
+                                   | t.z vd.xyz (the vertex colour is applied at shading); RAST_E_STATE after a
                                    batch in which other views drew this renderer's scene (their
                                    attributes are stored split, the view-independent part once) */
     RAST_DEBUG_TRI_RECORDS = 2, /* 16 words per triangle (valid for survivors): sx0 sy0 sx1 sy1 sx2 sy2 z0 area

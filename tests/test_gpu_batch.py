@@ -121,6 +121,10 @@ def batch_jobs(k, n_jobs):
     rs, jobs = [], []
     shared = R.Renderer(256, 160)
     shared.add_scene(S.config1().scene)
+    # and a mesh without maps, in random vertex colours (the views' colour path)
+    coloured = random_triangle_scene(77, 60).meshes[0]
+    coloured.verts[:, 11:14] = np.random.default_rng(77).uniform(0, 1, (coloured.verts.shape[0], 3)).astype(np.float32)
+    shared.add_mesh(coloured)
     rs.append(shared)
     for j in range(n_jobs):
         W = int(S.uniform(S.SEED_BASE + 0x400, 2 * j, 48, 400))

@@ -80,14 +80,15 @@ def test_vertex_outputs_bitwise():
         r.render(w.camera, w.opts)
         r.sync()
         pos = r.debug_read(R.DEBUG_VERTEX_POS, np.float32).reshape(-1, 4)
-        attr = r.debug_read(R.DEBUG_VERTEX_ATTR, np.float32).reshape(-1, 16)
+        attr = r.debug_read(R.DEBUG_VERTEX_ATTR, np.float32).reshape(-1, 12)
     opos, oattr = O.vertex_transform(w.scene, w.camera, w.width, w.height)
     assert_bitwise("pos4", pos, opos)
-    # GPU record: every attribute times 1/w: u v 1/w n.x | n.y n.z t.x t.y | t.z vd.xyz | c.rgb 0;
+    # GPU record: every attribute times 1/w: u v 1/w n.x | n.y n.z t.x t.y | t.z vd.xyz (the colour
+    # times 1/w is formed at shading: test_vertex_colour_path_without_diffuse_map);
     # oracle: uv n tan vd c 1/w (the product is one binary32 multiply on both sides)
-    gpu_in_oracle_order = attr[:, [0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 2]]
+    gpu_in_oracle_order = attr[:, [0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 2]]
     inv_w = oattr[:, 14:15].astype(np.float32)
-    want = np.concatenate([(oattr[:, :14].astype(np.float32) * inv_w).astype(np.float32), inv_w], axis=1)
+    want = np.concatenate([(oattr[:, :11].astype(np.float32) * inv_w).astype(np.float32), inv_w], axis=1)
     assert_bitwise("attr", np.ascontiguousarray(gpu_in_oracle_order), np.ascontiguousarray(want))
 
 
@@ -279,10 +280,16 @@ def test_texture_paths_for_map_sizes_and_mixed_meshes(sample):
 
 
 def test_vertex_colour_path_without_diffuse_map():
-    m = S.uv_sphere(48, 24)
+    """Random vertex colours on two meshes without maps; the second sphere
+    (in front of the first) has its vertices after the first's, so its
+    colours are found through its own vertex base."""
     rng = np.random.default_rng(2)
-    m.verts[:, 11:14] = rng.uniform(0, 1, (m.verts.shape[0], 3)).astype(np.float32)
-    check_frame(S.Scene([m]), S.Camera(), S.Opts(), 320, 240)
+    a = S.uv_sphere(48, 24)
+    a.verts[:, 11:14] = rng.uniform(0, 1, (a.verts.shape[0], 3)).astype(np.float32)
+    b = S.uv_sphere(24, 12)
+    b.verts[:, 11:14] = rng.uniform(0, 1, (b.verts.shape[0], 3)).astype(np.float32)
+    b.world = S.world_matrix((0.6, 0.4, 1.3), (0.2, 0.1, 0.0), 0.35)
+    check_frame(S.Scene([a, b]), S.Camera(), S.Opts(), 320, 240)
 
 
 def test_triangle_strip_topology():
