@@ -296,7 +296,7 @@ extern "C" int rast_renderer_create_view(rast_renderer *owner, uint32_t width, u
 
 static void free_mesh(rast_mesh *m)
 {
-    if (m->vert16) (void)hipFree(m->vert16);
+    if (m->ivert14) (void)hipFree(m->ivert14);
     if (m->idx) (void)hipFree(m->idx);
     delete m;
 }
@@ -305,9 +305,9 @@ static void free_mesh(rast_mesh *m)
 // stays allocated until rast_mesh_destroy.
 static void orphan_mesh(rast_mesh *m)
 {
-    if (m->vert16) (void)hipFree(m->vert16);
+    if (m->ivert14) (void)hipFree(m->ivert14);
     if (m->idx) (void)hipFree(m->idx);
-    m->vert16 = nullptr;
+    m->ivert14 = nullptr;
     m->idx = nullptr;
     for (auto &t : m->tex) t = nullptr;
     m->texq = nullptr;
@@ -421,10 +421,12 @@ extern "C" int rast_mesh_create(rast_renderer *r, const rast_ivertex *verts, uin
             return rast_set_error(RAST_E_INVALID_ARG, "index %u at position %u out of range (num_verts %u)", indices[i], i,
                         num_verts);
     HIP_TRY(hipSetDevice(r->device));
-    std::vector<float> v16((size_t)num_verts * 16, 0.f);
+    static_assert(sizeof(rast_ivertex) == 14 * sizeof(float), "IVertex: 14 floats");
+    // as given, plus 16 bytes: the vertex pass reads whole 16-byte words
+    std::vector<float> v14((size_t)num_verts * 14 + 4, 0.f);
     float attr_max = 0.f;
     for (uint32_t i = 0; i < num_verts; ++i) {
-        memcpy(&v16[(size_t)i * 16], &verts[i], sizeof(rast_ivertex));
+        memcpy(&v14[(size_t)i * 14], &verts[i], sizeof(rast_ivertex));
         for (int k = 0; k < 3; ++k) {
             const float a = std::fabs(verts[i].n[k]), b = std::fabs(verts[i].tan[k]);
             attr_max = std::max(attr_max, std::max(a, b));
@@ -440,9 +442,9 @@ extern "C" int rast_mesh_create(rast_renderer *r, const rast_ivertex *verts, uin
     m->topology = topology;
     m->attr_max = attr_max;
     for (int i = 0; i < 16; ++i) m->world[i] = world ? world[i] : ((i % 5 == 0) ? 1.f : 0.f);
-    hipError_t e = hipMalloc((void **)&m->vert16, v16.size() * sizeof(float));
+    hipError_t e = hipMalloc((void **)&m->ivert14, v14.size() * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&m->idx, (size_t)num_indices * 4);
-    if (e == hipSuccess) e = hipMemcpy(m->vert16, v16.data(), v16.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->ivert14, v14.data(), v14.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->idx, indices, (size_t)num_indices * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_mesh(m);
@@ -1083,7 +1085,7 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
             const rast_mesh *m = meshes[i];
             DevMesh &d = dm[me];
             memset(&d, 0, sizeof d);
-            d.ivert = m->vert16;
+            d.ivert = m->ivert14;
             d.idx = m->idx;
             d.num_verts = m->nv;
             d.num_tris = m->ntri;

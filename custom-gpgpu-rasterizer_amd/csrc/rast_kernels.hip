@@ -483,24 +483,36 @@ __device__ __forceinline__ void vertex_block(const DevMesh *__restrict__ meshes,
     // scene): the vertex is read once and transformed for each of them
     const uint2 gr = groups[block_group[blk]];
     const DevMesh &M0 = meshes[entries[gr.x]];
-    static_assert(kVertexChunk <= 256 && 256 % kVertexChunk == 0, "a chunk's quads are whole rounds of the block");
+    static_assert(kVertexChunk <= 256 && 256 % kVertexChunk == 0 && kVertexChunk % 4 == 0,
+                  "a chunk's outputs are whole rounds of the block and its input starts on a 16-byte word");
     const uint32_t vbase = (blk - M0.vs_block0) * kVertexChunk;
     const uint32_t nv = M0.num_verts - vbase < kVertexChunk ? M0.num_verts - vbase : kVertexChunk;  // vertices of this block
     const uint32_t tid = threadIdx.x;
-    const float4 *src = reinterpret_cast<const float4 *>(M0.ivert) + (size_t)vbase * 4;
+    // the block's 56-byte input vertices are one contiguous run of 16-byte
+    // words (its start is a multiple of 4 vertices; the last word may reach
+    // into the mesh buffer's 16 bytes of padding), copied to LDS as they are
+    const float4 *src = reinterpret_cast<const float4 *>(M0.ivert + (size_t)vbase * 14);
+    const uint32_t nq = (nv * 14u + 3u) / 4u;
 #pragma unroll
-    for (int k = 0; k < (int)(kVertexChunk * 4 / 256); ++k) {
-        const uint32_t w = (uint32_t)k * 256u + tid;  // quad w of the block: vertex w / 4, part w % 4
-        if (w < nv * 4u) rows[at(w >> 2, w & 3u)] = ldg(src + w);
+    for (int k = 0; k < (int)((kVertexChunk * 14 / 4 + 255) / 256); ++k) {
+        const uint32_t w = (uint32_t)k * 256u + tid;
+        if (w < nq) rows[w] = ldg(src + w);
     }
     __syncthreads();
     const bool live = tid < nv;
-    const float4 q0 = rows[at(tid, 0)], q1 = rows[at(tid, 1)], q2 = rows[at(tid, 2)];  // q3: colour, read at shading
+    // the lane's vertex: 7 float2 at float 14 * tid (px py pz u | v nx ny nz | tx ty tz; the colour,
+    // floats 11-13, is read at shading)
+    float2 f[6] = {};
+    if (tid < kVertexChunk) {
+        const float2 *vr = reinterpret_cast<const float2 *>(rows) + tid * 7u;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) f[k] = vr[k];
+    }
     const uint32_t v = vbase + tid;
-    const float px = q0.x, py = q0.y, pz = q0.z;
-    const float u = q0.w, vv = q1.x;
-    const float nx = q1.y, ny = q1.z, nz = q1.w;
-    const float tx = q2.x, ty = q2.y, tz = q2.z;
+    const float px = f[0].x, py = f[0].y, pz = f[1].x;
+    const float u = f[1].y, vv = f[2].x;
+    const float nx = f[2].y, ny = f[3].x, nz = f[3].y;
+    const float tx = f[4].x, ty = f[4].y, tz = f[5].x;
     if (jobs[M0.job].sattr) {
         // views of one scene (a launch's jobs drawing this mesh share sattr):
         // the view-independent attributes once, then per view the position
@@ -2173,9 +2185,8 @@ __device__ __forceinline__ void shade_body(const DevMesh *__restrict__ meshes, c
         const float *iv = ldg(&meshes[mi].ivert);
         const uint32_t vb = ldg(&meshes[mi].vert_base);
         auto colour = [&](uint32_t gv, float iw) {
-            const float4 *p = reinterpret_cast<const float4 *>(iv) + (size_t)(gv - vb) * 4;
-            const float4 c = ldg(p + 2), d = ldg(p + 3);
-            return make_float4(c.w * iw, d.x * iw, d.y * iw, 0.f);
+            const float *p = iv + (size_t)(gv - vb) * 14;
+            return make_float4(ldg(p + 11) * iw, ldg(p + 12) * iw, ldg(p + 13) * iw, 0.f);
         };
         const float4 a03 = colour(q3.x, a00.z), a13 = colour(q3.y, a10.z), a23 = colour(q3.z, a20.z);
         stg(rgba + pid, (uint32_t)(pack_rgba(RAST_FINTERP(a03.x, a13.x, a23.x), RAST_FINTERP(a03.y, a13.y, a23.y),

@@ -93,7 +93,7 @@ struct TexQuad {
 
 struct rast_mesh {
     rast_renderer *r = nullptr;
-    float *vert16 = nullptr;  // 16 floats per vertex (IVertex + 2 pad floats)
+    float *ivert14 = nullptr;  // 14 floats per vertex (the 56-byte IVertex as given) + 16 bytes of padding
     uint32_t *idx = nullptr;
     uint32_t nv = 0, ni = 0, ntri = 0;
     int topology = 0;
