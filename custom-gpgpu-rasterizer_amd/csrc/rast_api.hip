@@ -343,7 +343,7 @@ extern "C" int rast_renderer_destroy(rast_renderer *r)
     DBuf *bufs[] = { &r->pos4, &r->attr, &r->trirec,
                      &r->alive, &r->survivors, &r->tile_count[0], &r->tile_count[1], &r->tile_off, &r->tile_cap,
                      &r->tile_pool, &r->tile_cursor, &r->tile_min_depth[0], &r->tile_min_depth[1], &r->keys,
-                     &r->queue, &r->pool, &r->big, &r->ovf, &r->vis, &r->depth,
+                     &r->queue, &r->pool, &r->big, &r->bigp, &r->ovf, &r->vis, &r->depth,
                      &r->rgba };
     for (DBuf *b : bufs) dfree(*b);
     if (r->h_counters) (void)hipHostFree(r->h_counters);
@@ -640,6 +640,7 @@ static int ensure_frame_storage(rast_renderer *r, uint32_t ntiles)
     HIP_TRY(grow(r, r->queue, (size_t)r->queue_cap * 4, &synced));
     HIP_TRY(grow(r, r->pool, (size_t)r->pool_cap * 4, &synced));
     HIP_TRY(grow(r, r->big, (size_t)r->big_cap * 4, &synced));
+    HIP_TRY(grow(r, r->bigp, (size_t)r->big_cap * 16, &synced));
     return RAST_OK;
 }
 
@@ -1060,6 +1061,7 @@ static int stage_launch(LaunchPlan &P, rast_renderer *const *rs, uint32_t n)
         J.ovf = (uint2 *)r->ovf.p;
         J.survivors = (uint32_t *)r->survivors.p;
         J.big = (uint32_t *)r->big.p;
+        J.big_plane = (float4 *)r->bigp.p;
         J.sattr = P.sattr_off[j] != SIZE_MAX ? (float4 *)((char *)P.d_sattr.p + P.sattr_off[j]) : nullptr;
         r->last_shared_attr = J.sattr != nullptr;
         J.counters = (FrameCounters *)((char *)P.d_stage.p + stage_counters_offset(n) + (size_t)j * sizeof(FrameCounters));

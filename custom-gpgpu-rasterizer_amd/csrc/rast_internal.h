@@ -121,7 +121,7 @@ struct FrameCounters {
 // renderer that owns it.  Uploaded with the mesh table for every launch, so
 // a captured launch graph replays with new cameras, options and buffer
 // parities; only the grid shapes are fixed by the graph.
-struct JobDesc {
+struct alignas(16) JobDesc {  // 16-byte multiple: staged in 16-byte words
     FrameParams fp;
     uint32_t block0[ST_COUNT];   // first block of this job in each stage's grid
     uint32_t nblocks[ST_COUNT];  // its block count there (0: not part of that stage)
@@ -159,6 +159,9 @@ struct JobDesc {
     uint2 *ovf;                  // single pass: (tile, id) entries beyond capacity
     uint32_t *survivors;
     uint32_t *big;
+    // per screen-list slot: the triangle's depth plane over its pixel range,
+    // formed once by the setup pass (screen_plane); read by every resolve tile
+    float4 *big_plane;
     FrameCounters *counters;     // num_alive stays 0 here: see alive_part
     FrameCounters *host_counters;  // pinned host copy, written at the end of the launch
     // culling survivors per setup block (launch-wide, indexed by the setup

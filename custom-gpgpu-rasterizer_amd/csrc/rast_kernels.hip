@@ -431,6 +431,58 @@ __device__ __forceinline__ void load_tri(const uint32_t *__restrict__ rec, uint3
     t.inv = 0.f;
 }
 
+// The depth plane of a screen-list triangle over its pixel range (the
+// clipped bbox [x0, x1) x [y0, y1)), in double, formed once per frame by the
+// setup pass.  For edge j (vertex (ox, oy), float edge vector (ex, ey)) the
+// computed edge value at pixel p is within
+//   eps_j = 3.0001 u (Dx |ey| + Dy |ex|)
+// of W_j(p) = (px - ox) ey - (py - oy) ex (u = 2^-24; Dx, Dy the largest
+// |px - ox|, |py - oy| over the range's pixels): two roundings in each
+// product term, one in the difference.  With a_j = fl(w_j / area) and the
+// rounded three-term sum, the computed depth is within
+//   B = sum_j |z_j| (eps_j + 4.0001 u (|W_j|max + eps_j)) / area
+// of the affine L(p) = sum_j z_j W_j(p) / area.  Returns (L(x0, y0), dL/dx,
+// dL/dy, M) in float, M >= 1.01 B + 1e-12 |L terms| (the double rounding of
+// L), rounded up; M = +inf where no plane applies (non-finite or huge
+// values, or M >= 0.2).  Every pixel the per-pixel test admits lies in the
+// range, so the bound holds for each tile's share of it.
+__device__ __forceinline__ float4 screen_plane(const TriLds &t)
+{
+    const float4 none = make_float4(0.f, 0.f, 0.f, __int_as_float(0x7f800000));
+    const float vmax = fmaxf(fmaxf(fmaxf(fmaxf(fabsf(t.sx0), fabsf(t.sy0)), fmaxf(fabsf(t.sx1), fabsf(t.sy1))),
+                                   fmaxf(fmaxf(fabsf(t.sx2), fabsf(t.sy2)), fmaxf(fabsf(t.ax), fabsf(t.ay)))),
+                             fmaxf(fmaxf(fabsf(t.bx), fabsf(t.by)), fmaxf(fabsf(t.cx), fabsf(t.cy))));
+    if (!(vmax < 1.0e18f)) return none;
+    const double A = (double)t.area;
+    const double zmax = fmax(fmax(fabs((double)t.z0), fabs((double)t.z1)), fabs((double)t.z2));
+    if (!(A > 0.0 && zmax < 1.0e6)) return none;
+    // edge j runs from vertex j along (ex, ey); its function weights depth zz[j]
+    const double u = 1.0 / 16777216.0;
+    const double ox[3] = { t.sx0, t.sx1, t.sx2 }, oy[3] = { t.sy0, t.sy1, t.sy2 };
+    const double ex[3] = { t.ax, t.bx, t.cx }, ey[3] = { t.ay, t.by, t.cy };
+    const double zz[3] = { t.z2, t.z0, t.z1 };
+    const double xa = (double)(t.xr & 0xFFFFu), xb = (double)((t.yr & 0xFFFFu) - 1u);
+    const double ya = (double)(t.xr >> 16), yb = (double)((t.yr >> 16) - 1u);
+    double L0 = 0.0, Lx = 0.0, Ly = 0.0, B = 0.0, Lmag = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        const double dxa = xa - ox[j], dxb = xb - ox[j], dya = ya - oy[j], dyb = yb - oy[j];
+        const double Dx = fmax(fabs(dxa), fabs(dxb)), Dy = fmax(fabs(dya), fabs(dyb));
+        const double w00 = dxa * ey[j] - dya * ex[j], w10 = dxb * ey[j] - dya * ex[j];
+        const double w01 = dxa * ey[j] - dyb * ex[j], w11 = dxb * ey[j] - dyb * ex[j];
+        const double Wabs = fmax(fmax(fabs(w00), fabs(w10)), fmax(fabs(w01), fabs(w11)));
+        const double eps = 3.0001 * u * (Dx * fabs(ey[j]) + Dy * fabs(ex[j]));
+        B += fabs(zz[j]) * (eps + 4.0001 * u * (Wabs + eps));
+        Lmag += fabs(zz[j]) * Wabs;
+        L0 += zz[j] * w00;
+        Lx += zz[j] * ey[j];
+        Ly -= zz[j] * ex[j];
+    }
+    const double rA = 1.0 / A;
+    const double M = 1.01 * B * rA + 1.0e-12 * Lmag * rA + 1.0e-30;
+    if (!(M < 0.2)) return none;
+    return make_float4((float)(L0 * rA), (float)(Lx * rA), (float)(Ly * rA), (float)(M * (1.0 + 1.0e-6)));
+}
+
 // ---------------------------------------------------------------- kernels
 
 // Block blk of the init stage: resets 256 tiles of its job's per-tile
@@ -905,6 +957,14 @@ __global__ __launch_bounds__(256) void k_setup(const DevMesh *__restrict__ meshe
             if (++tx == tx0 + w) { tx = tx0; ++ty; }
         }
     }
+    // a screen-list triangle's depth plane over its pixel range, once per
+    // frame for every resolve tile (from the record this lane wrote, at the
+    // kernel's end, where little else is live)
+    if (is_big && bslot < J.big_cap) {
+        TriLds bt;
+        load_tri(trirec, gid, bt);
+        stg(J.big_plane + bslot, screen_plane(bt));
+    }
 }
 
 // Exclusive scan of a job's exact per-tile counts by one 1024-thread
@@ -1116,7 +1176,8 @@ __global__ __launch_bounds__(256) void k_fallback_fill(const JobDesc *__restrict
 
 // Bounds of a screen-list triangle t over the tile at (X0, Y0), used to skip
 // the tile or single pixels before the exact fragment test.  tmax is the
-// largest depth the tile's pixels hold; depths only decrease from there.
+// largest depth the pixels in question hold; depths only decrease from there.
+// sp is the triangle's screen_plane.
 //
 // Tile tests in float, exact.  Every operation of edges() and depth_of() is
 // a correctly rounded IEEE operation and rounding is monotone, so a computed
@@ -1129,26 +1190,22 @@ __global__ __launch_bounds__(256) void k_fallback_fill(const JobDesc *__restrict
 // tile.  These need finite values below 1e18 in magnitude (no overflow, so
 // no NaN); a NaN bound rejects nothing.
 //
-// Per-pixel plane in double.  For edge j (vertex (ox, oy), float edge vector
-// (ex, ey)) the computed edge value at pixel p is within
-//   eps_j = 3.0001 u (Dx |ey| + Dy |ex|)
-// of W_j(p) = (px - ox) ey - (py - oy) ex (u = 2^-24; Dx, Dy the largest
-// |px - ox|, |py - oy| over the tile's pixels): two roundings in each
-// product term, one in the difference.  With a_j = fl(w_j / area) and the
-// rounded three-term sum, the computed depth is within
-//   B = sum_j |z_j| (eps_j + 4.0001 u (|W_j|max + eps_j)) / area
-// of the affine L(p) = sum_j z_j W_j(p) / area, which is evaluated in double
-// at the tile corner.  M adds the float evaluation of the returned plane and
-// the double rounding of L; the plane is used while M < 0.25.
+// Per-pixel plane.  The stored plane, evaluated at the tile corner in float,
+// differs from the double plane L by its storage (u of each of L, dL/dx,
+// dL/dy: u S at the corner, u span over the tile; S = |L| + |dL/dx dX| +
+// |dL/dy dY| with dX, dY the tile corner's offset in the range) and by this
+// evaluation (3u S); M adds those, 4u (|L0| + span) for the corner sums
+// below and, as the returned plane is evaluated per pixel in float, 8u (|L0|
+// + span + 1); the plane is used while M < 0.25.
 //
 // Returns (x, y, z, w) with lo(p) = x + y (px - X0) + z (py - Y0) <= z(p) <=
 // lo(p) + w for every fragment of the tile: the plane (x = L(X0, Y0) - M,
-// y = dL/dx, z = dL/dy, w = 2 M) where it applies, else the constant float
-// lower bound (w = 1e30).  A fragment loses where lo(p) > the pixel's depth
-// and is out of range where lo(p) + w < 0.  x = +3e38: no pixel of the tile
-// can take a fragment of t (outside an edge, depth out of [0, 1], or behind
-// every pixel).
-__device__ __forceinline__ float4 tile_bound(const TriLds &t, int X0, int Y0, int T, float tmax)
+// y = dL/dx, z = dL/dy, w = 2 M + 8u (|L0| + span + 1)) where it applies,
+// else the constant float lower bound (w = 1e30).  A fragment loses where
+// lo(p) > the pixel's depth and is out of range where lo(p) + w < 0.
+// x = +3e38: no pixel of the tile can take a fragment of t (outside an
+// edge, depth out of [0, 1], or behind every pixel).
+__device__ __forceinline__ float4 tile_bound(const TriLds &t, const float4 sp, int X0, int Y0, int T, float tmax)
 {
     const float4 skip = make_float4(3.0e38f, 0.f, 0.f, 0.f);
     const float4 none = make_float4(-1.0e30f, 0.f, 0.f, 1.0e30f);
@@ -1167,40 +1224,22 @@ __device__ __forceinline__ float4 tile_bound(const TriLds &t, int X0, int Y0, in
         lo2 = fminf(lo2, w2); hi2 = fmaxf(hi2, w2);
     }
     if (hi0 < 0.f || hi1 < 0.f || hi2 < 0.f) return skip;
-    // the plane (double)
-    const double A = (double)t.area;
-    const double zmax = fmax(fmax(fabs((double)t.z0), fabs((double)t.z1)), fabs((double)t.z2));
-    if (A > 0.0 && zmax < 1.0e6) {
-        // edge j runs from vertex j along (ex, ey); its function weights depth zz[j]
-        const double u = 1.0 / 16777216.0;
-        const double ox[3] = { t.sx0, t.sx1, t.sx2 }, oy[3] = { t.sy0, t.sy1, t.sy2 };
-        const double ex[3] = { t.ax, t.bx, t.cx }, ey[3] = { t.ay, t.by, t.cy };
-        const double zz[3] = { t.z2, t.z0, t.z1 };
-        const double xa = (double)X0, xb = (double)(X0 + T - 1), ya = (double)Y0, yb = (double)(Y0 + T - 1);
-        double L0 = 0.0, Lx = 0.0, Ly = 0.0, B = 0.0, Lmag = 0.0;
-        for (int j = 0; j < 3; ++j) {
-            const double dxa = xa - ox[j], dxb = xb - ox[j], dya = ya - oy[j], dyb = yb - oy[j];
-            const double Dx = fmax(fabs(dxa), fabs(dxb)), Dy = fmax(fabs(dya), fabs(dyb));
-            const double w00 = dxa * ey[j] - dya * ex[j], w10 = dxb * ey[j] - dya * ex[j];
-            const double w01 = dxa * ey[j] - dyb * ex[j], w11 = dxb * ey[j] - dyb * ex[j];
-            const double Wabs = fmax(fmax(fabs(w00), fabs(w10)), fmax(fabs(w01), fabs(w11)));
-            const double eps = 3.0001 * u * (Dx * fabs(ey[j]) + Dy * fabs(ex[j]));
-            B += fabs(zz[j]) * (eps + 4.0001 * u * (Wabs + eps));
-            Lmag += fabs(zz[j]) * Wabs;
-            L0 += zz[j] * w00;
-            Lx += zz[j] * ey[j];
-            Ly -= zz[j] * ex[j];
-        }
-        const double rA = 1.0 / A;
-        B *= rA; Lmag *= rA; L0 *= rA; Lx *= rA; Ly *= rA;
-        const double span = (fabs(Lx) + fabs(Ly)) * (double)(T - 1);
-        const double M = 1.01 * B + 8.0 * u * (fabs(L0) + span + 1.0) + 1.0e-12 * Lmag + 1.0e-30;
-        if (M < 0.25) {
+    if (sp.w < 0.2f) {  // the triangle has a plane (+inf: none)
+        const float u = 1.0f / 16777216.0f;
+        const float dX = (float)(X0 - (int)(t.xr & 0xFFFFu)), dY = (float)(Y0 - (int)(t.xr >> 16));
+        const float px = sp.y * dX, py = sp.z * dY;
+        const float L0 = (sp.x + px) + py;
+        const float S = (fabsf(sp.x) + fabsf(px)) + fabsf(py);
+        const float Tm = (float)(T - 1);
+        const float span = (fabsf(sp.y) + fabsf(sp.z)) * Tm;
+        const float aL = fabsf(L0);
+        const float M = (sp.w + u * (4.02f * S + 12.0f * aL + 13.0f * span + 8.0f)) * 1.0001f + 1.0e-30f;
+        if (M < 0.25f) {
             // whole-tile tests at the tile's corners (L is affine)
-            const double Lmin = L0 + fmin(Lx, 0.0) * (double)(T - 1) + fmin(Ly, 0.0) * (double)(T - 1);
-            const double Lmax = L0 + fmax(Lx, 0.0) * (double)(T - 1) + fmax(Ly, 0.0) * (double)(T - 1);
-            if (Lmin - M > 1.0 || Lmax + M < 0.0 || Lmin - M > (double)tmax) return skip;
-            return make_float4((float)(L0 - M), (float)Lx, (float)Ly, (float)(2.0 * M + 8.0 * u * (fabs(L0) + span + 1.0)));
+            const float Lmin = (L0 + fminf(sp.y, 0.f) * Tm) + fminf(sp.z, 0.f) * Tm;
+            const float Lmax = (L0 + fmaxf(sp.y, 0.f) * Tm) + fmaxf(sp.z, 0.f) * Tm;
+            if (Lmin - M > 1.0f || Lmax + M < 0.0f || Lmin - M > tmax) return skip;
+            return make_float4(L0 - M, sp.y, sp.z, 2.0f * M + 8.0f * u * (aL + span + 1.0f));
         }
     }
     // no plane: the exact float depth range over the tile
@@ -1290,6 +1329,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
     const uint32_t pool_cap = J.pool_cap;
     const uint2 *__restrict__ ovf = J.ovf;
     const uint32_t *__restrict__ big = J.big;
+    const float4 *__restrict__ big_plane = J.big_plane;
     const uint32_t big_cap = J.big_cap;
     FrameCounters *__restrict__ ctr = J.counters;
     const uint32_t *__restrict__ trirec = J.trirec;
@@ -1666,7 +1706,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     const int x0 = (int)(t.xr & 0xFFFFu), y0 = (int)(t.xr >> 16);
                     const int x1 = (int)(t.yr & 0xFFFFu), y1 = (int)(t.yr >> 16);
                     if (x1 > X0 && x0 < X0 + T && y1 > Y0 && y0 < Y0 + T) {
-                        const float4 pl = tile_bound(t, X0, Y0, T, tmax);
+                        const float4 pl = tile_bound(t, ldg(big_plane + i), X0, Y0, T, tmax);
                         keep = !(pl.x > kSkipBound);
                         lb = pl.x + fminf(pl.y, 0.f) * (float)(T - 1) + fminf(pl.z, 0.f) * (float)(T - 1);
                     }
@@ -1714,7 +1754,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     const int x0 = (int)(t.xr & 0xFFFFu), y0 = (int)(t.xr >> 16);
                     const int x1 = (int)(t.yr & 0xFFFFu), y1 = (int)(t.yr >> 16);
                     if (x1 > X0 && x0 < X0 + T && y1 > Y0 && y0 < Y0 + T) {
-                        pl = tile_bound(t, X0, Y0, T, own ? wmax : tmax);
+                        pl = tile_bound(t, ldg(big_plane + li), X0, Y0, T, own ? wmax : tmax);
                         keep = !(pl.x > kSkipBound);
                     }
                 }

@@ -126,7 +126,7 @@ struct rast_renderer {
     // frame buffers
     DBuf pos4, attr, trirec, alive, survivors;
     DBuf tile_count[2], tile_off, tile_cap, tile_pool, tile_cursor, tile_min_depth[2];
-    DBuf queue, pool, big, ovf;
+    DBuf queue, pool, big, bigp, ovf;
     uint32_t queue_cap = 0, pool_cap = 0, big_cap = 0, ovf_cap = 0;
     DBuf vis, depth, rgba, keys;
     bool keys_dirty = true;  // key buffer must be reset before the next frame
