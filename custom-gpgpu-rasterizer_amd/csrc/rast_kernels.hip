@@ -1344,9 +1344,14 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
     static_assert(NPIX % NT == 0, "tile pixels must be a multiple of the workgroup size");
     static_assert(T <= 256 && NT <= 256, "ring entries hold the staged triangle and the pixel in 8 bits each");
     __shared__ uint64_t key[NPIX];
-    __shared__ TriLds stage[NT];
-    __shared__ float4 bplane[NT];  // screen-list triangles: conservative depth plane over the tile
-    __shared__ uint32_t bid[NT];   // and their IDs
+    // the queue's staged triangles (56 bytes each: the edge vectors are
+    // formed again where they are used, as in the setup pass); once the
+    // queue is resolved the same bytes hold the screen-list triangles' depth
+    // planes over the tile and their IDs.  Five workgroups fit a CU.
+    __shared__ __align__(16) MicroLds stage[NT];
+    static_assert(sizeof(stage) >= NT * (sizeof(float4) + sizeof(uint32_t)), "screen-list entries fit the staging area");
+    float4 *const bplane = reinterpret_cast<float4 *>(stage);
+    uint32_t *const bid = reinterpret_cast<uint32_t *>(bplane + NT);
     __shared__ uint32_t off[NT];
     __shared__ uint32_t wlist[NT];
     __shared__ uint32_t rring[NW * kFragRing];  // per wave: covered items waiting for their depth
@@ -1481,7 +1486,13 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     }
                 }
             }
-            stage[tid] = t;
+            {
+                MicroLds m;
+                m.sx0 = t.sx0; m.sy0 = t.sy0; m.sx1 = t.sx1; m.sy1 = t.sy1; m.sx2 = t.sx2; m.sy2 = t.sy2;
+                m.z0 = t.z0; m.z1 = t.z1; m.z2 = t.z2; m.area = t.area;
+                m.xr = t.xr; m.yr = t.yr; m.id = t.id; m.inv = t.inv;
+                stage[tid] = m;
+            }
             // block-wide exclusive scan of the item counts
             uint32_t incl = area;
             for (int o = 1; o < 64; o <<= 1) {
@@ -1522,7 +1533,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     if (lane < n) {
                         const uint32_t e = ring[(head + lane) & (kFragRing - 1u)];
-                        const TriLds &mt = stage[(uint16_t)(e & 0xFFu)];
+                        const MicroLds &mt = stage[(uint16_t)(e & 0xFFu)];
                         const int lx = (int)((e >> 8) & 0xFFu), ly = (int)(e >> 16);
                         float w0, w1, w2, z;
                         edges(mt, (float)(X0 + lx), (float)(Y0 + ly), w0, w1, w2);
@@ -1566,7 +1577,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                         uint32_t entry = 0;
                         if (kk < wend) {
                             const uint32_t j = olist[ocum[wi] + (uint32_t)__popcll(m & ((2ull << lane) - 1ull)) - 1u];
-                            const TriLds &mt = stage[(uint16_t)j];
+                            const MicroLds &mt = stage[(uint16_t)j];
                             int px, py;
                             item_pixel(mt, kk - off[j], px, py);
                             float w0, w1, w2;
@@ -1590,7 +1601,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
                 const int wy0 = Y0 + (int)(wave * 64) / T;
                 const int wy1 = Y0 + ((int)(wave * 64) + 63) / T + 1;
                 for (uint32_t q2 = 0; q2 < nwhole; ++q2) {
-                    const TriLds &st = stage[wlist[q2]];
+                    const MicroLds &st = stage[wlist[q2]];
                     const int x0 = (int)(st.xr & 0xFFFFu), x1 = (int)(st.xr >> 16);
                     const int y0 = (int)(st.yr & 0xFFFFu), y1 = (int)(st.yr >> 16);
                     if (PPT == 1 && (y1 <= wy0 || y0 >= wy1)) continue;
@@ -1850,7 +1861,7 @@ __device__ __forceinline__ void resolve_tile(const JobDesc &J, uint32_t tile, co
 // workgroups over the 8 XCDs, so neighbouring tiles run on different XCDs at
 // once (giving each XCD a contiguous band instead measured 30% slower).
 template <int T, int NT, int STAGE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_resolve(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_resolve(
     const JobDesc *__restrict__ jobs, const uint16_t *__restrict__ table)
 {
     uint32_t local;
